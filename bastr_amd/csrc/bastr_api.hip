@@ -1,0 +1,615 @@
+// bastr_api.hip -- C-ABI of include/bastr.h over the 2-D HIP step kernels.
+//
+// One handle = one periodic field resident in HBM, one HIP stream.  The
+// spectral state is the pair of normalised half spectra u1_hat, u2_hat; the
+// physical field exists only when bastr_get_fields asks for it.  One loop
+// iteration (solution.F90:54-97) is the launch sequence in run_iteration().
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/bastr.h"
+#include "k2d.hpp"
+
+using namespace k2d;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(BASTR_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(call)                                                                    \
+    do {                                                                                   \
+        if ((call) != 0) return fail(BASTR_EINVAL, "launch shape rejected: %s", #call);    \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess) return fail(BASTR_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+__global__ void fill_diag_kernel(bastr_diag* d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        bastr_diag z = {};
+        z.force_factor = z.force_E = z.force_energy = z.force_Fen = __builtin_nan("");
+        d[i] = z;
+    }
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(const A* __restrict__ in, B* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (B)in[i];
+}
+
+void* dalloc(size_t bytes, bool* ok) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) { *ok = false; return nullptr; }
+    if (hipMemset(p, 0, bytes) != hipSuccess) { *ok = false; }
+    return p;
+}
+
+}  // namespace
+
+struct bastr_handle {
+    bastr_params p{};
+    int nd = 2, n0 = 0, n1 = 0, prec = 0, allkmax = 0;
+    double nu = 0;
+    int nh = 0, hp = 0, cut2 = 0;
+    size_t npts = 0, nhalf = 0;
+    size_t esz = 8;  // sizeof(T)
+    hipStream_t stream = nullptr;
+    void* U[2] = {};
+    void* UB[2] = {};
+    void* KA[2] = {};
+    void* W[7] = {};
+    void* NX[2] = {};
+    void* ureal[2] = {};
+    double* dstage = nullptr;  // FP64 host-exchange staging (FP32 mode)
+    void* tw0 = nullptr;
+    void* tw1 = nullptr;
+    double* tw0d = nullptr;    // FP64 twiddles for the c2c entry points in FP32 mode
+    double* tw1d = nullptr;
+    void* cbuf = nullptr;      // complex128 work array of the c2c entry points
+    double* part_bins = nullptr;
+    double* part_tot = nullptr;
+    double* part_vg = nullptr;
+    int nblk_spec = 0, nblk_vg = 0;
+    int* status = nullptr;
+    bastr_diag* ddiag = nullptr;
+    double* dspec = nullptr;
+    int cap = 0;
+    std::vector<double> kn;
+    std::vector<int64_t> counts;
+    int64_t nstep = 0;
+    double time = 0;
+    bool prologue_pending = false, have_fields = false;
+};
+
+namespace {
+
+template <typename T>
+void make_twiddles(int n, std::vector<cx<T>>& tw) {
+    tw.resize(n);
+    for (int m = 0; m < n; ++m) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+        tw[m].x = (T)cosl(a);
+        tw[m].y = (T)(-sinl(a));
+    }
+}
+
+template <typename T>
+int upload_twiddles(int n, void** out) {
+    std::vector<cx<T>> tw;
+    make_twiddles<T>(n, tw);
+    HIPCHK(hipMalloc(out, sizeof(cx<T>) * n));
+    HIPCHK(hipMemcpy(*out, tw.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+Geo<T> geo(const bastr_handle* h) {
+    Geo<T> g;
+    g.n0 = h->n0;
+    g.n1 = h->n1;
+    g.nh = h->nh;
+    g.hp = h->hp;
+    g.cut2 = h->cut2;
+    g.lproject = h->p.lproject;
+    g.nu = h->nu;
+    g.dt = h->p.deltat;
+    g.inv_n = 1.0 / ((double)h->n0 * (double)h->n1);
+    return g;
+}
+
+template <typename T>
+cx<T>* C(void* p) { return (cx<T>*)p; }
+
+// u_hat = dealias(fft2d(u)) from the real fields in ureal (mainloop2D :41-52)
+template <typename T>
+int run_prologue(bastr_handle* h) {
+    const Gate open{h->status, 0, 0};
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.gate = open;
+    r.rin[0] = (const T*)h->ureal[0];
+    r.rin[1] = (const T*)h->ureal[1];
+    r.hout[0] = C<T>(h->NX[0]);
+    r.hout[1] = C<T>(h->NX[1]);
+    r.tw = C<T>(h->tw0);
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
+    ColFwdArgs<T> c{};
+    c.g = geo<T>(h);
+    c.gate = open;
+    c.nx[0] = C<T>(h->NX[0]);
+    c.nx[1] = C<T>(h->NX[1]);
+    c.U[0] = C<T>(h->U[0]);
+    c.U[1] = C<T>(h->U[1]);
+    c.pitch = h->hp;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(launch_col_fwd<T>(h->n1, CF_PROLOGUE, c, h->hp, h->stream));
+    return 0;
+}
+
+// u = ifft2d(u_hat) into ureal
+template <typename T>
+int run_to_physical(bastr_handle* h) {
+    const Gate open{h->status, 0, 0};
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    ColInvArgs<T> c{};
+    c.g = geo<T>(h);
+    c.gate = open;
+    c.in[0] = C<T>(h->U[0]);
+    c.in[1] = C<T>(h->U[1]);
+    c.out[0] = C<T>(h->W[0]);
+    c.out[1] = C<T>(h->W[1]);
+    c.nfield = 2;
+    c.pitch = h->hp;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(launch_col_inv<T>(h->n1, CI_PLAIN, c, h->hp, h->stream));
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.gate = open;
+    r.hin[0] = C<T>(h->W[0]);
+    r.hin[1] = C<T>(h->W[1]);
+    r.rout[0] = (T*)h->ureal[0];
+    r.rout[1] = (T*)h->ureal[1];
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(launch_row<T>(h->n0, R_INV_PACK, r, h->n1, h->stream));
+    return 0;
+}
+
+// One RK3 stage: y-inverse (derivatives), x pass (nonlinear term), y-forward + update.
+template <typename T>
+int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* Uin) {
+    ColInvArgs<T> ci{};
+    ci.g = geo<T>(h);
+    ci.gate = gstep;
+    ci.in[0] = C<T>((void*)Uin[0]);
+    ci.in[1] = C<T>((void*)Uin[1]);
+    for (int q = 0; q < 4; ++q) ci.out[q] = C<T>(h->W[q]);
+    ci.pitch = h->hp;
+    ci.tw = C<T>(h->tw1);
+    if (stage != 0) LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE, ci, h->hp, h->stream));
+    // (stage A's y-inverse ran in the diagnostic phase with the velgrad outputs)
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.gate = gstep;
+    for (int q = 0; q < 4; ++q) r.hin[q] = C<T>(h->W[q]);
+    r.hout[0] = C<T>(h->NX[0]);
+    r.hout[1] = C<T>(h->NX[1]);
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
+    ColFwdArgs<T> cf{};
+    cf.g = geo<T>(h);
+    cf.gate = gstep;
+    cf.nx[0] = C<T>(h->NX[0]);
+    cf.nx[1] = C<T>(h->NX[1]);
+    for (int c = 0; c < 2; ++c) {
+        cf.U[c] = C<T>(h->U[c]);
+        cf.UB[c] = C<T>(h->UB[c]);
+        cf.KA[c] = C<T>(h->KA[c]);
+    }
+    cf.pitch = h->hp;
+    cf.tw = C<T>(h->tw1);
+    LAUNCHCHK(launch_col_fwd<T>(h->n1, stage == 0 ? CF_A : (stage == 1 ? CF_B : CF_C), cf, h->hp, h->stream));
+    return 0;
+}
+
+// One iteration of mainloop2D's body (solution.F90:54-97).
+template <typename T>
+int run_iteration(bastr_handle* h, int it) {
+    const Gate gdiag{h->status, it, 0};
+    const Gate gstep{h->status, it, 1};
+    // spectra_compute2D on u_hat
+    SpecArgs<T> sa{};
+    sa.g = geo<T>(h);
+    sa.gate = gdiag;
+    sa.U[0] = C<T>(h->U[0]);
+    sa.U[1] = C<T>(h->U[1]);
+    sa.allkmax = h->allkmax;
+    sa.part_bins = h->part_bins;
+    sa.part_tot = h->part_tot;
+    LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
+    // velgrad_calculate2D: derivative fields shared with RK stage A
+    ColInvArgs<T> ci{};
+    ci.g = geo<T>(h);
+    ci.gate = gdiag;
+    ci.in[0] = C<T>(h->U[0]);
+    ci.in[1] = C<T>(h->U[1]);
+    for (int q = 0; q < 7; ++q) ci.out[q] = C<T>(h->W[q]);
+    ci.pitch = h->hp;
+    ci.tw = C<T>(h->tw1);
+    LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE_VG, ci, h->hp, h->stream));
+    VgArgs<T> va{};
+    va.g = geo<T>(h);
+    va.gate = gdiag;
+    for (int q = 0; q < 7; ++q) va.w[q] = C<T>(h->W[q]);
+    va.part = h->part_vg;
+    va.tw = C<T>(h->tw0);
+    LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg));
+    FinArgs fa{};
+    fa.gate = gdiag;
+    fa.nblk_spec = h->nblk_spec;
+    fa.nblk_vg = h->nblk_vg;
+    fa.allkmax = h->allkmax;
+    fa.ndims = 2;
+    fa.nu = h->nu;
+    fa.npts = (double)h->npts;
+    fa.part_bins = h->part_bins;
+    fa.part_tot = h->part_tot;
+    fa.part_vg = h->part_vg;
+    fa.diag = h->ddiag + it;
+    fa.spec = h->dspec + (size_t)it * 3 * h->allkmax;
+    fa.status = h->status;
+    LAUNCHCHK(launch_finalize(fa, h->stream));
+    // RK32D (solution.F90:172-203)
+    const void* Ua[2] = {h->U[0], h->U[1]};
+    const void* Ub[2] = {h->UB[0], h->UB[1]};
+    int rc;
+    if ((rc = run_stage<T>(h, 0, gstep, Ua))) return rc;
+    if ((rc = run_stage<T>(h, 1, gstep, Ub))) return rc;
+    if ((rc = run_stage<T>(h, 2, gstep, Ub))) return rc;
+    // forcing2D (solution.F90:303)
+    if (h->p.forcemethod == 2) {
+        ForceArgs<T> f{};
+        f.g = geo<T>(h);
+        f.gate = gstep;
+        f.U[0] = C<T>(h->U[0]);
+        f.U[1] = C<T>(h->U[1]);
+        f.forcek = h->p.forcek;
+        f.target_energy = h->p.target_energy;
+        f.diag = h->ddiag + it;
+        LAUNCHCHK(launch_force_band<T>(f, h->stream));
+    }
+    return 0;
+}
+
+int ensure_capacity(bastr_handle* h, int n) {
+    if (n <= h->cap) return 0;
+    if (h->ddiag) hipFree(h->ddiag);
+    if (h->dspec) hipFree(h->dspec);
+    h->ddiag = nullptr;
+    h->dspec = nullptr;
+    HIPCHK(hipMalloc(&h->ddiag, sizeof(bastr_diag) * n));
+    HIPCHK(hipMalloc(&h->dspec, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * n));
+    HIPCHK(hipMemset(h->dspec, 0, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * n));
+    h->cap = n;
+    return 0;
+}
+
+template <typename T>
+int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
+    int rc;
+    if (h->prologue_pending) {
+        if ((rc = run_prologue<T>(h))) return rc;
+        h->prologue_pending = false;
+    }
+    if ((rc = ensure_capacity(h, nsteps))) return rc;
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(fill_diag_kernel, dim3((nsteps + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nsteps);
+    HIPCHK(hipGetLastError());
+    for (int it = 0; it < nsteps; ++it)
+        if ((rc = run_iteration<T>(h, it))) return rc;
+    int status = INT_MAX;
+    HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int done = status == INT_MAX ? nsteps : status + 1;  // iterations with diagnostics
+    const int K = h->allkmax;
+    if (diags) {
+        HIPCHK(hipMemcpy(diags, h->ddiag, sizeof(bastr_diag) * done, hipMemcpyDeviceToHost));
+    }
+    if (spectra && K > 0) {
+        HIPCHK(hipMemcpy(spectra, h->dspec, sizeof(double) * 3 * K * (size_t)done, hipMemcpyDeviceToHost));
+        for (int it = 0; it < done; ++it)
+            for (int b = 1; b <= K; ++b) spectra[(size_t)it * 3 * K + b - 1] = h->kn[b];
+    }
+    // nstep/time bookkeeping exactly as the reference accumulates them
+    for (int it = 0; it < done; ++it) {
+        if (diags) {
+            diags[it].nstep = h->nstep;
+            diags[it].time = h->time;
+        }
+        if (it == status) break;  // the NaN iteration does not advance
+        h->nstep = h->nstep + 1;
+        h->time = h->time + h->p.deltat;
+    }
+    return status == INT_MAX ? BASTR_OK : BASTR_STOP_NAN;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bastr_last_error(void) { return g_err; }
+
+int bastr_create(const bastr_params* p, bastr_handle** out) {
+    if (!p || !out) return fail(BASTR_EINVAL, "bastr_create: NULL argument");
+    *out = nullptr;
+    if (p->ka != 0) return fail(BASTR_EINVAL, "bastr_create: 3-D grids (ka=%d) are not supported by this build", p->ka);
+    if (!pow2_in(p->ia, 16, 4096) || !pow2_in(p->ja, 16, 4096))
+        return fail(BASTR_EINVAL, "bastr_create: ia, ja must be powers of two in [16, 4096] (got %d, %d)", p->ia, p->ja);
+    if (p->timemethod != 1) return fail(BASTR_EINVAL, "bastr_create: timemethod %d not supported (RK3 = 1)", p->timemethod);
+    if (p->forcemethod != 0 && p->forcemethod != 2)
+        return fail(BASTR_EINVAL, "bastr_create: forcemethod %d not supported (0, 2)", p->forcemethod);
+    if (p->precision != BASTR_PREC_F64 && p->precision != BASTR_PREC_F32)
+        return fail(BASTR_EINVAL, "bastr_create: bad precision %d", p->precision);
+    if (p->kmax < 1) return fail(BASTR_EINVAL, "bastr_create: kmax must be >= 1");
+    if (p->forcemethod == 2 && 2 * p->forcek + 2 > p->ja)
+        return fail(BASTR_EINVAL, "bastr_create: forcek %d too large for ja=%d", p->forcek, p->ja);
+
+    bastr_handle* h = new bastr_handle();
+    h->p = *p;
+    h->nd = 2;
+    h->n0 = p->ia;
+    h->n1 = p->ja;
+    h->prec = p->precision;
+    h->esz = p->precision == BASTR_PREC_F32 ? 4 : 8;
+    h->allkmax = p->ia / 3 < p->kmax ? p->ia / 3 : p->kmax;
+    h->nu = p->reynolds == 0.0 ? 0.0 : 1.0 / p->reynolds;
+    const int cutoff = (int)std::floor((double)p->ia / 3.0);
+    h->cut2 = cutoff * cutoff;
+    h->nh = h->n0 / 2 + 1;
+    const int cb = col_block_width(h->n1, h->prec);
+    h->hp = (h->nh + cb - 1) / cb * cb;
+    h->npts = (size_t)h->n0 * h->n1;
+    h->nhalf = (size_t)h->hp * h->n1;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(BASTR_EDEVICE, "bastr_create: hipStreamCreate failed");
+    }
+    bool ok = true;
+    const size_t cbytes = h->nhalf * 2 * h->esz;
+    for (int c = 0; c < 2; ++c) {
+        h->U[c] = dalloc(cbytes, &ok);
+        h->UB[c] = dalloc(cbytes, &ok);
+        h->KA[c] = dalloc(cbytes, &ok);
+        h->NX[c] = dalloc(cbytes, &ok);
+        h->ureal[c] = dalloc(h->npts * h->esz, &ok);
+    }
+    for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes, &ok);
+    h->dstage = (double*)dalloc(h->npts * sizeof(double), &ok);
+    const int nbx = (h->nh + 63) / 64;
+    const long ntask = (long)h->n1 * nbx;
+    h->nblk_spec = (int)std::max(1L, std::min(512L, ntask / 8));
+    h->nblk_vg = velgrad_blocks(h->n0, h->n1, h->prec);
+    h->part_bins = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * (h->allkmax + 1) * 2, &ok);
+    h->part_tot = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * 4, &ok);
+    h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8, &ok);
+    h->status = (int*)dalloc(sizeof(int), &ok);
+    if (!ok) {
+        bastr_destroy(h);
+        return fail(BASTR_ENOMEM, "bastr_create: device allocation failed");
+    }
+    int rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n0, &h->tw0) : upload_twiddles<double>(h->n0, &h->tw0);
+    if (!rc) rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n1, &h->tw1) : upload_twiddles<double>(h->n1, &h->tw1);
+    if (rc) {
+        bastr_destroy(h);
+        return rc;
+    }
+    // static shell data: Ecount and kn = mean |k| per shell (solution.F90:974-996)
+    const int K = h->allkmax;
+    h->kn.assign(K + 1, 0.0);
+    h->counts.assign(K + 1, 0);
+    for (int j = 0; j < h->n1; ++j)
+        for (int i = 0; i < h->n0; ++i) {
+            const int k1 = i <= h->n0 / 2 ? i : i - h->n0;
+            const int k2 = j <= h->n1 / 2 ? j : j - h->n1;
+            const double kk = std::sqrt((double)k1 * k1 + (double)k2 * k2);
+            const int s = (int)std::lround(kk);
+            if (s <= K) {
+                h->counts[s] += 1;
+                h->kn[s] += kk;
+            }
+        }
+    for (int b = 1; b <= K; ++b) h->kn[b] = h->kn[b] / (double)h->counts[b];
+    *out = h;
+    return BASTR_OK;
+}
+
+int bastr_destroy(bastr_handle* h) {
+    if (!h) return BASTR_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    void* bufs[] = {h->U[0], h->U[1], h->UB[0], h->UB[1], h->KA[0], h->KA[1], h->NX[0], h->NX[1],
+                    h->ureal[0], h->ureal[1], h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf,
+                    h->part_bins, h->part_tot, h->part_vg, h->status, h->ddiag, h->dspec};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (int q = 0; q < 7; ++q)
+        if (h->W[q]) hipFree(h->W[q]);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return BASTR_OK;
+}
+
+int bastr_allkmax(const bastr_handle* h) { return h ? h->allkmax : -1; }
+
+int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const double* u3) {
+    (void)u3;
+    if (!h || !u1 || !u2) return fail(BASTR_EINVAL, "bastr_set_fields: NULL argument");
+    const double* src[2] = {u1, u2};
+    for (int c = 0; c < 2; ++c) {
+        if (h->prec == BASTR_PREC_F64) {
+            HIPCHK(hipMemcpy(h->ureal[c], src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemcpy(h->dstage, src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3(1024), dim3(256), 0, h->stream, h->dstage,
+                               (float*)h->ureal[c], h->npts);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+    }
+    h->prologue_pending = true;
+    h->have_fields = true;
+    return BASTR_OK;
+}
+
+int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3) {
+    (void)u3;
+    if (!h || !u1 || !u2) return fail(BASTR_EINVAL, "bastr_get_fields: NULL argument");
+    if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_get_fields: fields not set");
+    int rc;
+    if (h->prologue_pending) {
+        rc = h->prec == BASTR_PREC_F32 ? run_prologue<float>(h) : run_prologue<double>(h);
+        if (rc) return rc;
+        h->prologue_pending = false;
+    }
+    rc = h->prec == BASTR_PREC_F32 ? run_to_physical<float>(h) : run_to_physical<double>(h);
+    if (rc) return rc;
+    double* dst[2] = {u1, u2};
+    for (int c = 0; c < 2; ++c) {
+        if (h->prec == BASTR_PREC_F64) {
+            HIPCHK(hipMemcpyAsync(dst[c], h->ureal[c], sizeof(double) * h->npts, hipMemcpyDeviceToHost, h->stream));
+        } else {
+            hipLaunchKernelGGL((convert_kernel<float, double>), dim3(1024), dim3(256), 0, h->stream,
+                               (const float*)h->ureal[c], h->dstage, h->npts);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(dst[c], h->dstage, sizeof(double) * h->npts, hipMemcpyDeviceToHost, h->stream));
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return BASTR_OK;
+}
+
+int bastr_set_time(bastr_handle* h, int64_t nstep, double time) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_set_time: NULL handle");
+    h->nstep = nstep;
+    h->time = time;
+    return BASTR_OK;
+}
+
+int bastr_get_time(const bastr_handle* h, int64_t* nstep, double* time) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_get_time: NULL handle");
+    if (nstep) *nstep = h->nstep;
+    if (time) *time = h->time;
+    return BASTR_OK;
+}
+
+int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_step: NULL handle");
+    if (nsteps < 0) return fail(BASTR_EINVAL, "bastr_step: nsteps < 0");
+    if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_step: fields not set");
+    if (nsteps == 0) return BASTR_OK;
+    return h->prec == BASTR_PREC_F32 ? step_impl<float>(h, nsteps, diags, spectra)
+                                     : step_impl<double>(h, nsteps, diags, spectra);
+}
+
+int bastr_shell_counts(const bastr_handle* h, int64_t* counts) {
+    if (!h || !counts) return fail(BASTR_EINVAL, "bastr_shell_counts: NULL argument");
+    for (int b = 1; b <= h->allkmax; ++b) counts[b - 1] = h->counts[b];
+    return BASTR_OK;
+}
+
+static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
+    if (!h || !inout) return fail(BASTR_EINVAL, "bastr_fft: NULL argument");
+    const size_t bytes = sizeof(double) * 2 * h->npts;
+    if (!h->cbuf) HIPCHK(hipMalloc(&h->cbuf, bytes));
+    const cx<double>* tw0 = (const cx<double>*)h->tw0;
+    const cx<double>* tw1 = (const cx<double>*)h->tw1;
+    if (h->prec != BASTR_PREC_F64) {
+        if (!h->tw0d) {
+            int rc = upload_twiddles<double>(h->n0, (void**)&h->tw0d);
+            if (!rc) rc = upload_twiddles<double>(h->n1, (void**)&h->tw1d);
+            if (rc) return rc;
+        }
+        tw0 = (const cx<double>*)h->tw0d;
+        tw1 = (const cx<double>*)h->tw1d;
+    }
+    HIPCHK(hipMemcpyAsync(h->cbuf, inout, bytes, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    const Gate open{h->status, 0, 0};
+    Geo<double> g = geo<double>(h);
+    cx<double>* z = (cx<double>*)h->cbuf;
+    if (!inverse) {
+        RowArgs<double> r{};
+        r.g = g;
+        r.gate = open;
+        r.cin = z;
+        r.cout = z;
+        r.tw = tw0;
+        LAUNCHCHK(launch_row<double>(h->n0, R_C2C_FWD, r, h->n1, h->stream));
+        ColFwdArgs<double> c{};
+        c.g = g;
+        c.gate = open;
+        c.nx[0] = z;
+        c.out[0] = z;
+        c.nfield = 1;
+        c.pitch = h->n0;
+        c.tw = tw1;
+        if (col_block_width(h->n1, BASTR_PREC_F64) > h->n0) return fail(BASTR_EINVAL, "bastr_fft: ia too small for ja");
+        LAUNCHCHK(launch_col_fwd<double>(h->n1, CF_PLAIN, c, h->n0, h->stream));
+    } else {
+        ColInvArgs<double> c{};
+        c.g = g;
+        c.gate = open;
+        c.in[0] = z;
+        c.out[0] = z;
+        c.nfield = 1;
+        c.pitch = h->n0;
+        c.tw = tw1;
+        if (col_block_width(h->n1, BASTR_PREC_F64) > h->n0) return fail(BASTR_EINVAL, "bastr_fft: ia too small for ja");
+        LAUNCHCHK(launch_col_inv<double>(h->n1, CI_PLAIN, c, h->n0, h->stream));
+        RowArgs<double> r{};
+        r.g = g;
+        r.gate = open;
+        r.cin = z;
+        r.cout = z;
+        r.tw = tw0;
+        LAUNCHCHK(launch_row<double>(h->n0, R_C2C_INV, r, h->n1, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(inout, h->cbuf, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return BASTR_OK;
+}
+
+int bastr_fft_forward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, false); }
+int bastr_fft_backward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, true); }
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+// fft_block.hpp -- workgroup-level Stockham FFT primitives for gfx950.
+//
+// A line of N complex points is held by P = N/E threads, E points each, in
+// registers: thread j holds x[j + k*P] for k < E ("strided" order).  Each
+// radix pass is a butterfly on the thread's own registers; between passes the
+// line is exchanged through LDS (optionally in two halves, real then
+// imaginary, to halve the LDS footprint).  After the last pass thread j holds
+// X[j + k*P] in natural order, so loads before and stores after the transform
+// are coalesced along the line.
+//
+// Pass p of radix R with Ns = product of earlier radices (Stockham autosort):
+//   butterfly b reads x[b + r*N/R], twiddles by w_N^{(b mod Ns)*r*N/(Ns*R)},
+//   does a radix-R DFT and writes y[(b/Ns)*Ns*R + (b mod Ns) + r*Ns].
+// Radices are E for all passes but possibly the last (N = E^a * L, L | E).
+// Twiddles come from an FP64-accurate table tw[m] = exp(-2*pi*i*m/N) built on
+// the host; the inverse direction conjugates them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
+// i*a
+template <typename T>
+__device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.y, a.x}; }
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+// cos/sin(2*pi*t/16), t = 0..15
+__device__ constexpr double kC16[16] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+__device__ constexpr double kS16[16] = {
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+// a * exp(sgn*2*pi*i*t/16) with sgn = -1 (forward) or +1 (inverse); t known at
+// compile time after unrolling, so the trivial cases fold away.
+template <typename T, bool INV>
+__device__ __forceinline__ cx<T> rot16(cx<T> a, int t) {
+    t &= 15;
+    if (t == 0) return a;
+    if (t == 8) return {-a.x, -a.y};
+    if (t == 4) return INV ? cx<T>{-a.y, a.x} : cx<T>{a.y, -a.x};
+    if (t == 12) return INV ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
+    const T c = (T)kC16[t];
+    const T s = INV ? (T)kS16[t] : -(T)kS16[t];
+    if (t == 2 || t == 6 || t == 10 || t == 14) {
+        // |c| == |s| == sqrt(1/2): (x + iy)(sc + i ss) h with sc, ss = +-1
+        const T h = (T)0.70710678118654752440;
+        const T sc = c > 0 ? (T)1 : (T)-1, ss = s > 0 ? (T)1 : (T)-1;
+        return {(sc * a.x - ss * a.y) * h, (ss * a.x + sc * a.y) * h};
+    }
+    return {a.x * c - a.y * s, a.x * s + a.y * c};
+}
+
+constexpr int bitrev(int x, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+// In-register radix-R DFT (R = 2, 4, 8, 16): X[k] = sum_r v[r] w_R^{+-rk}.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cx<T>* v) {
+    constexpr int B = ilog2(R);
+    cx<T> a[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) a[i] = v[bitrev(i, B)];
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int j = 0; j < len / 2; ++j) {
+                const cx<T> u = a[i + j];
+                const cx<T> w = rot16<T, INV>(a[i + j + len / 2], j * (16 / len));
+                a[i + j] = cadd(u, w);
+                a[i + j + len / 2] = csub(u, w);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = a[i];
+}
+
+// Per-line LDS view.  SPLIT: real and imaginary parts exchanged in two rounds
+// through one T buffer of the line's length; otherwise one cx<T> buffer.
+template <typename T, bool SPLIT>
+struct LdsLine;
+
+template <typename T>
+struct LdsLine<T, false> {
+    cx<T>* base;
+    int stride;  // element e at base[e*stride]
+    __device__ __forceinline__ void st(int e, cx<T> v) const { base[e * stride] = v; }
+    __device__ __forceinline__ cx<T> ld(int e) const { return base[e * stride]; }
+};
+
+template <typename T>
+struct LdsLine<T, true> {
+    T* base;
+    int stride;
+};
+
+// Exchange: values v[r] written at positions widx(r), then read back so that
+// out[q] = line[ridx(q)].  All threads of the workgroup must call it.
+template <typename T, int NV, class W, class Rd>
+__device__ __forceinline__ void exchange(const LdsLine<T, false>& L, cx<T>* v, W widx, Rd ridx) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) L.st(widx(r), v[r]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) v[r] = L.ld(ridx(r));
+}
+
+template <typename T, int NV, class W, class Rd>
+__device__ __forceinline__ void exchange(const LdsLine<T, true>& L, cx<T>* v, W widx, Rd ridx) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) L.base[widx(r) * L.stride] = v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) v[r].x = L.base[ridx(r) * L.stride];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) L.base[widx(r) * L.stride] = v[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NV; ++r) v[r].y = L.base[ridx(r) * L.stride];
+}
+
+// twiddle v[r] *= w_N^{(b mod Ns) * r * N/(Ns*R)}, r = 1..R-1
+template <typename T, int N, int R, bool INV>
+__device__ __forceinline__ void twiddle(cx<T>* v, int b, int Ns, const cx<T>* __restrict__ tw) {
+    if (Ns == 1) return;
+    const int bm = b & (Ns - 1);
+    const int step = bm * (N / (Ns * R));
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        cx<T> w = tw[step * r];
+        if (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+}
+
+template <int N, int E>
+struct FftShape {
+    static_assert((N & (N - 1)) == 0 && N >= 4, "N must be a power of two");
+    static_assert(E == 4 || E == 8 || E == 16, "E must be 4, 8 or 16");
+    static constexpr int P = N / E;          // threads per line
+    static constexpr int NFULL = ilog2(N) / ilog2(E);
+    static constexpr int LAST = 1 << (ilog2(N) % ilog2(E));  // 1: no partial pass
+    static_assert(NFULL >= 1, "N must be >= E");
+};
+
+// Full line transform.  On entry v[k] = x[j + k*P]; on exit v[k] = X[j + k*P].
+// Unnormalised in both directions.
+template <typename T, int N, int E, bool INV, bool SPLIT>
+__device__ __forceinline__ void line_fft(cx<T>* v, int j, const LdsLine<T, SPLIT>& L,
+                                         const cx<T>* __restrict__ tw) {
+    using S = FftShape<N, E>;
+    constexpr int P = S::P;
+    int Ns = 1;
+#pragma unroll
+    for (int p = 0; p < S::NFULL; ++p) {
+        twiddle<T, N, E, INV>(v, j, Ns, tw);
+        dft<T, E, INV>(v);
+        const bool last = (p == S::NFULL - 1) && (S::LAST == 1);
+        if (!last) {
+            const int base = (j / Ns) * Ns * E + (j & (Ns - 1));
+            const int ns = Ns;
+            if (p < S::NFULL - 1) {
+                exchange<T, E>(L, v, [&](int r) { return base + r * ns; }, [&](int r) { return j + r * P; });
+            } else {
+                constexpr int Lr = S::LAST;
+                exchange<T, E>(L, v, [&](int r) { return base + r * ns; },
+                               [&](int q) { return j + (q / Lr) * P + (q % Lr) * (N / Lr); });
+            }
+        }
+        Ns *= E;
+    }
+    if constexpr (S::LAST > 1) {
+        constexpr int Lr = S::LAST;
+        constexpr int Q = E / Lr;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            twiddle<T, N, Lr, INV>(v + q * Lr, j + q * P, Ns, tw);
+            dft<T, Lr, INV>(v + q * Lr);
+        }
+        cx<T> t[E];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int r = 0; r < Lr; ++r) t[q + r * Q] = v[q * Lr + r];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = t[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// wavefront reductions (64 lanes), deterministic butterfly order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_min_i(int x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_max_i(int x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}

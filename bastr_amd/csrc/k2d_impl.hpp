@@ -1,0 +1,649 @@
+// k2d_impl.hpp -- kernel bodies of the 2-D step (included once per precision).
+//
+// Reference correspondence (src/solution.F90, src/tool.F90):
+//   col_inv_kernel<CI_STAGE>   spectral derivatives u_hat * i k_y (compute_ut2D :825-830),
+//                              y half of the inverse transforms (:832-839)
+//   row_kernel<R_STAGE>        x half of those inverse transforms, the nonlinear
+//                              term -u_j d_j u_i (:841-846), x half of fft2d (:850-851)
+//   col_fwd_kernel<CF_A/B/C>   y half of fft2d, normalisation (fftwlink.F90:334),
+//                              + nu*Lap(u) in spectral space, dealiasing (:853-854),
+//                              projection (:856-858), Kutta RK3 combination (:183-200)
+//   spectra_kernel             spectra_compute2D (:930-1016)
+//   col_inv<CI_STAGE_VG> +
+//   velgrad_kernel             velgrad_calculate2D (:1110-1186)
+//   force_band_kernel          forcing2D method 2 (:336-378)
+#pragma once
+#include <climits>
+#include "k2d.hpp"
+
+namespace k2d {
+
+__device__ __forceinline__ int ksigned(int m, int n) { return m <= n / 2 ? m : m - n; }
+
+// z = a + i b for packed position m of a length-N row, from half rows A, B
+// (kx = 0..N/2).  DC and Nyquist take real parts (the real part of a c2c
+// inverse equals the c2r inverse of the Hermitian part).
+template <typename T, int N>
+__device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, int m) {
+    if (m == 0 || m == N / 2) return {A[m].x, B[m].x};
+    if (m < N / 2) {
+        const cx<T> a = A[m], b = B[m];
+        return {a.x - b.y, a.y + b.x};
+    }
+    const cx<T> a = A[N - m], b = B[N - m];
+    return {a.x + b.y, b.x - a.y};
+}
+
+// Store the half spectra of a forward-transformed packed row: thread j holds
+// X[j + k*P]; writes (X[h] + conj X[N-h])/2 and (X[h] - conj X[N-h])/(2i).
+template <typename T, int N, int E>
+__device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, false>& L, cx<T>* __restrict__ o1,
+                                             cx<T>* __restrict__ o2, bool valid) {
+    constexpr int P = N / E;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) L.st(j + k * P, v[k]);
+    __syncthreads();
+    const T half = (T)0.5;
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k) {
+        const int h = j + k * P;
+        const cx<T> a = L.ld(h);
+        const cx<T> bm = L.ld((N - h) & (N - 1));
+        const cx<T> b = cconj(bm);
+        const cx<T> d = csub(a, b);
+        if (valid) {
+            o1[h] = {(a.x + b.x) * half, (a.y + b.y) * half};
+            o2[h] = {d.y * half, -d.x * half};
+        }
+    }
+    if (j == 0 && valid) {
+        const cx<T> a = L.ld(N / 2);
+        o1[N / 2] = {a.x, (T)0};
+        o2[N / 2] = {a.y, (T)0};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y-direction inverse pass
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_inv_kernel(ColInvArgs<T> a) {
+    using C = ColCfg<T, N>;
+    constexpr int E = C::E, P = C::P, CB = C::CB;
+    if (a.gate.closed()) return;
+    __shared__ T lds[CB * N];
+    const int tid = threadIdx.x;
+    const int l = tid % CB, j = tid / CB;
+    const int col = blockIdx.x * CB + l;
+    const LdsLine<T, true> L{lds + l, CB};
+    const size_t pitch = (size_t)a.pitch;
+    const int nf = MODE == CI_PLAIN ? a.nfield : 2;
+    for (int f = 0; f < nf; ++f) {
+        cx<T> u[E];
+        const cx<T>* __restrict__ src = a.in[f];
+#pragma unroll
+        for (int k = 0; k < E; ++k) u[k] = src[(size_t)(j + k * P) * pitch + col];
+        const int njob = MODE == CI_PLAIN ? 1 : (MODE == CI_STAGE ? 2 : (f == 0 ? 3 : 4));
+        for (int job = 0; job < njob; ++job) {
+            cx<T> v[E];
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int m = j + k * P;
+                const T ky = (T)ksigned(m, N);
+                const T kyo = (m == N / 2) ? (T)0 : ky;  // odd powers vanish at the Nyquist row
+                if (job == 0) v[k] = u[k];
+                else if (job == 1) v[k] = cscale(cmuli(u[k]), kyo);                 // i k2 u
+                else if (job == 2) v[k] = cscale(u[k], -(ky * ky));                 // -k2^2 u
+                else v[k] = cscale(cmuli(u[k]), -(kyo * kyo * kyo));                 // -i k2^3 u
+            }
+            line_fft<T, N, E, true, true>(v, j, L, a.tw);
+            cx<T>* __restrict__ dst;
+            if (MODE == CI_PLAIN) dst = a.out[f];
+            else if (job == 0) dst = a.out[f];
+            else if (job == 1) dst = a.out[2 + f];
+            else if (job == 2) dst = a.out[4 + f];
+            else dst = a.out[6];
+#pragma unroll
+            for (int k = 0; k < E; ++k) dst[(size_t)(j + k * P) * pitch + col] = v[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y-direction forward pass + spectral update
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwdArgs<T> a) {
+    using C = ColCfg<T, N>;
+    constexpr int E = C::E, P = C::P, CB = C::CB;
+    if (a.gate.closed()) return;
+    __shared__ T lds[CB * N];
+    const int tid = threadIdx.x;
+    const int l = tid % CB, j = tid / CB;
+    const int col = blockIdx.x * CB + l;
+    const LdsLine<T, true> L{lds + l, CB};
+    const size_t pitch = (size_t)a.pitch;
+    const T sc = (T)a.g.inv_n;
+
+    if (MODE == CF_PLAIN) {
+        for (int f = 0; f < a.nfield; ++f) {
+            cx<T> v[E];
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = a.nx[f][(size_t)(j + k * P) * pitch + col];
+            line_fft<T, N, E, false, true>(v, j, L, a.tw);
+#pragma unroll
+            for (int k = 0; k < E; ++k) a.out[f][(size_t)(j + k * P) * pitch + col] = cscale(v[k], sc);
+        }
+        return;
+    }
+
+    cx<T> v0[E], v1[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) v0[k] = a.nx[0][(size_t)(j + k * P) * pitch + col];
+    line_fft<T, N, E, false, true>(v0, j, L, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) v1[k] = a.nx[1][(size_t)(j + k * P) * pitch + col];
+    line_fft<T, N, E, false, true>(v1, j, L, a.tw);
+
+    const int kx = col;
+    const T dt = (T)a.g.dt;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int ky = ksigned(m, N);
+        const int kk2 = kx * kx + ky * ky;
+        const size_t idx = (size_t)m * pitch + col;
+        cx<T> t0 = cscale(v0[k], sc), t1 = cscale(v1[k], sc);
+        if (MODE == CF_PROLOGUE) {
+            // mainloop2D entry: u_hat = dealias(fft2d(u)) (solution.F90:41-47)
+            if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
+            a.U[0][idx] = t0;
+            a.U[1][idx] = t1;
+            continue;
+        }
+        const cx<T> U0 = a.U[0][idx], U1 = a.U[1][idx];
+        cx<T> s0, s1;  // this stage's input spectrum (for the viscous term)
+        cx<T> ka0, ka1;
+        if (MODE == CF_A) {
+            s0 = U0; s1 = U1;
+        } else if (MODE == CF_B) {
+            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+            s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
+            s1 = cadd(U1, cscale(cscale(ka1, dt), (T)0.5));
+        } else {
+            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+            s0 = a.UB[0][idx]; s1 = a.UB[1][idx];
+        }
+        // r_i = N_i + nu * Lap(u_i)   (compute_ut2D :841-846)
+        const T vis = (T)(-a.g.nu * (double)kk2);
+        t0 = cadd(t0, cscale(s0, vis));
+        t1 = cadd(t1, cscale(s1, vis));
+        if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
+        if (a.g.lproject) {
+            // projection2D (tool.F90:219-243)
+            if (kk2 > 0) {
+                const T kkr = sqrt((T)kk2);
+                const T c1 = (T)kx / kkr, c2 = (T)ky / kkr;
+                const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
+                t0 = cscale(ud, c1);
+                t1 = cscale(ud, c2);
+            } else {
+                t0 = {0, 0}; t1 = {0, 0};
+            }
+        }
+        if (MODE == CF_A) {
+            a.KA[0][idx] = t0; a.KA[1][idx] = t1;
+            a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
+            a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
+        } else if (MODE == CF_B) {
+            const T dt2 = (T)2 * dt;
+            a.UB[0][idx] = cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2));
+            a.UB[1][idx] = cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2));
+            a.KA[0][idx] = cadd(ka0, cscale(t0, (T)4));
+            a.KA[1][idx] = cadd(ka1, cscale(t1, (T)4));
+        } else {
+            const T r6 = (T)(1.0 / 6.0);
+            a.U[0][idx] = cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6));
+            a.U[1][idx] = cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-direction passes
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T> a) {
+    using C = RowCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * N];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < a.g.n1;
+    const int row = valid ? row0 : a.g.n1 - 1;
+    const LdsLine<T, false> Ls{lds + l * N, 1};
+    const size_t hp = (size_t)a.g.hp;
+
+    if (MODE == R_C2C_FWD || MODE == R_C2C_INV) {
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = a.cin[(size_t)row * N + j + k * P];
+        line_fft<T, N, E, MODE == R_C2C_INV, false>(v, j, Ls, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) a.cout[(size_t)row * N + j + k * P] = v[k];
+        }
+        return;
+    }
+    if (MODE == R_FWD_PACK) {
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const size_t o = (size_t)row * N + j + k * P;
+            v[k] = {a.rin[0][o], a.rin[1][o]};
+        }
+        line_fft<T, N, E, false, false>(v, j, Ls, a.tw);
+        unpack_store<T, N, E>(v, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+        return;
+    }
+    const cx<T>* A1 = a.hin[0] + (size_t)row * hp;
+    const cx<T>* A2 = a.hin[1] + (size_t)row * hp;
+    if (MODE == R_INV_PACK) {
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(A1, A2, j + k * P);
+        line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const size_t o = (size_t)row * N + j + k * P;
+                a.rout[0][o] = v[k].x;
+                a.rout[1][o] = v[k].y;
+            }
+        }
+        return;
+    }
+    // R_STAGE: u, du/dx, du/dy from half rows, nonlinear term, forward x pass
+    cx<T> z[E], zx[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        z[k] = pack_at<T, N>(A1, A2, m);
+        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
+        zx[k] = cscale(cmuli(z[k]), kxs);
+    }
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);   // (u1, u2)
+    line_fft<T, N, E, true, false>(zx, j, Ls, a.tw);  // (d1 u1, d1 u2)
+    cx<T> n[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * zx[k].x), -(z[k].x * zx[k].y)};
+    const cx<T>* B1 = a.hin[2] + (size_t)row * hp;
+    const cx<T>* B2 = a.hin[3] + (size_t)row * hp;
+#pragma unroll
+    for (int k = 0; k < E; ++k) zx[k] = pack_at<T, N>(B1, B2, j + k * P);
+    line_fft<T, N, E, true, false>(zx, j, Ls, a.tw);  // (d2 u1, d2 u2)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        n[k].x = n[k].x - z[k].y * zx[k].x;
+        n[k].y = n[k].y - z[k].y * zx[k].y;
+    }
+    line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+}
+
+// velocity-gradient moments (velgrad_calculate2D, solution.F90:1110-1186)
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void velgrad_kernel(VgArgs<T> a) {
+    using C = RowCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * N];
+    __shared__ double red[C::THREADS / 64][8];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < a.g.n1;
+    const int row = valid ? row0 : a.g.n1 - 1;
+    const LdsLine<T, false> Ls{lds + l * N, 1};
+    const size_t hp = (size_t)a.g.hp;
+    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
+    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
+    const cx<T>* B1 = a.w[2] + (size_t)row * hp;
+    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
+    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
+    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
+    const cx<T>* D = a.w[6] + (size_t)row * hp;
+    const double nu = a.g.nu;
+
+    cx<T> z[E], w[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        z[k] = pack_at<T, N>(A1, A2, m);
+        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
+        w[k] = cscale(cmuli(z[k]), kxs);
+    }
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a11 = d1u1, a21 = d1u2
+    double a11[E], a21[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) { a11[k] = w[k].x; a21[k] = w[k].y; }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(B1, B2, j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a12 = d2u1, a22 = d2u2
+
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0;
+    double eta = (double)(2.0f * 3.14f);
+    double dv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double u1 = z[k].x, u2 = z[k].y;
+        const double b11 = a11[k], b21 = a21[k], b12 = w[k].x, b22 = w[k].y;
+        const double div = b11 + b22;
+        const double uu = u1 * u1 + u2 * u2;
+        dv[k] = div;
+        s_um2 += div * div * uu;
+        s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
+        s_u2t += div * uu;
+        s_eps += nu * (b12 - b21) * (b12 - b21) + nu * div * div;
+        eta = fmin(eta, sqrt(nu / fabs(div)));
+        s_urms += u1 * u1 + u2 * u2;
+        s_dudx += b11 * b11 + b22 * b22;
+    }
+    // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int kx = m <= N / 2 ? m : N - m;
+        const T k1sq = (T)kx * (T)kx;
+        const cx<T> pa = pack_at<T, N>(A1, A2, m);
+        const cx<T> pc = pack_at<T, N>(C1, C2, m);
+        w[k] = cadd(cscale(pa, -k1sq), pc);
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        s_dis += 2.0 * (double)z[k].x * dv[k] * (double)w[k].x + 2.0 * (double)z[k].y * dv[k] * (double)w[k].y;
+    // Lap(theta) = x-inverse of (-i k1^3 A1 + i k1 C1 - k1^2 B2 + D), a real field
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1 = (T)h;
+        const T k1o = (h == N / 2) ? (T)0 : k1;
+        const cx<T> th = cadd(cadd(cscale(cmuli(A1[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
+                              cadd(cscale(B2[h], -(k1 * k1)), D[h]));
+        if (m == 0 || m == N / 2) w[k] = {th.x, (T)0};
+        else if (m < N / 2) w[k] = th;
+        else w[k] = cconj(th);
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
+        s_dis += uu * (double)w[k].x;
+    }
+    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = (double)(2.0f * 3.14f); }
+    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
+    const int wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
+    vals[7] = wave_min(vals[7]);
+    __syncthreads();
+    if (lane == 0)
+        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
+    __syncthreads();
+    if (tid < 8) {
+        double acc = tid == 7 ? (double)(2.0f * 3.14f) : 0.0;
+        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
+        a.part[(size_t)blockIdx.x * 8 + tid] = acc;
+    }
+}
+
+// shell spectra (spectra_compute2D, solution.F90:930-1016); deterministic:
+// each wave owns its LDS bins and adds to them in program order.
+template <typename T>
+__global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
+    if (a.gate.closed()) return;
+    extern __shared__ double sh[];
+    const int K = a.allkmax, nb = K + 1;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    double* bins = sh;                // [4][nb][2]
+    double* tots = sh + 4 * nb * 2;   // [4][4]
+    for (int i = tid; i < 4 * nb * 2; i += 256) bins[i] = 0;
+    __syncthreads();
+    const int n0 = a.g.n0, n1 = a.g.n1, nh = a.g.nh;
+    const size_t hp = (size_t)a.g.hp;
+    const int nbx = (nh + 63) / 64;
+    const long ntask = (long)n1 * nbx;
+    const long t0 = ntask * blockIdx.x / gridDim.x, t1 = ntask * (blockIdx.x + 1) / gridDim.x;
+    double tot[4] = {0, 0, 0, 0};
+    for (long t = t0 + wave; t < t1; t += 4) {
+        const int r = (int)(t / nbx);
+        const int kx = (int)(t % nbx) * 64 + lane;
+        double es = 0, ed = 0, w = 0;
+        int s = INT_MAX;
+        if (kx < nh) {
+            const int ky = ksigned(r, n1);
+            w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+            const double kk = sqrt((double)(kx * kx + ky * ky));
+            if (kk > 0.5) {
+                const cx<T> u1 = a.U[0][(size_t)r * hp + kx], u2 = a.U[1][(size_t)r * hp + kx];
+                const double k1 = kx, k2 = ky;
+                const double usx = (double)u1.x * k2 / kk - (double)u2.x * k1 / kk;
+                const double usy = (double)u1.y * k2 / kk - (double)u2.y * k1 / kk;
+                const double udx = (double)u1.x * k1 / kk + (double)u2.x * k2 / kk;
+                const double udy = (double)u1.y * k1 / kk + (double)u2.y * k2 / kk;
+                es = (usx * usx + usy * usy) / 2;
+                ed = (udx * udx + udy * udy) / 2;
+                tot[3] += w * (ed / kk);
+            }
+            tot[0] += w * es;
+            tot[1] += w * ed;
+            tot[2] += w * (ed * (kk * kk));
+            const int sh_ = (int)(kk + 0.5);
+            if (sh_ <= K) s = sh_;
+        }
+        const int smin = wave_min_i(s);
+        if (smin != INT_MAX) {
+            const int smax = wave_max_i(s == INT_MAX ? -1 : s);
+            for (int b = smin; b <= smax; ++b) {
+                const double des = wave_sum(s == b ? w * es : 0.0);
+                const double ded = wave_sum(s == b ? w * ed : 0.0);
+                if (lane == 0) {
+                    bins[(wave * nb + b) * 2 + 0] += des;
+                    bins[(wave * nb + b) * 2 + 1] += ded;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tot[q] = wave_sum(tot[q]);
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) tots[wave * 4 + q] = tot[q];
+    __syncthreads();
+    for (int b = tid; b < nb; b += 256) {
+        for (int c = 0; c < 2; ++c) {
+            double acc = 0;
+            for (int wv = 0; wv < 4; ++wv) acc += bins[(wv * nb + b) * 2 + c];
+            a.part_bins[((size_t)blockIdx.x * nb + b) * 2 + c] = acc;
+        }
+    }
+    if (tid < 4) {
+        double acc = 0;
+        for (int wv = 0; wv < 4; ++wv) acc += tots[wv * 4 + tid];
+        a.part_tot[(size_t)blockIdx.x * 4 + tid] = acc;
+    }
+}
+
+// band forcing, method 2 (forcing2D :336-378): E = sum over band modes of
+// |u1_hat|^2 + |u2_hat|^2 (Parseval of E = <f1^2 + f2^2>), factor =
+// sqrt(E_t/E) - 1, u_hat += factor * u_hat on the band.
+template <typename T>
+__global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
+    if (a.gate.closed()) return;
+    __shared__ double red[4];
+    __shared__ double fac;
+    const int tid = threadIdx.x;
+    const int n0 = a.g.n0, n1 = a.g.n1;
+    const size_t hp = (size_t)a.g.hp;
+    const int fk = a.forcek;
+    const int kxm = fk < n0 / 2 ? fk : n0 / 2;
+    const int kym = fk < n1 / 2 ? fk : n1 / 2 - 1;
+    const int nx = kxm + 1, ny = 2 * kym + 1;
+    double e = 0;
+    for (int t = tid; t < nx * ny; t += 256) {
+        const int kx = t % nx, ky = t / nx - kym;
+        const int r = ky >= 0 ? ky : n1 + ky;
+        const double kk = sqrt((double)(kx * kx + ky * ky));
+        if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
+            const double w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+            const cx<T> u1 = a.U[0][(size_t)r * hp + kx], u2 = a.U[1][(size_t)r * hp + kx];
+            e += w * ((double)u1.x * u1.x + (double)u1.y * u1.y + (double)u2.x * u2.x + (double)u2.y * u2.y);
+        }
+    }
+    e = wave_sum(e);
+    if (tid % 64 == 0) red[tid / 64] = e;
+    __syncthreads();
+    if (tid == 0) {
+        const double E = red[0] + red[1] + red[2] + red[3];
+        const double factor = sqrt(a.target_energy / E) - 1.0;
+        fac = factor;
+        a.diag->force_factor = factor;
+        a.diag->force_E = E;
+    }
+    __syncthreads();
+    const T f = (T)fac;
+    for (int t = tid; t < nx * ny; t += 256) {
+        const int kx = t % nx, ky = t / nx - kym;
+        const int r = ky >= 0 ? ky : n1 + ky;
+        const double kk = sqrt((double)(kx * kx + ky * ky));
+        if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
+            for (int c = 0; c < 2; ++c) {
+                cx<T>* p = a.U[c] + (size_t)r * hp + kx;
+                const cx<T> u = *p;
+                *p = cadd(u, cscale(u, f));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define K2D_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+template <typename T, int N>
+static int col_inv_n(int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s) {
+    using C = ColCfg<T, N>;
+    if (ncols % C::CB) return -1;
+    dim3 grid(ncols / C::CB), block(C::THREADS);
+    switch (mode) {
+        case CI_PLAIN: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_PLAIN>), grid, block, 0, s, a); break;
+        case CI_STAGE: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_STAGE>), grid, block, 0, s, a); break;
+        case CI_STAGE_VG: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_STAGE_VG>), grid, block, 0, s, a); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int col_fwd_n(int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s) {
+    using C = ColCfg<T, N>;
+    if (ncols % C::CB) return -1;
+    dim3 grid(ncols / C::CB), block(C::THREADS);
+    switch (mode) {
+        case CF_PLAIN: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PLAIN>), grid, block, 0, s, a); break;
+        case CF_PROLOGUE: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PROLOGUE>), grid, block, 0, s, a); break;
+        case CF_A: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_A>), grid, block, 0, s, a); break;
+        case CF_B: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_B>), grid, block, 0, s, a); break;
+        case CF_C: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_C>), grid, block, 0, s, a); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
+    using C = RowCfg<T, N>;
+    dim3 grid((nrows + C::L - 1) / C::L), block(C::THREADS);
+    switch (mode) {
+        case R_FWD_PACK: hipLaunchKernelGGL((row_kernel<T, N, R_FWD_PACK>), grid, block, 0, s, a); break;
+        case R_INV_PACK: hipLaunchKernelGGL((row_kernel<T, N, R_INV_PACK>), grid, block, 0, s, a); break;
+        case R_STAGE: hipLaunchKernelGGL((row_kernel<T, N, R_STAGE>), grid, block, 0, s, a); break;
+        case R_C2C_FWD: hipLaunchKernelGGL((row_kernel<T, N, R_C2C_FWD>), grid, block, 0, s, a); break;
+        case R_C2C_INV: hipLaunchKernelGGL((row_kernel<T, N, R_C2C_INV>), grid, block, 0, s, a); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
+    using C = RowCfg<T, N>;
+    const int g = (nrows + C::L - 1) / C::L;
+    if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
+    hipLaunchKernelGGL((velgrad_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    return 0;
+}
+
+template <typename T>
+int launch_col_inv(int n, int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return col_inv_n<T, NN>(mode, a, ncols, s);
+        K2D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_col_fwd(int n, int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return col_fwd_n<T, NN>(mode, a, ncols, s);
+        K2D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return row_n<T, NN>(mode, a, nrows, s);
+        K2D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
+    switch (n) {
+#define X(NN) case NN: return vg_n<T, NN>(a, nrows, s, nblk);
+        K2D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s) {
+    const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
+    hipLaunchKernelGGL((spectra_kernel<T>), dim3(nblk), dim3(256), shm, s, a);
+    return 0;
+}
+template <typename T>
+int launch_force_band(const ForceArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((force_band_kernel<T>), dim3(1), dim3(256), 0, s, a);
+    return 0;
+}
+
+#define K2D_INSTANTIATE(T)                                                                     \
+    template int launch_col_inv<T>(int, int, const ColInvArgs<T>&, int, hipStream_t);          \
+    template int launch_col_fwd<T>(int, int, const ColFwdArgs<T>&, int, hipStream_t);          \
+    template int launch_row<T>(int, int, const RowArgs<T>&, int, hipStream_t);                 \
+    template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*);             \
+    template int launch_spectra<T>(const SpecArgs<T>&, int, hipStream_t);                      \
+    template int launch_force_band<T>(const ForceArgs<T>&, hipStream_t);
+
+}  // namespace k2d

@@ -1,0 +1,145 @@
+/*
+ * bastr.h -- C-ABI of the MI355X-native Bastr pseudo-spectral time step.
+ *
+ * This library takes the place of the reference's FFT boundary module
+ * (src/fftwlink.F90) together with the solver core it feeds
+ * (src/solution.F90, src/tool.F90).  The reference keeps every field on the
+ * host and calls FFTW-MPI 49-91 times per step; here the whole periodic
+ * domain stays resident in one GPU's HBM, and one call advances the field by
+ * whole iterations of the reference's main loop, diagnostics included.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative BASTR_E* code on failure; the message of
+ *     the last failure on the calling thread is bastr_last_error().  This is
+ *     the fail-fast contract of the reference's `stop '...'`
+ *     (fftwlink.F90:79,300,319,419): a Fortran caller turns a nonzero return
+ *     into `error stop bastr_last_error()`.
+ *   - real fields are Fortran column-major, x (index i) fastest:
+ *     u(i,j) at u[i + ia*j] (2-D), u(i,j,k) at u[i + ia*(j + ja*k)] (3-D),
+ *     exactly the layout of u1/u2/u3 in src/commvar.F90:27.
+ *   - calls are synchronous on return; one HIP stream per handle; a handle
+ *     is not thread-safe.  Device memory is owned by the library.
+ *   - no torch or HIP types appear in any signature.
+ */
+#ifndef BASTR_H
+#define BASTR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BASTR_OK            0
+#define BASTR_STOP_NAN      1   /* Es+Ed was NaN: the loop stopped like the reference's
+                                   `stop '"E" is a NaN'` (solution.F90:69-81) */
+#define BASTR_EINVAL      (-1)  /* bad argument / unsupported configuration        */
+#define BASTR_EDEVICE     (-2)  /* HIP runtime error                                */
+#define BASTR_ENOMEM      (-3)  /* device allocation failed                         */
+
+#define BASTR_PREC_F64      0   /* everything in FP64 (parity mode)                 */
+#define BASTR_PREC_F32      1   /* FFTs and pointwise work in FP32; energy/spectrum
+                                   reductions, forcing normalisation in FP64         */
+
+/*
+ * Run parameters.  Each field is the reference variable of the same meaning
+ * (src/commvar.F90:16-66, parsed from datin/input.dat by readinput,
+ * src/readwrite.F90:19-112).
+ */
+typedef struct bastr_params {
+    int32_t ia, ja, ka;        /* grid; ka == 0 selects 2-D (readwrite.F90:69-80)       */
+    double  reynolds;          /* nu = 1/reynolds (solution.F90:1306-1310; miucal(1)=1) */
+    double  deltat;            /* time step                                             */
+    int32_t timemethod;        /* 1 = Kutta RK3 (solution.F90:172), 2 = 6-sweep CN (:205) */
+    int32_t forcemethod;       /* 0 none, 1 rescale, 2 band, 5 projected band (:303-540) */
+    int32_t forcek;            /* forcing wavenumber (band 4.5<|k|<5.5 for forcek=5)    */
+    double  target_energy;     /* forcing target energy                                 */
+    int32_t lproject;          /* lprojectd: project tendency on dilatational part     */
+    int32_t kmax;              /* allkmax = min(int(ia/3), kmax) (readwrite.F90:75-83) */
+    int32_t precision;         /* BASTR_PREC_F64 / BASTR_PREC_F32                       */
+} bastr_params;
+
+/*
+ * Diagnostics of one loop iteration, in the reference's order
+ * (solution.F90:54-97): spectra and velocity-gradient statistics are taken
+ * from the field at `nstep` BEFORE that iteration's step; the forcing values
+ * come after the step and belong to log line nstep+1 (listwrite after
+ * `nstep = nstep + 1`).
+ */
+typedef struct bastr_diag {
+    int64_t nstep;             /* nstep of the pre-step diagnostics                      */
+    double  time;              /* time  of the pre-step diagnostics                      */
+    /* spectra_compute (solution.F90:930-1016): log/stat?d_specall.dat               */
+    double  Esspe, Edspe, k2Edspe, kMEdspe;
+    /* velgrad_calculate (solution.F90:1110-1286): log/stat?d_velgrad.dat            */
+    double  umumtheta2, umumijji, u2theta, dissp, eta_min, Taylength, ReTay, Kollength;
+    /* forcing (solution.F90:303-540): log/factor_output.dat (methods 1,2,5)         */
+    double  force_factor, force_E, force_energy, force_Fen;
+    int32_t nan_flag;          /* 1 when isnan(Esspe+Edspe)                              */
+    int32_t pad_;
+} bastr_diag;
+
+typedef struct bastr_handle bastr_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char* bastr_last_error(void);
+
+/*
+ * Create a solver: replaces mpisizedis_fftw + allocation + prepareplan_fftw_plan
+ * + quantity_prepare (fftwlink.F90:34,370,284; solution.F90:1288).
+ * Supported grids: ia, ja, ka powers of two in [16, 4096]; 3-D needs ia==ja==ka.
+ */
+int bastr_create(const bastr_params* p, bastr_handle** out);
+int bastr_destroy(bastr_handle* h);
+
+/* allkmax actually used (min(int(ia/3), kmax)). */
+int bastr_allkmax(const bastr_handle* h);
+
+/*
+ * Load the initial (or restart) velocity field: replaces read_initial_field /
+ * read_continue_field (readwrite.F90:114-220).  u3 may be NULL in 2-D.
+ * The next bastr_step first performs mainloop's entry dealias
+ * (solution.F90:41-52 / 105-122), as every reference run does.
+ */
+int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const double* u3);
+/* Physical velocity field (Fortran order) at the current nstep. */
+int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3);
+
+/* nstep/time of the next iteration (restart sets them from the dump file). */
+int bastr_set_time(bastr_handle* h, int64_t nstep, double time);
+int bastr_get_time(const bastr_handle* h, int64_t* nstep, double* time);
+
+/*
+ * Run `nsteps` iterations of the reference main loop body (solution.F90:54-97
+ * for 2-D, :124-169 for 3-D): diagnostics at nstep, NaN stop, nstep+=1,
+ * time+=deltat, RK3 (or CN) step, forcing.
+ *   diags   : NULL or nsteps entries.
+ *   spectra : NULL or nsteps * 3 * allkmax doubles; iteration s writes
+ *             kn[1..allkmax], Es[1..allkmax], Ed[1..allkmax] at
+ *             spectra + s*3*allkmax (+0, +allkmax, +2*allkmax).
+ * Returns BASTR_STOP_NAN when an iteration's Es+Ed is NaN: that iteration's
+ * diagnostics are filled, the field is left as it was before that
+ * iteration's step (the state the reference dumps before stopping) and
+ * nstep/time are those of that iteration.
+ */
+int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra);
+
+/* Number of modes per shell 1..allkmax (Ecount, solution.F90:976). */
+int bastr_shell_counts(const bastr_handle* h, int64_t* counts);
+
+/*
+ * fftwlink-compatible whole-field transforms on caller memory
+ * (fftwlink.F90:324-368): in-place c2c over (ia, ja[, ka]) complex128 values
+ * (interleaved re,im; Fortran order).  Forward is e^{-ik.x} followed by the
+ * 1/(ia*ja[*ka]) normalisation of fft2d/fft3d; backward is e^{+ik.x},
+ * unnormalised, as ifft2d/ifft3d.  Always FP64.
+ */
+int bastr_fft_forward(bastr_handle* h, double* inout);
+int bastr_fft_backward(bastr_handle* h, double* inout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BASTR_H */

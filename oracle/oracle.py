@@ -1,0 +1,51 @@
+"""TEST INFRASTRUCTURE ONLY: Python handle on the C oracle (oracle/liboracle.so).
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg.  The oracle exports the same entry points as the HIP library under the
+prefix "bo", so it is driven through the same bastr_amd.solver.Solver class.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from bastr_amd import abi
+from bastr_amd.solver import Solver
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} missing (run `make -C oracle`)")
+        L = C.CDLL(path)
+        abi.bind(L, "bo")
+        L.bo_fft_c2c.restype = C.c_int
+        L.bo_fft_c2c.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.bo_set_threads.argtypes = [C.c_int]
+        L.bo_get_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def solver(params):
+    return Solver(lib(), "bo", params)
+
+
+def set_threads(n):
+    lib().bo_set_threads(int(n))
+
+
+def fft_c2c(a, direction):
+    """Fortran-order c2c with fftwlink conventions: direction -1 forward (normalised), +1 backward."""
+    z = np.asfortranarray(np.asarray(a, dtype=np.complex128)).copy(order="F")
+    shp = z.shape
+    n2 = shp[2] if len(shp) == 3 else 0
+    rc = lib().bo_fft_c2c(shp[0], shp[1], n2, z.ctypes.data_as(C.POINTER(C.c_double)), int(direction))
+    if rc:
+        raise RuntimeError("bo_fft_c2c failed")
+    return z
