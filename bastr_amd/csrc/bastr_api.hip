@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/bastr.h"
+#include "../../include/bastr_prof.h"
 #include "k2d.hpp"
 
 using namespace k2d;
@@ -96,6 +97,7 @@ struct bastr_handle {
     double* part_bins = nullptr;
     double* part_tot = nullptr;
     double* part_vg = nullptr;
+    double* red = nullptr;
     int nblk_spec = 0, nblk_vg = 0;
     int* status = nullptr;
     bastr_diag* ddiag = nullptr;
@@ -106,9 +108,60 @@ struct bastr_handle {
     int64_t nstep = 0;
     double time = 0;
     bool prologue_pending = false, have_fields = false;
+    // per-kernel event timing (bastr_prof.h)
+    uint32_t prof_mask = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_pending;
+    int64_t prof_launch[BASTR_K_NCLASS] = {};
+    double prof_ms[BASTR_K_NCLASS] = {};
 };
 
 namespace {
+
+hipEvent_t pool_event(bastr_handle* h) {
+    if (h->ev_used == h->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        h->ev_pool.push_back(e);
+    }
+    return h->ev_pool[h->ev_used++];
+}
+
+// Bracket one launch of class `cls` with events when that class is enabled.
+struct ProfScope {
+    bastr_handle* h;
+    int cls;
+    bool on;
+    ProfScope(bastr_handle* h_, int c) : h(h_), cls(c), on((h_->prof_mask >> c) & 1u) {
+        if (on) {
+            hipEvent_t e = pool_event(h);
+            if (!e) { on = false; return; }
+            h->ev_pending.push_back({cls, h->ev_used - 1});
+            hipEventRecord(e, h->stream);
+        }
+    }
+    ~ProfScope() {
+        if (on) {
+            hipEvent_t e = pool_event(h);
+            if (e) hipEventRecord(e, h->stream);
+            else h->ev_pending.pop_back();
+        }
+    }
+};
+
+// After a stream synchronisation: fold the recorded pairs into the totals.
+void prof_collect(bastr_handle* h) {
+    for (auto& pc : h->ev_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, h->ev_pool[pc.second], h->ev_pool[pc.second + 1]) == hipSuccess) {
+            h->prof_launch[pc.first] += 1;
+            h->prof_ms[pc.first] += ms;
+        }
+    }
+    h->ev_pending.clear();
+    h->ev_used = 0;
+}
 
 template <typename T>
 void make_twiddles(int n, std::vector<cx<T>>& tw) {
@@ -215,7 +268,10 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     for (int q = 0; q < 4; ++q) ci.out[q] = C<T>(h->W[q]);
     ci.pitch = h->hp;
     ci.tw = C<T>(h->tw1);
-    if (stage != 0) LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE, ci, h->hp, h->stream));
+    if (stage != 0) {
+        ProfScope ps(h, BASTR_K_COL_INV);
+        LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE, ci, h->hp, h->stream));
+    }
     // (stage A's y-inverse ran in the diagnostic phase with the velgrad outputs)
     RowArgs<T> r{};
     r.g = geo<T>(h);
@@ -224,7 +280,10 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     r.hout[0] = C<T>(h->NX[0]);
     r.hout[1] = C<T>(h->NX[1]);
     r.tw = C<T>(h->tw0);
-    LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
+    {
+        ProfScope ps(h, BASTR_K_ROW_STAGE);
+        LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
+    }
     ColFwdArgs<T> cf{};
     cf.g = geo<T>(h);
     cf.gate = gstep;
@@ -237,7 +296,10 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     }
     cf.pitch = h->hp;
     cf.tw = C<T>(h->tw1);
-    LAUNCHCHK(launch_col_fwd<T>(h->n1, stage == 0 ? CF_A : (stage == 1 ? CF_B : CF_C), cf, h->hp, h->stream));
+    {
+        ProfScope ps(h, BASTR_K_COL_FWD_A + stage);
+        LAUNCHCHK(launch_col_fwd<T>(h->n1, stage == 0 ? CF_A : (stage == 1 ? CF_B : CF_C), cf, h->hp, h->stream));
+    }
     return 0;
 }
 
@@ -255,7 +317,10 @@ int run_iteration(bastr_handle* h, int it) {
     sa.allkmax = h->allkmax;
     sa.part_bins = h->part_bins;
     sa.part_tot = h->part_tot;
-    LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
+    {
+        ProfScope ps(h, BASTR_K_SPECTRA);
+        LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
+    }
     // velgrad_calculate2D: derivative fields shared with RK stage A
     ColInvArgs<T> ci{};
     ci.g = geo<T>(h);
@@ -265,14 +330,20 @@ int run_iteration(bastr_handle* h, int it) {
     for (int q = 0; q < 7; ++q) ci.out[q] = C<T>(h->W[q]);
     ci.pitch = h->hp;
     ci.tw = C<T>(h->tw1);
-    LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE_VG, ci, h->hp, h->stream));
+    {
+        ProfScope ps(h, BASTR_K_COL_INV_VG);
+        LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE_VG, ci, h->hp, h->stream));
+    }
     VgArgs<T> va{};
     va.g = geo<T>(h);
     va.gate = gdiag;
     for (int q = 0; q < 7; ++q) va.w[q] = C<T>(h->W[q]);
     va.part = h->part_vg;
     va.tw = C<T>(h->tw0);
-    LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg));
+    {
+        ProfScope ps(h, BASTR_K_VELGRAD);
+        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg));
+    }
     FinArgs fa{};
     fa.gate = gdiag;
     fa.nblk_spec = h->nblk_spec;
@@ -284,10 +355,14 @@ int run_iteration(bastr_handle* h, int it) {
     fa.part_bins = h->part_bins;
     fa.part_tot = h->part_tot;
     fa.part_vg = h->part_vg;
+    fa.red = h->red;
     fa.diag = h->ddiag + it;
     fa.spec = h->dspec + (size_t)it * 3 * h->allkmax;
     fa.status = h->status;
-    LAUNCHCHK(launch_finalize(fa, h->stream));
+    {
+        ProfScope ps(h, BASTR_K_FINALIZE);
+        LAUNCHCHK(launch_finalize(fa, h->stream));
+    }
     // RK32D (solution.F90:172-203)
     const void* Ua[2] = {h->U[0], h->U[1]};
     const void* Ub[2] = {h->UB[0], h->UB[1]};
@@ -305,6 +380,7 @@ int run_iteration(bastr_handle* h, int it) {
         f.forcek = h->p.forcek;
         f.target_energy = h->p.target_energy;
         f.diag = h->ddiag + it;
+        ProfScope ps(h, BASTR_K_FORCE);
         LAUNCHCHK(launch_force_band<T>(f, h->stream));
     }
     return 0;
@@ -340,6 +416,7 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     int status = INT_MAX;
     HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    prof_collect(h);
     const int done = status == INT_MAX ? nsteps : status + 1;  // iterations with diagnostics
     const int K = h->allkmax;
     if (diags) {
@@ -423,6 +500,7 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     h->part_tot = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * 4, &ok);
     h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8, &ok);
     h->status = (int*)dalloc(sizeof(int), &ok);
+    h->red = (double*)dalloc(sizeof(double) * (2 * (size_t)h->allkmax + 12), &ok);
     if (!ok) {
         bastr_destroy(h);
         return fail(BASTR_ENOMEM, "bastr_create: device allocation failed");
@@ -458,11 +536,12 @@ int bastr_destroy(bastr_handle* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     void* bufs[] = {h->U[0], h->U[1], h->UB[0], h->UB[1], h->KA[0], h->KA[1], h->NX[0], h->NX[1],
                     h->ureal[0], h->ureal[1], h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf,
-                    h->part_bins, h->part_tot, h->part_vg, h->status, h->ddiag, h->dspec};
+                    h->part_bins, h->part_tot, h->part_vg, h->red, h->status, h->ddiag, h->dspec};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (int q = 0; q < 7; ++q)
         if (h->W[q]) hipFree(h->W[q]);
+    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return BASTR_OK;
@@ -610,6 +689,33 @@ static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
 }
 
 int bastr_fft_forward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, false); }
+
+int bastr_prof_enable(bastr_handle* h, uint32_t mask) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_prof_enable: NULL handle");
+    h->prof_mask = mask;
+    for (int c = 0; c < BASTR_K_NCLASS; ++c) {
+        h->prof_launch[c] = 0;
+        h->prof_ms[c] = 0;
+    }
+    return BASTR_OK;
+}
+
+int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* total_ms, double* bytes) {
+    if (!h || c < 0 || c >= BASTR_K_NCLASS) return fail(BASTR_EINVAL, "bastr_prof_read: bad argument");
+    // compulsory traffic in units of one half-spectrum field (nh*n1 complex)
+    static const double units[BASTR_K_NCLASS] = {2, 9, 7, 0, 6, 6, 8, 10, 10, 0};
+    const double half = (double)h->nh * h->n1 * 2.0 * h->esz;
+    if (launches) *launches = h->prof_launch[c];
+    if (total_ms) *total_ms = h->prof_ms[c];
+    if (bytes) *bytes = units[c] * half;
+    return BASTR_OK;
+}
+
+const char* bastr_prof_name(int c) {
+    static const char* names[BASTR_K_NCLASS] = {"spectra", "col_inv_vg", "velgrad", "finalize", "col_inv",
+                                                "row_stage", "col_fwd_a", "col_fwd_b", "col_fwd_c", "force"};
+    return (c >= 0 && c < BASTR_K_NCLASS) ? names[c] : "";
+}
 int bastr_fft_backward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, true); }
 
 }  // extern "C"

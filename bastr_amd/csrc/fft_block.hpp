@@ -150,17 +150,27 @@ __device__ __forceinline__ void exchange(const LdsLine<T, true>& L, cx<T>* v, W 
     for (int r = 0; r < NV; ++r) v[r].y = L.base[ridx(r) * L.stride];
 }
 
-// twiddle v[r] *= w_N^{(b mod Ns) * r * N/(Ns*R)}, r = 1..R-1
+// twiddle v[r] *= w_N^{(b mod Ns) * r * N/(Ns*R)}, r = 1..R-1.  w^r is the
+// running product with w, refreshed from the table at every 4th power, so
+// each factor is at most three roundings away from a table value while only
+// two twiddles are live at a time.
 template <typename T, int N, int R, bool INV>
 __device__ __forceinline__ void twiddle(cx<T>* v, int b, int Ns, const cx<T>* __restrict__ tw) {
     if (Ns == 1) return;
     const int bm = b & (Ns - 1);
     const int step = bm * (N / (Ns * R));
+    cx<T> w1 = tw[step];
+    if (INV) w1.y = -w1.y;
+    cx<T> t = w1;
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-        cx<T> w = tw[step * r];
-        if (INV) w.y = -w.y;
-        v[r] = cmul(v[r], w);
+        if (r % 4 == 0) {
+            t = tw[step * r];
+            if (INV) t.y = -t.y;
+        } else if (r > 1) {
+            t = cmul(t, w1);
+        }
+        v[r] = cmul(v[r], t);
     }
 }
 

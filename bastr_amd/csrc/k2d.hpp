@@ -17,20 +17,22 @@
 
 namespace k2d {
 
-// Column-pass launch shape for (T, N1): E points per thread, CB columns per
-// workgroup, CB*N1/E threads, CB*N1*sizeof(T) bytes of LDS (split exchange).
+// Column-pass launch shape for (T, N1): CB adjacent columns per workgroup
+// (CB*sizeof(complex) >= 64 B contiguous per row), E points per thread,
+// CB*N1/E threads (<= 512 in FP64 so a thread may hold 256 VGPRs without
+// spilling), CB*N1*sizeof(T) bytes of LDS (split exchange, <= 64 KiB).
 template <typename T, int N>
 struct ColCfg {
     static constexpr int CBMIN = 64 / (int)sizeof(cx<T>);
-    static constexpr int E0 = 8;
-    static constexpr int CB0 = (256 / (N / E0)) > CBMIN ? ((256 / (N / E0)) > 16 ? 16 : (256 / (N / E0))) : CBMIN;
     static constexpr int LDS_CB = 65536 / (N * (int)sizeof(T));
-    static constexpr int CB1 = CB0 < LDS_CB ? CB0 : LDS_CB;
-    static constexpr int E = (CB1 * (N / E0) > 1024) ? 16 : E0;
-    static constexpr int CB = (CB1 * (N / E) > 1024) ? 1024 / (N / E) : CB1;
+    static constexpr int CBW = (256 / (N / 8)) > CBMIN ? (256 / (N / 8)) : CBMIN;
+    static constexpr int CB2 = CBW > 16 ? 16 : CBW;
+    static constexpr int CB = CB2 < LDS_CB ? CB2 : LDS_CB;
+    static constexpr int MAXT = sizeof(T) == 8 ? 512 : 1024;
+    static constexpr int E = (CB * (N / 8) <= MAXT) ? 8 : 16;
     static constexpr int P = N / E;
     static constexpr int THREADS = CB * P;
-    static_assert(CB >= 1 && THREADS <= 1024, "column config");
+    static_assert(CB >= 1 && THREADS <= MAXT && N >= E, "column config");
 };
 
 // Row-pass launch shape for (T, N0): L rows per workgroup.
@@ -135,6 +137,7 @@ struct FinArgs {
     const double* part_bins;
     const double* part_tot;
     const double* part_vg;
+    double* red;        // [2*allkmax + 12] reduced sums
     bastr_diag* diag;   // slot of this iteration
     double* spec;       // [3][allkmax] slot of this iteration (Es at +allkmax, Ed at +2*allkmax)
     int* status;

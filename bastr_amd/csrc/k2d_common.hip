@@ -5,32 +5,56 @@
 
 namespace k2d {
 
-// Reduce the per-workgroup partials of one iteration into its diagnostics
-// slot, in a fixed order (spectra_compute2D :991-1000, velgrad :1158-1180),
-// and raise the NaN stop (solution.F90:69-81).
-__global__ __launch_bounds__(256) void finalize_kernel(FinArgs a) {
+// Reduce the per-workgroup partials of one iteration (spectra_compute2D
+// :991-1000, velgrad :1158-1175).  One workgroup per quantity, fixed
+// summation order: thread t adds partials t, t+256, ..., then a fixed LDS
+// tree.  Quantities: Es[1..K], Ed[1..K], 4 spectral totals, 7 velgrad sums,
+// then the eta_min minimum.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
     if (a.gate.closed()) return;
-    const int K = a.allkmax, nb = K + 1;
-    const int tid = threadIdx.x;
-    for (int b = 1 + tid; b <= K; b += blockDim.x) {
-        double es = 0, ed = 0;
-        for (int w = 0; w < a.nblk_spec; ++w) {
-            es += a.part_bins[((size_t)w * nb + b) * 2 + 0];
-            ed += a.part_bins[((size_t)w * nb + b) * 2 + 1];
-        }
-        a.spec[K + b - 1] = es;
-        a.spec[2 * K + b - 1] = ed;
+    __shared__ double sh[256];
+    const int K = a.allkmax;
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const double* src;
+    int n;
+    bool is_min = false;
+    if (q < 2 * K) {
+        const int b = q % K + 1, c = q / K;  // Es bins then Ed bins
+        src = a.part_bins + ((size_t)b * 2 + c) * a.nblk_spec;
+        n = a.nblk_spec;
+    } else if (q < 2 * K + 4) {
+        src = a.part_tot + (size_t)(q - 2 * K) * a.nblk_spec;
+        n = a.nblk_spec;
+    } else {
+        const int v = q - 2 * K - 4;
+        src = a.part_vg + (size_t)v * a.nblk_vg;
+        n = a.nblk_vg;
+        is_min = v == 7;
     }
-    if (tid == 0) {
-        double t[4] = {0, 0, 0, 0};
-        for (int w = 0; w < a.nblk_spec; ++w)
-            for (int q = 0; q < 4; ++q) t[q] += a.part_tot[(size_t)w * 4 + q];
-        double v[7] = {0, 0, 0, 0, 0, 0, 0};
-        double eta = (double)(2.0f * 3.14f);
-        for (int w = 0; w < a.nblk_vg; ++w) {
-            for (int q = 0; q < 7; ++q) v[q] += a.part_vg[(size_t)w * 8 + q];
-            eta = fmin(eta, a.part_vg[(size_t)w * 8 + 7]);
-        }
+    double acc = is_min ? (double)(2.0f * 3.14f) : 0.0;
+    for (int w = tid; w < n; w += 256) acc = is_min ? fmin(acc, src[w]) : acc + src[w];
+    sh[tid] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] = is_min ? fmin(sh[tid], sh[tid + o]) : sh[tid] + sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) a.red[q] = sh[0];
+}
+
+// Diagnostics of the iteration from the reduced sums; NaN stop
+// (solution.F90:69-81).
+__global__ void finalize_kernel(FinArgs a) {
+    if (a.gate.closed()) return;
+    const int K = a.allkmax;
+    const double* r = a.red;
+    for (int b = threadIdx.x; b < K; b += blockDim.x) {
+        a.spec[K + b] = r[b];
+        a.spec[2 * K + b] = r[K + b];
+    }
+    if (threadIdx.x == 0) {
+        const double* t = r + 2 * K;
+        const double* v = r + 2 * K + 4;
         const double N = a.npts, nu = a.nu;
         const double dim = a.ndims == 2 ? 2.0 : 3.0;
         const double urms = sqrt(v[4] / N), dudx2 = v[5] / N, eps = v[6] / N;
@@ -45,7 +69,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinArgs a) {
         d->umumijji = v[1] / N;
         d->u2theta = v[2] / N;
         d->dissp = v[3] / N;
-        d->eta_min = eta;
+        d->eta_min = v[7];
         d->Taylength = tay;
         d->ReTay = ufluc * tay / nu;
         d->Kollength = sqrt(sqrt(nu * nu * nu / eps));
@@ -57,6 +81,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinArgs a) {
 }
 
 int launch_finalize(const FinArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(2 * a.allkmax + 12), dim3(256), 0, s, a);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, a);
     return 0;
 }

@@ -20,6 +20,19 @@ namespace k2d {
 
 __device__ __forceinline__ int ksigned(int m, int n) { return m <= n / 2 ? m : m - n; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each XCD with its own L2.  Column blocks share 128-byte lines with
+// their neighbours, so map consecutive column blocks to the same XCD:
+// XCD x handles the contiguous range [x*nb/8, (x+1)*nb/8).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b & 7, q = b >> 3;
+    const int lo = (int)(((long)nb * x) >> 3);
+    const int hi = (int)(((long)nb * (x + 1)) >> 3);
+    const int v = lo + q;
+    return (v < hi) ? v : -1;  // -1: this slot has no block (only when nb % 8 != 0)
+}
+__device__ __forceinline__ int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
+
 // z = a + i b for packed position m of a length-N row, from half rows A, B
 // (kx = 0..N/2).  DC and Nyquist take real parts (the real part of a c2c
 // inverse equals the c2r inverse of the Hermitian part).
@@ -73,29 +86,31 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_inv_kernel(ColInv
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * N];
+    const int blk = xcd_block(blockIdx.x, a.pitch / CB);
+    if (blk < 0) return;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
-    const int col = blockIdx.x * CB + l;
+    const int col = blk * CB + l;
     const LdsLine<T, true> L{lds + l, CB};
     const size_t pitch = (size_t)a.pitch;
     const int nf = MODE == CI_PLAIN ? a.nfield : 2;
     for (int f = 0; f < nf; ++f) {
-        cx<T> u[E];
         const cx<T>* __restrict__ src = a.in[f];
-#pragma unroll
-        for (int k = 0; k < E; ++k) u[k] = src[(size_t)(j + k * P) * pitch + col];
         const int njob = MODE == CI_PLAIN ? 1 : (MODE == CI_STAGE ? 2 : (f == 0 ? 3 : 4));
         for (int job = 0; job < njob; ++job) {
+            // the column is re-read per job (an L2 hit after the first) rather
+            // than kept in registers across the transforms
             cx<T> v[E];
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int m = j + k * P;
+                const cx<T> u = src[(size_t)m * pitch + col];
                 const T ky = (T)ksigned(m, N);
                 const T kyo = (m == N / 2) ? (T)0 : ky;  // odd powers vanish at the Nyquist row
-                if (job == 0) v[k] = u[k];
-                else if (job == 1) v[k] = cscale(cmuli(u[k]), kyo);                 // i k2 u
-                else if (job == 2) v[k] = cscale(u[k], -(ky * ky));                 // -k2^2 u
-                else v[k] = cscale(cmuli(u[k]), -(kyo * kyo * kyo));                 // -i k2^3 u
+                if (job == 0) v[k] = u;
+                else if (job == 1) v[k] = cscale(cmuli(u), kyo);                 // i k2 u
+                else if (job == 2) v[k] = cscale(u, -(ky * ky));                 // -k2^2 u
+                else v[k] = cscale(cmuli(u), -(kyo * kyo * kyo));                 // -i k2^3 u
             }
             line_fft<T, N, E, true, true>(v, j, L, a.tw);
             cx<T>* __restrict__ dst;
@@ -119,9 +134,11 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * N];
+    const int blk = xcd_block(blockIdx.x, a.pitch / CB);
+    if (blk < 0) return;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
-    const int col = blockIdx.x * CB + l;
+    const int col = blk * CB + l;
     const LdsLine<T, true> L{lds + l, CB};
     const size_t pitch = (size_t)a.pitch;
     const T sc = (T)a.g.inv_n;
@@ -150,6 +167,9 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
     const T dt = (T)a.g.dt;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
+        // groups of 4 modes: keeps the compiler from hoisting every state load
+        // of the thread ahead of the arithmetic (register pressure)
+        if (k % 4 == 0) __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int ky = ksigned(m, N);
         const int kk2 = kx * kx + ky * ky;
@@ -399,7 +419,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void velgrad_kernel(VgArgs
     if (tid < 8) {
         double acc = tid == 7 ? (double)(2.0f * 3.14f) : 0.0;
         for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
-        a.part[(size_t)blockIdx.x * 8 + tid] = acc;
+        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;  // [quantity][block]
     }
 }
 
@@ -465,17 +485,18 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     if (lane == 0)
         for (int q = 0; q < 4; ++q) tots[wave * 4 + q] = tot[q];
     __syncthreads();
+    // partials are stored [quantity][block] so the reduction reads them coalesced
     for (int b = tid; b < nb; b += 256) {
         for (int c = 0; c < 2; ++c) {
             double acc = 0;
             for (int wv = 0; wv < 4; ++wv) acc += bins[(wv * nb + b) * 2 + c];
-            a.part_bins[((size_t)blockIdx.x * nb + b) * 2 + c] = acc;
+            a.part_bins[((size_t)b * 2 + c) * gridDim.x + blockIdx.x] = acc;
         }
     }
     if (tid < 4) {
         double acc = 0;
         for (int wv = 0; wv < 4; ++wv) acc += tots[wv * 4 + tid];
-        a.part_tot[(size_t)blockIdx.x * 4 + tid] = acc;
+        a.part_tot[(size_t)tid * gridDim.x + blockIdx.x] = acc;
     }
 }
 
@@ -534,13 +555,15 @@ __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static inline int xcd_grid_h(int nb) { return ((nb + 7) / 8) * 8; }
+
 #define K2D_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 template <typename T, int N>
 static int col_inv_n(int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s) {
     using C = ColCfg<T, N>;
     if (ncols % C::CB) return -1;
-    dim3 grid(ncols / C::CB), block(C::THREADS);
+    dim3 grid(xcd_grid_h(ncols / C::CB)), block(C::THREADS);
     switch (mode) {
         case CI_PLAIN: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_PLAIN>), grid, block, 0, s, a); break;
         case CI_STAGE: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_STAGE>), grid, block, 0, s, a); break;
@@ -554,7 +577,7 @@ template <typename T, int N>
 static int col_fwd_n(int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s) {
     using C = ColCfg<T, N>;
     if (ncols % C::CB) return -1;
-    dim3 grid(ncols / C::CB), block(C::THREADS);
+    dim3 grid(xcd_grid_h(ncols / C::CB)), block(C::THREADS);
     switch (mode) {
         case CF_PLAIN: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PLAIN>), grid, block, 0, s, a); break;
         case CF_PROLOGUE: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PROLOGUE>), grid, block, 0, s, a); break;

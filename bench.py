@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the Bastr pseudo-spectral time step on one MI355X.
+
+Workload (default): 2-D viscous Burgers on a 2048^2 periodic grid, FP64,
+Re = 5000, dt = 1e-4, Kutta RK3, band forcing at forcek = 5 with target
+energy 0.075, dilatational projection on, spectra to kmax = 200; initial
+field: seeded random-phase spectrum E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1.
+A "step" is one iteration of the reference main loop (solution.F90:54-97):
+spectra + velocity-gradient diagnostics, RK3 step, forcing.  The field stays
+resident in HBM for the whole timed region; bastr_step is called in chunks
+of 200 iterations (the spectra cadence of the configuration) and returns the
+per-iteration diagnostics to the host, as a log-writing driver would.
+
+Prints one JSON line.  `value` = grid points x steps / wall seconds.
+`roofline` is the dominant kernel: its compulsory HBM bytes per launch over
+its mean duration, timed with HIP events on the library's stream during the
+timed region.  `cpu_baseline` is the FP64 CPU restatement (oracle/) timed on
+this host's cores over a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "grid-pt·steps/s at 2048² & 512³ (FP64 vs FP32), % HBM roofline, E(k) err"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: (ia, ja, precision, forcemethod, lproject, kmax)
+    "c2": (2048, 2048, 0, 2, True, 200),
+    "c2-fp32": (2048, 2048, 1, 2, True, 200),
+    "c1": (256, 256, 0, 0, False, 128),
+}
+
+
+def prof_api(lib):
+    lib.bastr_prof_enable.argtypes = [C.c_void_p, C.c_uint32]
+    lib.bastr_prof_enable.restype = C.c_int
+    lib.bastr_prof_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+    lib.bastr_prof_read.restype = C.c_int
+    lib.bastr_prof_name.argtypes = [C.c_int]
+    lib.bastr_prof_name.restype = C.c_char_p
+
+
+def read_prof(lib, h, ncls=10):
+    out = {}
+    for c in range(ncls):
+        n = C.c_int64()
+        ms = C.c_double()
+        by = C.c_double()
+        lib.bastr_prof_read(h, c, C.byref(n), C.byref(ms), C.byref(by))
+        out[lib.bastr_prof_name(c).decode()] = (c, n.value, ms.value, by.value)
+    return out
+
+
+# kernel class -> symbol template of the kernel it launches (2-D path)
+KCLASS_SYMBOL = {
+    "row_stage": "void k2d::row_kernel<{T}, {N0}, 2>(",
+    "col_inv": "void k2d::col_inv_kernel<{T}, {N1}, 1>(",
+    "col_inv_vg": "void k2d::col_inv_kernel<{T}, {N1}, 2>(",
+    "velgrad": "void k2d::velgrad_kernel<{T}, {N0}>(",
+    "col_fwd_a": "void k2d::col_fwd_kernel<{T}, {N1}, 2>(",
+    "col_fwd_b": "void k2d::col_fwd_kernel<{T}, {N1}, 3>(",
+    "col_fwd_c": "void k2d::col_fwd_kernel<{T}, {N1}, 4>(",
+    "spectra": "void k2d::spectra_kernel<{T}>(",
+}
+
+
+def traffic_from_profile(kclass, prec, ia, ja):
+    """HBM bytes per launch of the kernel from the committed counter summary
+    (profiles/hbm_traffic_2d_<n>_<prec>.txt), or None if not profiled."""
+    T = "float" if prec == 1 else "double"
+    tag = f"2d_{ia}_{'fp32' if prec == 1 else 'fp64'}" if ia == ja else None
+    sym = KCLASS_SYMBOL.get(kclass)
+    if tag is None or sym is None:
+        return None
+    sym = sym.format(T=T, N0=ia, N1=ja)
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(sym[:60]):
+            parts = line.split()
+            try:
+                return float(parts[-3]) * 1e6  # total_MB column
+            except (ValueError, IndexError):
+                return None
+    return None
+
+
+def run_steps(s, n, chunk):
+    done = 0
+    while done < n:
+        k = min(chunk, n - done)
+        rc, _, _ = s.step(k, want_spectra=True)
+        if rc != 0:
+            raise RuntimeError(f"bastr_step returned {rc}")
+        done += k
+
+
+def cpu_baseline(params, fields, sample_steps):
+    from oracle import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = min(threads, 16)
+    oracle.set_threads(threads)
+    o = oracle.solver(params)
+    o.set_fields(*fields)
+    o.step(1)  # prologue + one iteration (untimed)
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    dt = time.perf_counter() - t0
+    npts = params.ia * params.ja * max(params.ka, 1)
+    return npts * sample_steps / dt, threads, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
+              "MI355X's HBM); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import bastr_amd
+    from bastr_amd.fields import hitgen
+    from bastr_amd.solver import make_params
+
+    ia, ja, prec, force, proj, kmax = CONFIGS[args.config]
+    params = make_params(ia, ja, 0, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
+                         target_energy=0.075, lproject=proj, kmax=kmax, precision=prec)
+    fields = hitgen((ia, ja), k0=4.0, seed=1, urms=1.0)
+    lib = bastr_amd.load()
+    prof_api(lib)
+    s = bastr_amd.solver(params)
+    s.set_fields(*fields)
+
+    # warmup, with every kernel class timed to find the dominant one
+    lib.bastr_prof_enable(s._h, 0x3FF)
+    run_steps(s, max(args.warmup, 1), args.chunk)
+    warm = read_prof(lib, s._h)
+    dom = max(warm.items(), key=lambda kv: kv[1][2])
+    dom_name, (dom_cls, _, _, _) = dom
+    for name, (c, n, ms, by) in sorted(warm.items(), key=lambda kv: -kv[1][2]):
+        if n:
+            print(f"  warmup {name:12s} launches={n:6d} mean={ms / n * 1e3:9.2f} us "
+                  f"bytes/launch={by / 1e6:8.2f} MB  {by / (ms / n * 1e-3) / 1e9:8.1f} GB/s", file=sys.stderr)
+
+    # timed region: only the dominant kernel is bracketed by events
+    lib.bastr_prof_enable(s._h, 1 << dom_cls)
+    t0 = time.perf_counter()
+    run_steps(s, args.steps, args.chunk)
+    wall = time.perf_counter() - t0
+    prof = read_prof(lib, s._h)
+    _, n_l, ms_l, by_l = prof[dom_name]
+    mean_s = ms_l / n_l * 1e-3
+    achieved = by_l / mean_s / 1e9
+
+    npts = ia * ja
+    value = npts * args.steps / wall
+    rd = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, ia, ja),
+          "kernel": dom_name,
+          "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)}
+    cpu = None
+    if not args.no_cpu:
+        p64 = make_params(ia, ja, 0, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
+                          target_energy=0.075, lproject=proj, kmax=kmax, precision=0)
+        v, thr, dt = cpu_baseline(p64, fields, args.cpu_steps)
+        cpu = {"value": v, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
+               "sample": f"{args.cpu_steps} loop iterations of the same {ia}x{ja} FP64 workload "
+                         f"({dt:.1f} s), FP64 C restatement with its own radix-2 FFT, OpenMP"}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == 1 else "f64",
+        "data": "synthetic: seeded random-phase field, E(k)~k^4 exp(-2(k/4)^2), u_rms=1",
+        "config": {"workload": f"2-D viscous Burgers {ia}x{ja}, Re=5000, dt=1e-4, RK3, "
+                               f"{'band forcing k=5 E_t=0.075' if force == 2 else 'free decay'}, "
+                               f"lproject={'t' if proj else 'f'}, spectra kmax={kmax}",
+                   "grid": [ia, ja], "precision": "fp32" if prec == 1 else "fp64"},
+        "roofline": rd,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

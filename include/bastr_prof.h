@@ -1,0 +1,44 @@
+/*
+ * bastr_prof.h -- per-kernel timing of a bastr handle (no counterpart in the
+ * reference, which has no timers beyond an unused MPI_Wtime,
+ * parallel.F90:2658-2664).  For benchmarks: when a kernel class is enabled,
+ * every launch of it is bracketed by HIP events on the handle's stream and
+ * its elapsed time is accumulated at the end of each bastr_step call.
+ */
+#ifndef BASTR_PROF_H
+#define BASTR_PROF_H
+
+#include <stdint.h>
+#include "bastr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bastr_kclass {
+    BASTR_K_SPECTRA = 0,     /* shell spectra partial sums                     */
+    BASTR_K_COL_INV_VG,      /* y-inverse of stage A + velgrad fields           */
+    BASTR_K_VELGRAD,         /* x-inverse + velocity-gradient moments           */
+    BASTR_K_FINALIZE,        /* diagnostics reduction                           */
+    BASTR_K_COL_INV,         /* y-inverse of stages B, C                        */
+    BASTR_K_ROW_STAGE,       /* x-inverse, nonlinear term, x-forward            */
+    BASTR_K_COL_FWD_A,       /* y-forward + RK stage A update                   */
+    BASTR_K_COL_FWD_B,
+    BASTR_K_COL_FWD_C,
+    BASTR_K_FORCE,           /* band forcing                                    */
+    BASTR_K_NCLASS
+};
+
+/* Enable event timing for the classes in mask (bit c = class c); 0 disables.
+ * Resets the accumulated counters. */
+int bastr_prof_enable(bastr_handle* h, uint32_t mask);
+/* Launch count, summed kernel time (ms) and compulsory HBM bytes per launch
+ * (every array element the kernel must read or write, counted once). */
+int bastr_prof_read(const bastr_handle* h, int kclass, int64_t* launches, double* total_ms,
+                    double* bytes_per_launch);
+const char* bastr_prof_name(int kclass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+"""Pinning the CPU oracle (oracle/bastr_oracle.c) with checks that do not
+restate its own formulas: a brute-force DFT, numpy.fft (pocketfft, an
+independent library), exact lattice counts, closed-form solutions of the
+reference scheme, and an independent numpy implementation of the
+diagnostics built on numpy.fft.
+
+The reference publishes no golden vectors or tests (its repository has none),
+so these are the pins; each test names the reference lines it pins.
+"""
+import numpy as np
+import pytest
+
+from bastr_amd import abi
+from bastr_amd.fields import hitgen
+from bastr_amd.solver import diag_table, make_params
+
+
+def kgrid(shape):
+    ks = []
+    for n in shape:
+        k = np.arange(n, dtype=np.float64)
+        k[n // 2 + 1:] -= n  # GenerateWave: Nyquist index carries +n/2 (tool.F90:31-37)
+        ks.append(k)
+    return np.meshgrid(*ks, indexing="ij")
+
+
+def brute_dft(a, sign):
+    out = a.astype(np.complex128)
+    for ax, n in enumerate(a.shape):
+        m = np.arange(n)
+        W = np.exp(sign * 2j * np.pi * np.outer(m, m) / n)
+        out = np.moveaxis(np.tensordot(W, np.moveaxis(out, ax, 0), axes=(1, 0)), 0, ax)
+    return out
+
+
+# --------------------------------------------------------------------------
+# transforms (fftwlink.F90:324-368)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(4, 4), (8, 2), (6, 10), (16, 8), (5, 3, 4)])
+def test_fft_matches_brute_force_dft(orc, shape):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f = orc.fft_c2c(a, -1)
+    assert np.allclose(f, brute_dft(a, -1) / a.size, rtol=0, atol=1e-14 * np.abs(a).max())
+    b = orc.fft_c2c(a, +1)
+    assert np.allclose(b, brute_dft(a, +1), rtol=0, atol=1e-13 * np.abs(a).max() * a.size)
+
+
+@pytest.mark.parametrize("shape", [(256, 128), (64, 64, 32)])
+def test_fft_matches_numpy(orc, shape):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f = orc.fft_c2c(a, -1)
+    assert np.abs(f - np.fft.fftn(a) / a.size).max() < 1e-15 * np.abs(a).max() * 10
+    assert np.abs(orc.fft_c2c(f, +1) - a).max() < 1e-13
+
+
+def test_fft_single_mode_closed_form(orc):
+    n = 32
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    a = np.cos(3 * X - 5 * Y)
+    f = orc.fft_c2c(a, -1)
+    expect = np.zeros((n, n), complex)
+    expect[3, -5] = 0.5
+    expect[-3, 5] = 0.5
+    assert np.abs(f - expect).max() < 1e-15
+
+
+# --------------------------------------------------------------------------
+# shells and dealiasing (tool.F90:126-217, readwrite.F90:75-83)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("shape,kmax", [((64, 64), 128), ((128, 64), 10), ((16, 16, 16), 128)])
+def test_shell_counts_are_lattice_counts(orc, shape, kmax):
+    nd = len(shape)
+    p = make_params(*shape, kmax=kmax) if nd == 2 else make_params(*shape[:2], ka=shape[2], kmax=kmax)
+    s = orc.solver(p)
+    K = min(shape[0] // 3, kmax)
+    assert s.allkmax == K
+    ks = kgrid(shape)
+    kk = np.sqrt(sum(k * k for k in ks))
+    shell = np.floor(kk + 0.5).astype(int)
+    expect = np.array([(shell == b).sum() for b in range(1, K + 1)])
+    assert np.array_equal(s.shell_counts(), expect)
+
+
+def test_entry_dealias_removes_exactly_the_spherical_cutoff(orc):
+    # cutoff floor(ia/3) uses ia even for non-square grids (tool.F90:185-191)
+    shape = (48, 96)
+    rng = np.random.default_rng(2)
+    u = [np.asfortranarray(rng.standard_normal(shape)) for _ in range(2)]
+    s = orc.solver(make_params(*shape))
+    s.set_fields(*u)
+    v = s.get_fields()
+    k1, k2 = kgrid(shape)
+    keep = np.sqrt(k1 ** 2 + k2 ** 2) < 48 // 3
+    for a, b in zip(u, v):
+        expect = np.real(np.fft.ifft2(np.fft.fft2(a) * keep))
+        assert np.abs(b - expect).max() < 1e-13
+
+
+# --------------------------------------------------------------------------
+# time stepping: closed forms (solution.F90:172-231, 816-863)
+# --------------------------------------------------------------------------
+def shear_flow(n, modes):
+    y = np.arange(n) * 2 * np.pi / n
+    f = np.zeros(n)
+    for m, a, ph in modes:
+        f += a * np.sin(m * y + ph)
+    u1 = np.asfortranarray(np.tile(f, (n, 1)))  # u1(i, j) = f(y_j)
+    return u1, np.zeros((n, n), order="F")
+
+
+def rk3_gain(z):
+    return 1 + z + z * z / 2 + z ** 3 / 6
+
+
+def cn_gain(z):
+    # u^(1) = u0 + z u0; u^(m) = u0 + z (u^(m-1) + u0)/2, six evaluations
+    g = 1 + z
+    for _ in range(5):
+        g = 1 + z * (g + 1) / 2
+    return g
+
+
+@pytest.mark.parametrize("timemethod,gain", [(1, rk3_gain), (2, cn_gain)])
+def test_shear_flow_decays_by_exact_scheme_gain(orc, timemethod, gain):
+    """u = (f(y), 0): the nonlinear term vanishes identically, so each Fourier
+    mode m is multiplied by the scheme's gain at z = -nu m^2 dt per step."""
+    n, nsteps, re, dt = 32, 25, 50.0, 1e-2
+    modes = [(1, 1.0, 0.3), (3, 0.5, 1.1), (7, 0.2, -0.4)]
+    u1, u2 = shear_flow(n, modes)
+    p = make_params(n, n, reynolds=re, deltat=dt, timemethod=timemethod, kmax=128)
+    s = orc.solver(p)
+    s.set_fields(u1, u2)
+    rc, d, _ = s.step(nsteps)
+    assert rc == 0
+    y = np.arange(n) * 2 * np.pi / n
+    f = sum(a * gain(-(m * m) * dt / re) ** nsteps * np.sin(m * y + ph) for m, a, ph in modes)
+    v1, v2 = s.get_fields()
+    assert np.abs(v1 - f[None, :]).max() < 1e-13
+    assert np.abs(v2).max() < 1e-13
+    # loop bookkeeping: nstep and time accumulate by one deltat per iteration
+    assert s.get_time()[0] == nsteps
+    assert [x.nstep for x in d] == list(range(nsteps))
+    # the shear flow is solenoidal: Es = 1/2 <u1^2>, Ed = 0 (Parseval)
+    f0 = sum(a * np.sin(m * y + ph) for m, a, ph in modes)
+    assert abs(d[0].Esspe - 0.5 * np.mean(f0 ** 2)) < 1e-14
+    assert abs(d[0].Edspe) < 1e-28
+
+
+def test_projection_freezes_solenoidal_field(orc):
+    """lproject=t keeps only the dilatational part of the tendency; a solenoidal
+    field has a solenoidal tendency, so it does not move (tool.F90:219-243)."""
+    n = 32
+    u1, u2 = shear_flow(n, [(2, 1.0, 0.0), (5, 0.3, 0.7)])
+    s = orc.solver(make_params(n, n, lproject=True, reynolds=100.0, deltat=1e-3))
+    s.set_fields(u1, u2)
+    s.step(10)
+    v1, v2 = s.get_fields()
+    assert np.abs(v1 - u1).max() < 1e-14 and np.abs(v2).max() < 1e-14
+
+
+def test_potential_flow_tendency(orc):
+    """u = grad(sin x sin y): u_j d_j u_i = d_i |u|^2/2, so the RHS is
+    -grad(|u|^2/2) + nu Lap(u); one tiny step gives u(dt) - u(0) = dt F + O(dt^2)."""
+    n, dt, re = 64, 1e-7, 10.0
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    u1 = np.cos(X) * np.sin(Y)
+    u2 = np.sin(X) * np.cos(Y)
+    # |u|^2/2 = (cos^2 x sin^2 y + sin^2 x cos^2 y)/2
+    dq1 = -np.cos(X) * np.sin(X) * np.sin(Y) ** 2 + np.sin(X) * np.cos(X) * np.cos(Y) ** 2
+    dq2 = np.cos(X) ** 2 * np.sin(Y) * np.cos(Y) - np.sin(X) ** 2 * np.cos(Y) * np.sin(Y)
+    F1 = -dq1 + (1 / re) * (-2 * u1)
+    F2 = -dq2 + (1 / re) * (-2 * u2)
+    s = orc.solver(make_params(n, n, reynolds=re, deltat=dt))
+    s.set_fields(np.asfortranarray(u1), np.asfortranarray(u2))
+    s.step(1)
+    v1, v2 = s.get_fields()
+    assert np.abs((v1 - u1) / dt - F1).max() < 1e-5
+    assert np.abs((v2 - u2) / dt - F2).max() < 1e-5
+
+
+# --------------------------------------------------------------------------
+# diagnostics against an independent numpy implementation
+# --------------------------------------------------------------------------
+def numpy_diagnostics(u, nu, K):
+    """spectra_compute2D and velgrad_calculate2D (solution.F90:930-1186)
+    written independently on numpy.fft."""
+    shape = u[0].shape
+    N = float(np.prod(shape))
+    k1, k2 = kgrid(shape)
+    kk = np.sqrt(k1 ** 2 + k2 ** 2)
+    uh = [np.fft.fft2(x) / N for x in u]
+    safe = np.where(kk > 0.5, kk, 1.0)
+    us = np.where(kk > 0.5, (uh[0] * k2 - uh[1] * k1) / safe, 0)
+    ud = np.where(kk > 0.5, (uh[0] * k1 + uh[1] * k2) / safe, 0)
+    es = np.abs(us) ** 2 / 2
+    ed = np.abs(ud) ** 2 / 2
+    shell = np.floor(kk + 0.5).astype(int)
+    Es = np.array([es[shell == b].sum() for b in range(1, K + 1)])
+    Ed = np.array([ed[shell == b].sum() for b in range(1, K + 1)])
+    kn = np.array([kk[shell == b].mean() for b in range(1, K + 1)])
+    spec = dict(Esspe=es.sum(), Edspe=ed.sum(), k2Edspe=(ed * kk ** 2).sum(),
+                kMEdspe=np.where(kk > 0.5, ed / safe, 0).sum())
+
+    def der(f, k):
+        return np.real(np.fft.ifft2(1j * k * np.fft.fft2(f)))
+
+    a11, a12 = der(u[0], k1), der(u[0], k2)
+    a21, a22 = der(u[1], k1), der(u[1], k2)
+    lap = [np.real(np.fft.ifft2(-(kk ** 2) * np.fft.fft2(f))) for f in u]
+    div = a11 + a22
+    lth = np.real(np.fft.ifft2(-(kk ** 2) * np.fft.fft2(div)))
+    uu = u[0] ** 2 + u[1] ** 2
+    urms = np.sqrt(uu.mean())
+    dudx2 = (a11 ** 2 + a22 ** 2).mean()
+    eps = (nu * (a12 - a21) ** 2 + nu * div ** 2).mean()
+    ufl = urms / np.sqrt(2)
+    tay = ufl / np.sqrt(dudx2 / 2)
+    with np.errstate(divide="ignore"):
+        eta = min(np.float64(np.float32(2) * np.float32(3.14)), np.sqrt(nu / np.abs(div)).min())
+    vg = dict(umumtheta2=(div ** 2 * uu).mean(), umumijji=((a11 ** 2 + a22 ** 2 + 2 * a12 * a21) * uu).mean(),
+              u2theta=(div * uu).mean(),
+              dissp=(2 * u[0] * div * lap[0] + 2 * u[1] * div * lap[1] + uu * lth).mean(),
+              eta_min=eta, Taylength=tay, ReTay=ufl * tay / nu, Kollength=(nu ** 3 / eps) ** 0.25)
+    return spec, vg, np.stack([kn, Es, Ed])
+
+
+def test_diagnostics_match_independent_numpy(orc):
+    shape = (64, 96)
+    d = hitgen(shape, k0=6, seed=4)
+    r = hitgen(shape, k0=3, seed=5)
+    u = [np.asfortranarray(d[0] - 0.7 * r[1]), np.asfortranarray(d[1] + 0.7 * r[0])]
+    p = make_params(*shape, reynolds=300.0, kmax=18)
+    s = orc.solver(p)
+    s.set_fields(*u)
+    u = s.get_fields()  # entry-dealiased field, the one the diagnostics see
+    rc, dg, sp = s.step(1)
+    spec, vg, spx = numpy_diagnostics(u, 1 / 300.0, 18)
+    for k, v in {**spec, **vg}.items():
+        assert abs(getattr(dg[0], k) - v) <= 1e-12 * abs(v) + 1e-16, (k, getattr(dg[0], k), v)
+    assert np.abs(sp[0] - spx).max() < 1e-12 * np.abs(spx).max()
+
+
+def test_velgrad_single_mode_closed_form(orc):
+    """u = (A sin x, 0): div = A cos x; the moments are A^4/8, A^4/8, 0, 0,
+    eta_min = sqrt(nu/A), Taylor length 1, Re_lambda = A/(2 nu),
+    Kolmogorov length (2 nu^2 / A^2)^(1/4)."""
+    n, A, re = 32, 1.7, 200.0
+    nu = 1 / re
+    x = np.arange(n) * 2 * np.pi / n
+    u1 = np.asfortranarray(np.tile(A * np.sin(x)[:, None], (1, n)))
+    s = orc.solver(make_params(n, n, reynolds=re))
+    s.set_fields(u1, np.zeros((n, n), order="F"))
+    _, d, _ = s.step(1)
+    g = d[0]
+    assert abs(g.umumtheta2 - A ** 4 / 8) < 1e-13
+    assert abs(g.umumijji - A ** 4 / 8) < 1e-13
+    assert abs(g.u2theta) < 1e-14 and abs(g.dissp) < 1e-13
+    assert abs(g.eta_min - np.sqrt(nu / A)) < 1e-14
+    assert abs(g.Taylength - 1) < 1e-14
+    assert abs(g.ReTay - A / (2 * nu)) < 1e-10
+    assert abs(g.Kollength - (2 * nu ** 2 / A ** 2) ** 0.25) < 1e-14
+    # Parseval on the dilatational shell: Ed(1) = <u1^2>/2 = A^2/4
+    assert abs(g.Edspe - A ** 2 / 4) < 1e-14 and abs(g.Esspe) < 1e-30
+
+
+def test_band_forcing_against_numpy(orc):
+    """forcing2D method 2 (solution.F90:336-378): f = band-pass(u) over the open
+    band 4.5 < |k| < 5.5, E = <f1^2 + f2^2> (no 1/2), u += (sqrt(E_t/E)-1) f."""
+    shape = (64, 64)
+    u = hitgen(shape, k0=5, seed=8)
+    base = dict(reynolds=1000.0, deltat=1e-3, forcek=5, target_energy=0.075)
+    s0 = orc.solver(make_params(*shape, forcemethod=0, **base))
+    s2 = orc.solver(make_params(*shape, forcemethod=2, **base))
+    s0.set_fields(*u)
+    s2.set_fields(*u)
+    s0.step(1)
+    _, d2, _ = s2.step(1)
+    v = s0.get_fields()
+    k1, k2 = kgrid(shape)
+    kk = np.sqrt(k1 ** 2 + k2 ** 2)
+    band = (kk - 0.5 < 5) & (kk + 0.5 > 5)
+    f = [np.real(np.fft.ifft2(np.fft.fft2(x) * band)) for x in v]
+    E = np.mean(f[0] ** 2 + f[1] ** 2)
+    factor = np.sqrt(0.075 / E) - 1
+    w = s2.get_fields()
+    assert abs(d2[0].force_E - E) < 1e-13 * E
+    assert abs(d2[0].force_factor - factor) < 1e-12
+    for a, b, c in zip(w, v, f):
+        assert np.abs(a - (b + factor * c)).max() < 1e-13
+    # afterwards the band holds exactly the target energy
+    fw = [np.real(np.fft.ifft2(np.fft.fft2(x) * band)) for x in w]
+    assert abs(np.mean(fw[0] ** 2 + fw[1] ** 2) - 0.075) < 1e-14
+
+
+def test_3d_shear_flow_and_solenoidal_projection(orc):
+    """3-D path (compute_ut3D, RK33D): u = (f(z), g(z), 0) is an exact
+    nonlinear-free shear flow decaying by the RK3 gain."""
+    n, nsteps, re, dt = 16, 6, 20.0, 1e-2
+    z = np.arange(n) * 2 * np.pi / n
+    f = np.sin(2 * z) + 0.3 * np.cos(3 * z)
+    g = 0.5 * np.cos(z)
+    u1 = np.asfortranarray(np.broadcast_to(f[None, None, :], (n, n, n)).copy())
+    u2 = np.asfortranarray(np.broadcast_to(g[None, None, :], (n, n, n)).copy())
+    u3 = np.zeros((n, n, n), order="F")
+    s = orc.solver(make_params(n, n, ka=n, reynolds=re, deltat=dt))
+    s.set_fields(u1, u2, u3)
+    s.step(nsteps)
+    v1, v2, v3 = s.get_fields()
+    G = lambda m: rk3_gain(-(m * m) * dt / re) ** nsteps  # noqa: E731
+    assert np.abs(v1 - (G(2) * np.sin(2 * z) + 0.3 * G(3) * np.cos(3 * z))[None, None, :]).max() < 1e-13
+    assert np.abs(v2 - (0.5 * G(1) * np.cos(z))[None, None, :]).max() < 1e-13
+    assert np.abs(v3).max() < 1e-13
+
+
+def test_eta_min_initial_value_is_default_real(orc):
+    # eta_min = 2*3.14 in default (single) precision (solution.F90:1148)
+    n = 16
+    s = orc.solver(make_params(n, n))
+    s.set_fields(np.zeros((n, n), order="F"), np.zeros((n, n), order="F"))
+    _, d, _ = s.step(1)
+    assert d[0].eta_min == float(np.float32(2.0) * np.float32(3.14))
+    assert d[0].eta_min != 6.28
