@@ -98,7 +98,9 @@ struct bastr_handle {
     double* part_tot = nullptr;
     double* part_vg = nullptr;
     double* red = nullptr;
-    int nblk_spec = 0, nblk_vg = 0;
+    double* part_en = nullptr;   // energy partials (forcing methods 1, 5)
+    double* dfactor = nullptr;   // forcing method 1 factor
+    int nblk_spec = 0, nblk_vg = 0, nblk_en = 0;
     int* status = nullptr;
     bastr_diag* ddiag = nullptr;
     double* dspec = nullptr;
@@ -257,9 +259,12 @@ int run_to_physical(bastr_handle* h) {
     return 0;
 }
 
-// One RK3 stage: y-inverse (derivatives), x pass (nonlinear term), y-forward + update.
+// One RK3 stage (stage 0..2) or CN sweep (stage 3 = middle sweep, 4 = last
+// sweep; cn_first marks sweep 1): y-inverse (derivatives), x pass (nonlinear
+// term), y-forward + update.  The first stage/sweep's y-inverse and x pass ran
+// in the diagnostic phase (shared with velgrad).
 template <typename T>
-int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* Uin) {
+int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* Uin, bool first = false) {
     ColInvArgs<T> ci{};
     ci.g = geo<T>(h);
     ci.gate = gstep;
@@ -268,7 +273,7 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     for (int q = 0; q < 4; ++q) ci.out[q] = C<T>(h->W[q]);
     ci.pitch = h->hp;
     ci.tw = C<T>(h->tw1);
-    if (stage != 0) {
+    if (!first) {
         ProfScope ps(h, BASTR_K_COL_INV);
         LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE, ci, h->hp, h->stream));
     }
@@ -280,7 +285,7 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     r.hout[0] = C<T>(h->NX[0]);
     r.hout[1] = C<T>(h->NX[1]);
     r.tw = C<T>(h->tw0);
-    {
+    if (!first) {  // the first x pass ran fused with velgrad in the diagnostic phase
         ProfScope ps(h, BASTR_K_ROW_STAGE);
         LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
     }
@@ -295,10 +300,12 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
         cf.KA[c] = C<T>(h->KA[c]);
     }
     cf.pitch = h->hp;
+    cf.cn_first = first ? 1 : 0;
     cf.tw = C<T>(h->tw1);
     {
-        ProfScope ps(h, BASTR_K_COL_FWD_A + stage);
-        LAUNCHCHK(launch_col_fwd<T>(h->n1, stage == 0 ? CF_A : (stage == 1 ? CF_B : CF_C), cf, h->hp, h->stream));
+        static const int modes[5] = {CF_A, CF_B, CF_C, CF_CN_MID, CF_CN_LAST};
+        ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
+        LAUNCHCHK(launch_col_fwd<T>(h->n1, modes[stage], cf, h->hp, h->stream));
     }
     return 0;
 }
@@ -339,6 +346,8 @@ int run_iteration(bastr_handle* h, int it) {
     va.gate = gdiag;
     for (int q = 0; q < 7; ++q) va.w[q] = C<T>(h->W[q]);
     va.part = h->part_vg;
+    va.hout[0] = C<T>(h->NX[0]);
+    va.hout[1] = C<T>(h->NX[1]);
     va.tw = C<T>(h->tw0);
     {
         ProfScope ps(h, BASTR_K_VELGRAD);
@@ -363,25 +372,54 @@ int run_iteration(bastr_handle* h, int it) {
         ProfScope ps(h, BASTR_K_FINALIZE);
         LAUNCHCHK(launch_finalize(fa, h->stream));
     }
-    // RK32D (solution.F90:172-203)
     const void* Ua[2] = {h->U[0], h->U[1]};
     const void* Ub[2] = {h->UB[0], h->UB[1]};
     int rc;
-    if ((rc = run_stage<T>(h, 0, gstep, Ua))) return rc;
-    if ((rc = run_stage<T>(h, 1, gstep, Ub))) return rc;
-    if ((rc = run_stage<T>(h, 2, gstep, Ub))) return rc;
-    // forcing2D (solution.F90:303)
-    if (h->p.forcemethod == 2) {
+    if (h->p.timemethod == 1) {
+        // RK32D (solution.F90:172-203)
+        if ((rc = run_stage<T>(h, 0, gstep, Ua, true))) return rc;
+        if ((rc = run_stage<T>(h, 1, gstep, Ub))) return rc;
+        if ((rc = run_stage<T>(h, 2, gstep, Ub))) return rc;
+    } else {
+        // CN2D (solution.F90:205-231): six sweeps u = uold + dt*F(0.5*(u + uold))
+        for (int sweep = 1; sweep <= 6; ++sweep)
+            if ((rc = run_stage<T>(h, sweep == 6 ? 4 : 3, gstep, Ub, sweep == 1))) return rc;
+    }
+    // forcing2D (solution.F90:303-540)
+    const int fm = h->p.forcemethod;
+    if (fm == 1 || fm == 2 || fm == 5) {
+        ProfScope ps(h, BASTR_K_FORCE);
+        if (fm == 1 || fm == 5) {
+            EnergyArgs<T> e{};
+            e.g = geo<T>(h);
+            e.gate = gstep;
+            e.U[0] = C<T>(h->U[0]);
+            e.U[1] = C<T>(h->U[1]);
+            e.part_en = h->part_en;
+            LAUNCHCHK(launch_energy<T>(e, h->nblk_en, h->stream));
+        }
         ForceArgs<T> f{};
         f.g = geo<T>(h);
         f.gate = gstep;
         f.U[0] = C<T>(h->U[0]);
         f.U[1] = C<T>(h->U[1]);
+        f.forcemethod = fm;
         f.forcek = h->p.forcek;
         f.target_energy = h->p.target_energy;
+        f.part_en = h->part_en;
+        f.nblk_en = h->nblk_en;
+        f.factor_out = h->dfactor;
         f.diag = h->ddiag + it;
-        ProfScope ps(h, BASTR_K_FORCE);
         LAUNCHCHK(launch_force_band<T>(f, h->stream));
+        if (fm == 1) {
+            ScaleArgs<T> sc{};
+            sc.g = geo<T>(h);
+            sc.gate = gstep;
+            sc.U[0] = C<T>(h->U[0]);
+            sc.U[1] = C<T>(h->U[1]);
+            sc.factor = h->dfactor;
+            LAUNCHCHK(launch_scale<T>(sc, h->stream));
+        }
     }
     return 0;
 }
@@ -452,13 +490,16 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     if (p->ka != 0) return fail(BASTR_EINVAL, "bastr_create: 3-D grids (ka=%d) are not supported by this build", p->ka);
     if (!pow2_in(p->ia, 16, 4096) || !pow2_in(p->ja, 16, 4096))
         return fail(BASTR_EINVAL, "bastr_create: ia, ja must be powers of two in [16, 4096] (got %d, %d)", p->ia, p->ja);
-    if (p->timemethod != 1) return fail(BASTR_EINVAL, "bastr_create: timemethod %d not supported (RK3 = 1)", p->timemethod);
-    if (p->forcemethod != 0 && p->forcemethod != 2)
-        return fail(BASTR_EINVAL, "bastr_create: forcemethod %d not supported (0, 2)", p->forcemethod);
+    if (p->timemethod != 1 && p->timemethod != 2)
+        return fail(BASTR_EINVAL, "Undefined time method %d (1 = RK3, 2 = CN)", p->timemethod);
+    if (p->forcemethod < 0 || p->forcemethod > 5 || p->forcemethod == 3 || p->forcemethod == 4)
+        return fail(BASTR_EINVAL,
+                    "bastr_create: forcemethod %d not supported (0, 1, 2, 5; methods 3 and 4 draw from the "
+                    "reference's unseeded Fortran RNG)", p->forcemethod);
     if (p->precision != BASTR_PREC_F64 && p->precision != BASTR_PREC_F32)
         return fail(BASTR_EINVAL, "bastr_create: bad precision %d", p->precision);
     if (p->kmax < 1) return fail(BASTR_EINVAL, "bastr_create: kmax must be >= 1");
-    if (p->forcemethod == 2 && 2 * p->forcek + 2 > p->ja)
+    if ((p->forcemethod == 2 || p->forcemethod == 5) && 2 * p->forcek + 2 > p->ja)
         return fail(BASTR_EINVAL, "bastr_create: forcek %d too large for ja=%d", p->forcek, p->ja);
 
     bastr_handle* h = new bastr_handle();
@@ -501,6 +542,9 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8, &ok);
     h->status = (int*)dalloc(sizeof(int), &ok);
     h->red = (double*)dalloc(sizeof(double) * (2 * (size_t)h->allkmax + 12), &ok);
+    h->nblk_en = 512;
+    h->part_en = (double*)dalloc(sizeof(double) * h->nblk_en, &ok);
+    h->dfactor = (double*)dalloc(sizeof(double), &ok);
     if (!ok) {
         bastr_destroy(h);
         return fail(BASTR_ENOMEM, "bastr_create: device allocation failed");
@@ -536,7 +580,8 @@ int bastr_destroy(bastr_handle* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     void* bufs[] = {h->U[0], h->U[1], h->UB[0], h->UB[1], h->KA[0], h->KA[1], h->NX[0], h->NX[1],
                     h->ureal[0], h->ureal[1], h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf,
-                    h->part_bins, h->part_tot, h->part_vg, h->red, h->status, h->ddiag, h->dspec};
+                    h->part_bins, h->part_tot, h->part_vg, h->red, h->part_en, h->dfactor,
+                    h->status, h->ddiag, h->dspec};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (int q = 0; q < 7; ++q)
@@ -703,7 +748,7 @@ int bastr_prof_enable(bastr_handle* h, uint32_t mask) {
 int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* total_ms, double* bytes) {
     if (!h || c < 0 || c >= BASTR_K_NCLASS) return fail(BASTR_EINVAL, "bastr_prof_read: bad argument");
     // compulsory traffic in units of one half-spectrum field (nh*n1 complex)
-    static const double units[BASTR_K_NCLASS] = {2, 9, 7, 0, 6, 6, 8, 10, 10, 0};
+    static const double units[BASTR_K_NCLASS] = {2, 9, 9, 0, 6, 6, 8, 10, 10, 0};
     const double half = (double)h->nh * h->n1 * 2.0 * h->esz;
     if (launches) *launches = h->prof_launch[c];
     if (total_ms) *total_ms = h->prof_ms[c];

@@ -108,18 +108,26 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 template <typename T, bool SPLIT>
 struct LdsLine;
 
+// Element e of a line lives at padded slot e + e/16: one spare slot every 16
+// breaks the power-of-two strides of the Stockham write pattern, which
+// otherwise land up to 8 lanes of a pass on the same LDS banks.
+__device__ __forceinline__ int lds_slot(int e) { return e + (e >> 4); }
+// LDS elements needed per line of length n
+constexpr int lds_line(int n) { return n + n / 16; }
+
 template <typename T>
 struct LdsLine<T, false> {
     cx<T>* base;
-    int stride;  // element e at base[e*stride]
-    __device__ __forceinline__ void st(int e, cx<T> v) const { base[e * stride] = v; }
-    __device__ __forceinline__ cx<T> ld(int e) const { return base[e * stride]; }
+    int stride;  // element e at base[lds_slot(e)*stride]
+    __device__ __forceinline__ void st(int e, cx<T> v) const { base[lds_slot(e) * stride] = v; }
+    __device__ __forceinline__ cx<T> ld(int e) const { return base[lds_slot(e) * stride]; }
 };
 
 template <typename T>
 struct LdsLine<T, true> {
     T* base;
     int stride;
+    __device__ __forceinline__ T& at(int e) const { return base[lds_slot(e) * stride]; }
 };
 
 // Exchange: values v[r] written at positions widx(r), then read back so that
@@ -138,16 +146,16 @@ template <typename T, int NV, class W, class Rd>
 __device__ __forceinline__ void exchange(const LdsLine<T, true>& L, cx<T>* v, W widx, Rd ridx) {
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < NV; ++r) L.base[widx(r) * L.stride] = v[r].x;
+    for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].x;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < NV; ++r) v[r].x = L.base[ridx(r) * L.stride];
+    for (int r = 0; r < NV; ++r) v[r].x = L.at(ridx(r));
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < NV; ++r) L.base[widx(r) * L.stride] = v[r].y;
+    for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].y;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < NV; ++r) v[r].y = L.base[ridx(r) * L.stride];
+    for (int r = 0; r < NV; ++r) v[r].y = L.at(ridx(r));
 }
 
 // twiddle v[r] *= w_N^{(b mod Ns) * r * N/(Ns*R)}, r = 1..R-1.  w^r is the

@@ -24,7 +24,7 @@ namespace k2d {
 template <typename T, int N>
 struct ColCfg {
     static constexpr int CBMIN = 64 / (int)sizeof(cx<T>);
-    static constexpr int LDS_CB = 65536 / (N * (int)sizeof(T));
+    static constexpr int LDS_CB = 65536 / (N * (int)sizeof(T));  // (+1/16 padding, <= 68 KiB)
     static constexpr int CBW = (256 / (N / 8)) > CBMIN ? (256 / (N / 8)) : CBMIN;
     static constexpr int CB2 = CBW > 16 ? 16 : CBW;
     static constexpr int CB = CB2 < LDS_CB ? CB2 : LDS_CB;
@@ -41,7 +41,7 @@ struct RowCfg {
     static constexpr int E = 8;
     static constexpr int P = N / E;
     static constexpr int L0 = P >= 256 ? 1 : 256 / P;
-    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));
+    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/16 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
 };
@@ -70,7 +70,7 @@ struct Gate {
 };
 
 enum ColInvMode { CI_PLAIN = 0, CI_STAGE = 1, CI_STAGE_VG = 2 };
-enum ColFwdMode { CF_PLAIN = 0, CF_PROLOGUE = 1, CF_A = 2, CF_B = 3, CF_C = 4 };
+enum ColFwdMode { CF_PLAIN = 0, CF_PROLOGUE = 1, CF_A = 2, CF_B = 3, CF_C = 4, CF_CN_MID = 5, CF_CN_LAST = 6 };
 enum RowMode { R_FWD_PACK = 0, R_INV_PACK = 1, R_STAGE = 2, R_C2C_FWD = 3, R_C2C_INV = 4 };
 
 template <typename T>
@@ -95,6 +95,7 @@ struct ColFwdArgs {
     cx<T>* out[2];       // CF_PLAIN output
     int nfield;
     int pitch;
+    int cn_first;        // CF_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
     const cx<T>* tw;
 };
 
@@ -116,7 +117,8 @@ struct VgArgs {
     Geo<T> g;
     Gate gate;
     const cx<T>* w[7];  // A1 A2 B1 B2 C1 C2 D
-    double* part;       // [gridDim.x][8]
+    double* part;       // [8][gridDim.x]
+    cx<T>* hout[2];     // non-null: also produce stage A's x-forward nonlinear term
     const cx<T>* tw;
 };
 
@@ -148,9 +150,29 @@ struct ForceArgs {
     Geo<T> g;
     Gate gate;
     cx<T>* U[2];
+    int forcemethod;          // 1, 2 or 5
     int forcek;
     double target_energy;
+    const double* part_en;    // [nblk_en] partial sums of |u_hat|^2 over all modes (methods 1, 5)
+    int nblk_en;
+    double* factor_out;       // method 1: the factor for scale_kernel
     bastr_diag* diag;
+};
+
+template <typename T>
+struct EnergyArgs {
+    Geo<T> g;
+    Gate gate;
+    const cx<T>* U[2];
+    double* part_en;          // [gridDim.x]
+};
+
+template <typename T>
+struct ScaleArgs {
+    Geo<T> g;
+    Gate gate;
+    cx<T>* U[2];
+    const double* factor;
 };
 
 // launchers (k2d.hip); n is the transform length along the pass
@@ -161,6 +183,8 @@ template <typename T> int launch_velgrad(int n, const VgArgs<T>& a, int nrows, h
 template <typename T> int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s);
 int launch_finalize(const FinArgs& a, hipStream_t s);
 template <typename T> int launch_force_band(const ForceArgs<T>& a, hipStream_t s);
+template <typename T> int launch_energy(const EnergyArgs<T>& a, int nblk, hipStream_t s);
+template <typename T> int launch_scale(const ScaleArgs<T>& a, hipStream_t s);
 int col_block_width(int n, int prec);  // CB of the column pass for (n, precision)
 int row_block_rows(int n, int prec);   // L of the row pass
 int velgrad_blocks(int n0, int n1, int prec);

@@ -85,7 +85,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_inv_kernel(ColInv
     using C = ColCfg<T, N>;
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
-    __shared__ T lds[CB * N];
+    __shared__ T lds[CB * lds_line(N)];
     const int blk = xcd_block(blockIdx.x, a.pitch / CB);
     if (blk < 0) return;
     const int tid = threadIdx.x;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
     using C = ColCfg<T, N>;
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
-    __shared__ T lds[CB * N];
+    __shared__ T lds[CB * lds_line(N)];
     const int blk = xcd_block(blockIdx.x, a.pitch / CB);
     if (blk < 0) return;
     const int tid = threadIdx.x;
@@ -187,6 +187,9 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
         cx<T> ka0, ka1;
         if (MODE == CF_A) {
             s0 = U0; s1 = U1;
+        } else if (MODE == CF_CN_MID || MODE == CF_CN_LAST) {
+            if (a.cn_first) { s0 = U0; s1 = U1; }                  // first sweep: u_hat
+            else { s0 = a.UB[0][idx]; s1 = a.UB[1][idx]; }         // this sweep's midpoint
         } else if (MODE == CF_B) {
             ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
             s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
@@ -212,7 +215,15 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
                 t0 = {0, 0}; t1 = {0, 0};
             }
         }
-        if (MODE == CF_A) {
+        if (MODE == CF_CN_MID) {
+            // CN2D (solution.F90:215-225): u = uold + dt*r; next midpoint 0.5*(u + uold)
+            const cx<T> n0 = cadd(U0, cscale(t0, dt)), n1 = cadd(U1, cscale(t1, dt));
+            a.UB[0][idx] = cscale(cadd(n0, U0), (T)0.5);
+            a.UB[1][idx] = cscale(cadd(n1, U1), (T)0.5);
+        } else if (MODE == CF_CN_LAST) {
+            a.U[0][idx] = cadd(U0, cscale(t0, dt));
+            a.U[1][idx] = cadd(U1, cscale(t1, dt));
+        } else if (MODE == CF_A) {
             a.KA[0][idx] = t0; a.KA[1][idx] = t1;
             a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
             a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
@@ -238,13 +249,13 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * N];
+    __shared__ cx<T> lds[LR * lds_line(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * N, 1};
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
     const size_t hp = (size_t)a.g.hp;
 
     if (MODE == R_C2C_FWD || MODE == R_C2C_INV) {
@@ -314,20 +325,136 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
 }
 
-// velocity-gradient moments (velgrad_calculate2D, solution.F90:1110-1186)
+// Stage A of RK3 fused with velgrad_calculate2D (solution.F90:1110-1186):
+// the derivative fields of the nonlinear term are the velgrad ones, so one
+// pass over A1 A2 B1 B2 C1 C2 D produces both the x-forward nonlinear term
+// and the 8 moments' partial sums (part[q * gridDim.x + block]).
 template <typename T, int N>
-__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void velgrad_kernel(VgArgs<T> a) {
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_stage_vg_kernel(VgArgs<T> a) {
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * N];
+    __shared__ cx<T> lds[LR * lds_line(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * N, 1};
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const size_t hp = (size_t)a.g.hp;
+    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
+    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
+    const cx<T>* B1 = a.w[2] + (size_t)row * hp;
+    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
+    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
+    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
+    const cx<T>* D = a.w[6] + (size_t)row * hp;
+    const double nu = a.g.nu;
+    const double eta0 = (double)(2.0f * 3.14f);
+
+    cx<T> z[E], w[E], n[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        z[k] = pack_at<T, N>(A1, A2, m);
+        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
+        w[k] = cscale(cmuli(z[k]), kxs);
+    }
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d1u1, d1u2
+#pragma unroll
+    for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
+    cx<T> y[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) y[k] = pack_at<T, N>(B1, B2, j + k * P);
+    line_fft<T, N, E, true, false>(y, j, Ls, a.tw);  // d2u1, d2u2
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
+    T dv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        n[k].x = n[k].x - z[k].y * y[k].x;
+        n[k].y = n[k].y - z[k].y * y[k].y;
+        const double u1 = z[k].x, u2 = z[k].y;
+        const double b11 = w[k].x, b21 = w[k].y, b12 = y[k].x, b22 = y[k].y;
+        const double div = b11 + b22;
+        const double uu = u1 * u1 + u2 * u2;
+        dv[k] = (T)div;
+        s_um2 += div * div * uu;
+        s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
+        s_u2t += div * uu;
+        s_eps += nu * (b12 - b21) * (b12 - b21) + nu * div * div;
+        eta = fmin(eta, sqrt(nu / fabs(div)));
+        s_urms += u1 * u1 + u2 * u2;
+        s_dudx += b11 * b11 + b22 * b22;
+    }
+    // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int kx = m <= N / 2 ? m : N - m;
+        const T k1sq = (T)kx * (T)kx;
+        w[k] = cadd(cscale(pack_at<T, N>(A1, A2, m), -k1sq), pack_at<T, N>(C1, C2, m));
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        s_dis += 2.0 * (double)z[k].x * (double)dv[k] * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)dv[k] * (double)w[k].y;
+    // Lap(theta) = x-inverse of (-i k1^3 A1 + i k1 C1 - k1^2 B2 + D), a real field
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1 = (T)h;
+        const T k1o = (h == N / 2) ? (T)0 : k1;
+        const cx<T> th = cadd(cadd(cscale(cmuli(A1[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
+                              cadd(cscale(B2[h], -(k1 * k1)), D[h]));
+        if (m == 0 || m == N / 2) w[k] = {th.x, (T)0};
+        else if (m < N / 2) w[k] = th;
+        else w[k] = cconj(th);
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
+        s_dis += uu * (double)w[k].x;
+    }
+    // nonlinear term: x-forward and split into the two half rows
+    line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+
+    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
+    const int wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
+    vals[7] = wave_min(vals[7]);
+    __syncthreads();
+    if (lane == 0)
+        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
+    __syncthreads();
+    if (tid < 8) {
+        double acc = tid == 7 ? eta0 : 0.0;
+        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
+        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+// velocity-gradient moments (velgrad_calculate2D, solution.F90:1110-1186)
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void velgrad_kernel(VgArgs<T> a) {
+    using C = RowCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ double red[C::THREADS / 64][8];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < a.g.n1;
+    const int row = valid ? row0 : a.g.n1 - 1;
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
     const size_t hp = (size_t)a.g.hp;
     const cx<T>* A1 = a.w[0] + (size_t)row * hp;
     const cx<T>* A2 = a.w[1] + (size_t)row * hp;
@@ -439,43 +566,66 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     const size_t hp = (size_t)a.g.hp;
     const int nbx = (nh + 63) / 64;
     const long ntask = (long)n1 * nbx;
-    const long t0 = ntask * blockIdx.x / gridDim.x, t1 = ntask * (blockIdx.x + 1) / gridDim.x;
+    // tasks are dealt round-robin over blocks and waves, so the few tasks
+    // with many shells <= allkmax (small |k|) are spread over the machine
+    const long tstride = 4L * gridDim.x;
+    const long t1 = ntask;
     double tot[4] = {0, 0, 0, 0};
-    for (long t = t0 + wave; t < t1; t += 4) {
-        const int r = (int)(t / nbx);
-        const int kx = (int)(t % nbx) * 64 + lane;
-        double es = 0, ed = 0, w = 0;
-        int s = INT_MAX;
-        if (kx < nh) {
-            const int ky = ksigned(r, n1);
-            w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
-            const double kk = sqrt((double)(kx * kx + ky * ky));
-            if (kk > 0.5) {
-                const cx<T> u1 = a.U[0][(size_t)r * hp + kx], u2 = a.U[1][(size_t)r * hp + kx];
-                const double k1 = kx, k2 = ky;
-                const double usx = (double)u1.x * k2 / kk - (double)u2.x * k1 / kk;
-                const double usy = (double)u1.y * k2 / kk - (double)u2.y * k1 / kk;
-                const double udx = (double)u1.x * k1 / kk + (double)u2.x * k2 / kk;
-                const double udy = (double)u1.y * k1 / kk + (double)u2.y * k2 / kk;
-                es = (usx * usx + usy * usy) / 2;
-                ed = (udx * udx + udy * udy) / 2;
-                tot[3] += w * (ed / kk);
+    constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
+    for (long t = (long)blockIdx.x * 4 + wave; t < t1; t += tstride * TB) {
+        cx<T> u1[TB], u2[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const long tq = t + tstride * q;
+            const int r = (int)(tq / nbx);
+            const int kx = (int)(tq % nbx) * 64 + lane;
+            if (tq < t1 && kx < nh) {
+                u1[q] = a.U[0][(size_t)r * hp + kx];
+                u2[q] = a.U[1][(size_t)r * hp + kx];
+            } else {
+                u1[q] = {0, 0};
+                u2[q] = {0, 0};
             }
-            tot[0] += w * es;
-            tot[1] += w * ed;
-            tot[2] += w * (ed * (kk * kk));
-            const int sh_ = (int)(kk + 0.5);
-            if (sh_ <= K) s = sh_;
         }
-        const int smin = wave_min_i(s);
-        if (smin != INT_MAX) {
-            const int smax = wave_max_i(s == INT_MAX ? -1 : s);
-            for (int b = smin; b <= smax; ++b) {
-                const double des = wave_sum(s == b ? w * es : 0.0);
-                const double ded = wave_sum(s == b ? w * ed : 0.0);
-                if (lane == 0) {
-                    bins[(wave * nb + b) * 2 + 0] += des;
-                    bins[(wave * nb + b) * 2 + 1] += ded;
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const long tq = t + tstride * q;
+            if (tq >= t1) break;  // wave-uniform
+            const int r = (int)(tq / nbx);
+            const int kx = (int)(tq % nbx) * 64 + lane;
+            double es = 0, ed = 0, w = 0;
+            int s = INT_MAX;
+            if (kx < nh) {
+                const int ky = ksigned(r, n1);
+                w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+                const double kk = sqrt((double)(kx * kx + ky * ky));
+                if (kk > 0.5) {
+                    const double ik = 1.0 / kk;
+                    const double c1 = kx * ik, c2 = ky * ik;
+                    const double usx = (double)u1[q].x * c2 - (double)u2[q].x * c1;
+                    const double usy = (double)u1[q].y * c2 - (double)u2[q].y * c1;
+                    const double udx = (double)u1[q].x * c1 + (double)u2[q].x * c2;
+                    const double udy = (double)u1[q].y * c1 + (double)u2[q].y * c2;
+                    es = (usx * usx + usy * usy) * 0.5;
+                    ed = (udx * udx + udy * udy) * 0.5;
+                    tot[3] += w * (ed * ik);
+                }
+                tot[0] += w * es;
+                tot[1] += w * ed;
+                tot[2] += w * (ed * (kk * kk));
+                const int sh_ = (int)(kk + 0.5);
+                if (sh_ <= K) s = sh_;
+            }
+            const int smin = wave_min_i(s);
+            if (smin != INT_MAX) {
+                const int smax = wave_max_i(s == INT_MAX ? -1 : s);
+                for (int b = smin; b <= smax; ++b) {
+                    const double des = wave_sum(s == b ? w * es : 0.0);
+                    const double ded = wave_sum(s == b ? w * ed : 0.0);
+                    if (lane == 0) {
+                        bins[(wave * nb + b) * 2 + 0] += des;
+                        bins[(wave * nb + b) * 2 + 1] += ded;
+                    }
                 }
             }
         }
@@ -500,21 +650,65 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     }
 }
 
-// band forcing, method 2 (forcing2D :336-378): E = sum over band modes of
-// |u1_hat|^2 + |u2_hat|^2 (Parseval of E = <f1^2 + f2^2>), factor =
-// sqrt(E_t/E) - 1, u_hat += factor * u_hat on the band.
+// sum over all modes of w*(|u1_hat|^2 + |u2_hat|^2) = <u1^2 + u2^2> (Parseval,
+// normalised coefficients), per block, for forcing methods 1 and 5
+template <typename T>
+__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
+    if (a.gate.closed()) return;
+    __shared__ double red[4];
+    const int n0 = a.g.n0, nh = a.g.nh;
+    const size_t hp = (size_t)a.g.hp;
+    const size_t total = (size_t)nh * a.g.n1;
+    double e = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int kx = (int)(i % nh), r = (int)(i / nh);
+        const double w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+        const cx<T> u1 = a.U[0][(size_t)r * hp + kx], u2 = a.U[1][(size_t)r * hp + kx];
+        e += w * ((double)u1.x * u1.x + (double)u1.y * u1.y + (double)u2.x * u2.x + (double)u2.y * u2.y);
+    }
+    e = wave_sum(e);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) a.part_en[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// forcing2D (solution.F90:303-540) methods 1, 2, 5 on the half spectrum.
+//   m1: energy = <u.u>/2, factor = sqrt(E_t/energy), u *= factor (scale_kernel)
+//   m2: f = band(u), E = <f.f> (no 1/2), factor = sqrt(E_t/E) - 1, u += factor f
+//   m5: f = band(u) [dilatational part if lproject], E = <f.f>, energy = <u.u>,
+//       factor = sqrt((E_t-energy)/E + 1) - 1 if E_t > energy else 0, u += factor f
+// band: (|k| - 0.5) < forcek < (|k| + 0.5).
 template <typename T>
 __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
     if (a.gate.closed()) return;
     __shared__ double red[4];
     __shared__ double fac;
     const int tid = threadIdx.x;
+    const int fm = a.forcemethod;
+    double energy = 0;
+    if (fm == 1 || fm == 5) {
+        if (tid == 0) {
+            double s = 0;
+            for (int b = 0; b < a.nblk_en; ++b) s += a.part_en[b];
+            energy = fm == 1 ? s / 2 : s;
+        }
+    }
+    if (fm == 1) {
+        if (tid == 0) {
+            const double factor = sqrt(a.target_energy / energy);
+            *a.factor_out = factor;
+            a.diag->force_factor = factor;
+            a.diag->force_energy = energy;
+        }
+        return;
+    }
     const int n0 = a.g.n0, n1 = a.g.n1;
     const size_t hp = (size_t)a.g.hp;
     const int fk = a.forcek;
     const int kxm = fk < n0 / 2 ? fk : n0 / 2;
     const int kym = fk < n1 / 2 ? fk : n1 / 2 - 1;
     const int nx = kxm + 1, ny = 2 * kym + 1;
+    const bool proj = fm == 5 && a.g.lproject;
     double e = 0;
     for (int t = tid; t < nx * ny; t += 256) {
         const int kx = t % nx, ky = t / nx - kym;
@@ -522,8 +716,14 @@ __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
         const double kk = sqrt((double)(kx * kx + ky * ky));
         if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
             const double w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
-            const cx<T> u1 = a.U[0][(size_t)r * hp + kx], u2 = a.U[1][(size_t)r * hp + kx];
-            e += w * ((double)u1.x * u1.x + (double)u1.y * u1.y + (double)u2.x * u2.x + (double)u2.y * u2.y);
+            double f1x = a.U[0][(size_t)r * hp + kx].x, f1y = a.U[0][(size_t)r * hp + kx].y;
+            double f2x = a.U[1][(size_t)r * hp + kx].x, f2y = a.U[1][(size_t)r * hp + kx].y;
+            if (proj) {
+                const double c1 = kx / kk, c2 = ky / kk;
+                const double udx = f1x * c1 + f2x * c2, udy = f1y * c1 + f2y * c2;
+                f1x = udx * c1; f1y = udy * c1; f2x = udx * c2; f2y = udy * c2;
+            }
+            e += w * (f1x * f1x + f1y * f1y + f2x * f2x + f2y * f2y);
         }
     }
     e = wave_sum(e);
@@ -531,24 +731,46 @@ __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
     __syncthreads();
     if (tid == 0) {
         const double E = red[0] + red[1] + red[2] + red[3];
-        const double factor = sqrt(a.target_energy / E) - 1.0;
+        double factor;
+        if (fm == 2) factor = sqrt(a.target_energy / E) - 1.0;
+        else factor = a.target_energy > energy ? sqrt((a.target_energy - energy) / E + 1.0) - 1.0 : 0.0;
         fac = factor;
         a.diag->force_factor = factor;
         a.diag->force_E = E;
+        if (fm == 5) a.diag->force_energy = energy;
     }
     __syncthreads();
-    const T f = (T)fac;
+    const double f = fac;
     for (int t = tid; t < nx * ny; t += 256) {
         const int kx = t % nx, ky = t / nx - kym;
         const int r = ky >= 0 ? ky : n1 + ky;
         const double kk = sqrt((double)(kx * kx + ky * ky));
         if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
-            for (int c = 0; c < 2; ++c) {
-                cx<T>* p = a.U[c] + (size_t)r * hp + kx;
-                const cx<T> u = *p;
-                *p = cadd(u, cscale(u, f));
+            cx<T>* p1 = a.U[0] + (size_t)r * hp + kx;
+            cx<T>* p2 = a.U[1] + (size_t)r * hp + kx;
+            const cx<T> u1 = *p1, u2 = *p2;
+            cx<T> f1 = u1, f2 = u2;
+            if (proj) {
+                const T c1 = (T)(kx / kk), c2 = (T)(ky / kk);
+                const cx<T> ud = cadd(cscale(u1, c1), cscale(u2, c2));
+                f1 = cscale(ud, c1);
+                f2 = cscale(ud, c2);
             }
+            *p1 = cadd(u1, cscale(f1, (T)f));
+            *p2 = cadd(u2, cscale(f2, (T)f));
         }
+    }
+}
+
+// u_hat *= factor on every mode (forcing method 1)
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(ScaleArgs<T> a) {
+    if (a.gate.closed()) return;
+    const T f = (T)*a.factor;
+    const size_t total = (size_t)a.g.hp * a.g.n1;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        a.U[0][i] = cscale(a.U[0][i], f);
+        a.U[1][i] = cscale(a.U[1][i], f);
     }
 }
 
@@ -584,6 +806,8 @@ static int col_fwd_n(int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s)
         case CF_A: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_A>), grid, block, 0, s, a); break;
         case CF_B: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_B>), grid, block, 0, s, a); break;
         case CF_C: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_C>), grid, block, 0, s, a); break;
+        case CF_CN_MID: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_CN_MID>), grid, block, 0, s, a); break;
+        case CF_CN_LAST: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_CN_LAST>), grid, block, 0, s, a); break;
         default: return -1;
     }
     return 0;
@@ -609,7 +833,8 @@ static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
     using C = RowCfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
-    hipLaunchKernelGGL((velgrad_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    if (a.hout[0]) hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    else hipLaunchKernelGGL((velgrad_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
@@ -660,6 +885,16 @@ int launch_force_band(const ForceArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((force_band_kernel<T>), dim3(1), dim3(256), 0, s, a);
     return 0;
 }
+template <typename T>
+int launch_energy(const EnergyArgs<T>& a, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL((energy_kernel<T>), dim3(nblk), dim3(256), 0, s, a);
+    return 0;
+}
+template <typename T>
+int launch_scale(const ScaleArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((scale_kernel<T>), dim3(1024), dim3(256), 0, s, a);
+    return 0;
+}
 
 #define K2D_INSTANTIATE(T)                                                                     \
     template int launch_col_inv<T>(int, int, const ColInvArgs<T>&, int, hipStream_t);          \
@@ -667,6 +902,8 @@ int launch_force_band(const ForceArgs<T>& a, hipStream_t s) {
     template int launch_row<T>(int, int, const RowArgs<T>&, int, hipStream_t);                 \
     template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*);             \
     template int launch_spectra<T>(const SpecArgs<T>&, int, hipStream_t);                      \
-    template int launch_force_band<T>(const ForceArgs<T>&, hipStream_t);
+    template int launch_force_band<T>(const ForceArgs<T>&, hipStream_t);                      \
+    template int launch_energy<T>(const EnergyArgs<T>&, int, hipStream_t);                     \
+    template int launch_scale<T>(const ScaleArgs<T>&, hipStream_t);
 
 }  // namespace k2d

@@ -124,6 +124,31 @@ def test_step_parity_large_grids(hip, orc, n, nsteps):
     assert np.abs(sg - so).max() / np.abs(so[:, 1:, :]).max() < 1e-10
 
 
+@pytest.mark.parametrize("shape,timemethod,force,proj,nsteps", [
+    ((128, 128), 2, 0, False, 6),   # CN2D, free decay
+    ((64, 128), 2, 2, True, 5),     # CN2D + band forcing + projection
+    ((128, 128), 1, 1, False, 8),   # forcing method 1
+    ((128, 128), 1, 5, True, 8),    # forcing method 5, projected band
+    ((64, 64), 1, 5, False, 8),     # forcing method 5
+])
+def test_cn_and_forcing_methods(hip, orc, shape, timemethod, force, proj, nsteps):
+    print(f"case methods {shape} tm={timemethod} fm={force} proj={proj}", flush=True)
+    p = make_params(*shape, timemethod=timemethod, forcemethod=force, lproject=proj, kmax=128,
+                    target_energy=0.6 if force in (1, 5) else 0.075)
+    u = mixed_field(shape, seed=13)
+    g, o, out = run_pair(hip, orc, p, u, nsteps)
+    (rg, dg, sg), (ro, do, so) = out[0]
+    assert rg == 0 and ro == 0
+    err = rel_l2(g.get_fields(), o.get_fields())
+    print("rel-L2", err)
+    assert err < 1e-11
+    check_diags(dg, do)
+    if force:
+        ff = np.array([d.force_factor for d in dg])
+        fo = np.array([d.force_factor for d in do])
+        assert np.abs(ff - fo).max() <= 1e-9 * max(1.0, np.abs(fo).max())
+
+
 def test_step_chunks_equal_single_call(hip):
     print("case chunks 64x64", flush=True)
     p = make_params(64, 64, forcemethod=2, lproject=True)

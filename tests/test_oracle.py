@@ -324,3 +324,58 @@ def test_eta_min_initial_value_is_default_real(orc):
     _, d, _ = s.step(1)
     assert d[0].eta_min == float(np.float32(2.0) * np.float32(3.14))
     assert d[0].eta_min != 6.28
+
+
+def test_forcing_m1_rescales_to_target_energy(orc):
+    """forcing2D method 1 (solution.F90:315-335): u *= sqrt(E_t/<u.u/2>), so
+    afterwards <(u1^2 + u2^2)/2> = E_t exactly; the factor matches an
+    independent computation on the unforced step."""
+    shape = (64, 64)
+    u = hitgen(shape, k0=5, seed=9)
+    base = dict(reynolds=1000.0, deltat=1e-3, target_energy=0.3)
+    s0 = orc.solver(make_params(*shape, forcemethod=0, **base))
+    s1 = orc.solver(make_params(*shape, forcemethod=1, **base))
+    s0.set_fields(*u)
+    s1.set_fields(*u)
+    s0.step(1)
+    _, d1, _ = s1.step(1)
+    v = s0.get_fields()
+    energy = np.mean((v[0] ** 2 + v[1] ** 2) / 2)
+    assert abs(d1[0].force_factor - np.sqrt(0.3 / energy)) < 1e-13
+    w = s1.get_fields()
+    assert abs(np.mean((w[0] ** 2 + w[1] ** 2) / 2) - 0.3) < 1e-14
+    assert np.abs(w[0] - v[0] * np.sqrt(0.3 / energy)).max() < 1e-14
+
+
+@pytest.mark.parametrize("proj", [False, True])
+def test_forcing_m5_against_numpy(orc, proj):
+    """forcing2D method 5 (solution.F90:483-538): band (dilatational part of the
+    band if lproject), E = <f.f>, energy = <u.u> (no 1/2),
+    factor = sqrt((E_t - energy)/E + 1) - 1 when E_t > energy."""
+    shape = (64, 64)
+    u = hitgen(shape, k0=5, seed=10, urms=0.3)
+    base = dict(reynolds=1000.0, deltat=1e-3, forcek=5, target_energy=0.2, lproject=proj)
+    s0 = orc.solver(make_params(*shape, forcemethod=0, **base))
+    s5 = orc.solver(make_params(*shape, forcemethod=5, **base))
+    s0.set_fields(*u)
+    s5.set_fields(*u)
+    s0.step(1)
+    _, d5, _ = s5.step(1)
+    v = s0.get_fields()
+    k1, k2 = kgrid(shape)
+    kk = np.sqrt(k1 ** 2 + k2 ** 2)
+    band = (kk - 0.5 < 5) & (kk + 0.5 > 5)
+    vh = [np.fft.fft2(x) * band for x in v]
+    if proj:
+        safe = np.where(kk > 0, kk, 1)
+        ud = (vh[0] * k1 + vh[1] * k2) / safe
+        vh = [ud * k1 / safe, ud * k2 / safe]
+    f = [np.real(np.fft.ifft2(x)) for x in vh]
+    E = np.mean(f[0] ** 2 + f[1] ** 2)
+    energy = np.mean(v[0] ** 2 + v[1] ** 2)
+    assert 0.2 > energy
+    factor = np.sqrt((0.2 - energy) / E + 1) - 1
+    assert abs(d5[0].force_factor - factor) < 1e-12 and abs(d5[0].force_E - E) < 1e-13 * E
+    w = s5.get_fields()
+    for a, b, c in zip(w, v, f):
+        assert np.abs(a - (b + factor * c)).max() < 1e-13
