@@ -17,6 +17,7 @@
 #include "../../include/bastr.h"
 #include "../../include/bastr_prof.h"
 #include "k2d.hpp"
+#include "k3d.hpp"
 
 using namespace k2d;
 
@@ -82,12 +83,15 @@ struct bastr_handle {
     size_t npts = 0, nhalf = 0;
     size_t esz = 8;  // sizeof(T)
     hipStream_t stream = nullptr;
-    void* U[2] = {};
-    void* UB[2] = {};
-    void* KA[2] = {};
-    void* W[7] = {};
-    void* NX[2] = {};
-    void* ureal[2] = {};
+    void* U[3] = {};
+    void* UB[3] = {};
+    void* KA[3] = {};
+    void* W[13] = {};            // 2-D: 7 y-inverse outputs; 3-D: 13 y-inverse outputs
+    void* Z[10] = {};            // 3-D: z-inverse outputs
+    void* NX[3] = {};            // x-pass outputs (3-D: aliases Z[0..2])
+    void* NY[3] = {};            // 3-D y-forward outputs (aliases Z[3..5])
+    void* ureal[3] = {};         // physical fields (3-D: aliases W[0..2])
+    int n2 = 1;
     double* dstage = nullptr;  // FP64 host-exchange staging (FP32 mode)
     void* tw0 = nullptr;
     void* tw1 = nullptr;
@@ -424,6 +428,310 @@ int run_iteration(bastr_handle* h, int it) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// 3-D launch sequence (mainloop3D body, solution.F90:124-169)
+// ---------------------------------------------------------------------------
+template <typename T>
+k3d::Geo3<T> geo3(const bastr_handle* h) {
+    k3d::Geo3<T> g;
+    g.n0 = h->n0;
+    g.n1 = h->n1;
+    g.n2 = h->n2;
+    g.nh = h->nh;
+    g.hp = h->hp;
+    g.cut2 = h->cut2;
+    g.lproject = h->p.lproject;
+    g.nu = h->nu;
+    g.dt = h->p.deltat;
+    g.inv_n = 1.0 / ((double)h->n0 * (double)h->n1 * (double)h->n2);
+    return g;
+}
+
+k3d::ColJob job1(int in, int mult, int out) {
+    k3d::ColJob j{};
+    j.nterm = 1;
+    j.in[0] = in;
+    j.mult[0] = mult;
+    j.out = out;
+    return j;
+}
+
+// z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: Z33_c = -k3^2, Z333 = -i k3^3 of u3)
+template <typename T>
+int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < 3; ++q) c.in[q] = C<T>(src[q]);
+    for (int q = 0; q < 10; ++q) c.out[q] = C<T>(h->Z[q]);
+    int nj = 0;
+    for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_ONE, q);
+    if (mode >= 1)
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_IK, 3 + q);
+    if (mode == 2) {
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_MK2, 6 + q);
+        c.job[nj++] = job1(2, k3d::M_MIK3, 9);
+    }
+    c.njob = nj;
+    c.pitch = h->hp * h->n1;
+    c.ncols = h->hp * h->n1;
+    c.plane_stride = 0;
+    c.nplanes = 1;
+    c.scale = 1.0;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(h->n2, true, c, h->stream));
+    return 0;
+}
+
+// y-inverse of the z-inverse outputs into W: Y_c, Y2_c = i k2 Y, Y3_c (and with
+// vg: L_c = y-part of -(k2^2 + k3^2) u_c, D = y-part of -(k2^2+k3^2)(i k2 u2 + i k3 u3))
+template <typename T>
+int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/, int out0 = 0) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < 10; ++q) c.in[q] = C<T>(h->Z[q]);
+    for (int q = 0; q < 13; ++q) c.out[q] = C<T>(h->W[q]);
+    int nj = 0;
+    for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_ONE, out0 + q);
+    if (mode >= 1) {
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_IK, 3 + q);
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(3 + q, k3d::M_ONE, 6 + q);
+    }
+    if (mode == 2) {
+        for (int q = 0; q < 3; ++q) {
+            k3d::ColJob j{};
+            j.nterm = 2;
+            j.in[0] = q;      j.mult[0] = k3d::M_MK2;
+            j.in[1] = 6 + q;  j.mult[1] = k3d::M_ONE;
+            j.out = 9 + q;
+            c.job[nj++] = j;
+        }
+        k3d::ColJob d{};
+        d.nterm = 4;
+        d.in[0] = 1; d.mult[0] = k3d::M_MIK3;  // -i k2^3 Z_u2
+        d.in[1] = 7; d.mult[1] = k3d::M_IK;    //  i k2   Z33_u2
+        d.in[2] = 5; d.mult[2] = k3d::M_MK2;   // -k2^2   Z3_u3
+        d.in[3] = 9; d.mult[3] = k3d::M_ONE;   //         Z333_u3
+        d.out = 12;
+        c.job[nj++] = d;
+    }
+    c.njob = nj;
+    c.pitch = h->hp;
+    c.ncols = h->hp;
+    c.plane_stride = (long)h->hp * h->n1;
+    c.nplanes = h->n2;
+    c.scale = 1.0;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(h->n1, true, c, h->stream));
+    return 0;
+}
+
+// y-forward of the x-pass outputs NX -> NY
+template <typename T>
+int run_yfwd3(bastr_handle* h, const Gate& g) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < 3; ++q) {
+        c.in[q] = C<T>(h->NX[q]);
+        c.out[q] = C<T>(h->NY[q]);
+        c.job[q] = job1(q, k3d::M_ONE, q);
+    }
+    c.njob = 3;
+    c.pitch = h->hp;
+    c.ncols = h->hp;
+    c.plane_stride = (long)h->hp * h->n1;
+    c.nplanes = h->n2;
+    c.scale = 1.0;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(h->n1, false, c, h->stream));
+    return 0;
+}
+
+template <typename T>
+int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first) {
+    k3d::Cf3Args<T> c{};
+    c.g = geo3<T>(h);
+    c.gate = g;
+    for (int q = 0; q < 3; ++q) {
+        c.nx[q] = C<T>(h->NY[q]);
+        c.U[q] = C<T>(h->U[q]);
+        c.UB[q] = C<T>(h->UB[q]);
+        c.KA[q] = C<T>(h->KA[q]);
+    }
+    c.cn_first = first ? 1 : 0;
+    c.pitch = h->hp * h->n1;
+    c.ncols = h->hp * h->n1;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_cf3<T>(h->n2, mode, c, h->stream));
+    return 0;
+}
+
+template <typename T>
+int run_x3(bastr_handle* h, const Gate& g) {
+    k3d::Row3Args<T> r{};
+    r.g = geo3<T>(h);
+    r.gate = g;
+    for (int q = 0; q < 9; ++q) r.hin[q] = C<T>(h->W[q]);
+    for (int q = 0; q < 3; ++q) r.hout[q] = C<T>(h->NX[q]);
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_STAGE, r, h->n1 * h->n2, h->stream));
+    return 0;
+}
+
+template <typename T>
+int run_prologue3(bastr_handle* h) {
+    const Gate open{h->status, 0, 0};
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    k3d::Row3Args<T> r{};
+    r.g = geo3<T>(h);
+    r.gate = open;
+    for (int q = 0; q < 3; ++q) {
+        r.rin[q] = (const T*)h->ureal[q];
+        r.hout[q] = C<T>(h->NX[q]);
+    }
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
+    int rc;
+    if ((rc = run_yfwd3<T>(h, open))) return rc;
+    return run_cf3<T>(h, open, k3d::C3_PROLOGUE, false);
+}
+
+template <typename T>
+int run_to_physical3(bastr_handle* h) {
+    const Gate open{h->status, 0, 0};
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    int rc;
+    if ((rc = run_zinv3<T>(h, open, h->U, 0))) return rc;
+    if ((rc = run_yinv3<T>(h, open, 0, 3))) return rc;  // into W[3..5]; ureal aliases W[0..2]
+    k3d::Row3Args<T> r{};
+    r.g = geo3<T>(h);
+    r.gate = open;
+    for (int q = 0; q < 3; ++q) {
+        r.hin[q] = C<T>(h->W[3 + q]);
+        r.rout[q] = (T*)h->ureal[q];
+    }
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_INV_PACK, r, h->n1 * h->n2, h->stream));
+    return 0;
+}
+
+// stage: 0..2 RK3 stages, 3 CN middle sweep, 4 CN last sweep
+template <typename T>
+int run_stage3(bastr_handle* h, int stage, const Gate& g, bool first) {
+    int rc;
+    if (!first) {
+        {
+            ProfScope ps(h, BASTR_K_COL_INV);
+            if ((rc = run_zinv3<T>(h, g, h->UB, 1))) return rc;
+            if ((rc = run_yinv3<T>(h, g, 1))) return rc;
+        }
+    }
+    {
+        ProfScope ps(h, BASTR_K_ROW_STAGE);
+        if ((rc = run_x3<T>(h, g))) return rc;
+    }
+    static const int modes[5] = {k3d::C3_A, k3d::C3_B, k3d::C3_C, k3d::C3_CN_MID, k3d::C3_CN_LAST};
+    ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
+    if ((rc = run_yfwd3<T>(h, g))) return rc;
+    return run_cf3<T>(h, g, modes[stage], first);
+}
+
+template <typename T>
+int run_iteration3(bastr_handle* h, int it) {
+    const Gate gdiag{h->status, it, 0};
+    const Gate gstep{h->status, it, 1};
+    int rc;
+    {
+        ProfScope ps(h, BASTR_K_SPECTRA);
+        k3d::Spec3Args<T> sa{};
+        sa.g = geo3<T>(h);
+        sa.gate = gdiag;
+        for (int q = 0; q < 3; ++q) sa.U[q] = C<T>(h->U[q]);
+        sa.allkmax = h->allkmax;
+        sa.part_bins = h->part_bins;
+        sa.part_tot = h->part_tot;
+        LAUNCHCHK(k3d::launch_spec3<T>(sa, h->nblk_spec, h->stream));
+    }
+    {
+        ProfScope ps(h, BASTR_K_COL_INV_VG);
+        if ((rc = run_zinv3<T>(h, gdiag, h->U, 2))) return rc;
+        if ((rc = run_yinv3<T>(h, gdiag, 2))) return rc;
+    }
+    {
+        ProfScope ps(h, BASTR_K_VELGRAD);
+        k3d::Vg3Args<T> va{};
+        va.g = geo3<T>(h);
+        va.gate = gdiag;
+        for (int q = 0; q < 13; ++q) va.w[q] = C<T>(h->W[q]);
+        va.part = h->part_vg;
+        va.tw = C<T>(h->tw0);
+        LAUNCHCHK(k3d::launch_vg3<T>(h->n0, va, h->n1 * h->n2, h->stream, h->nblk_vg));
+    }
+    {
+        FinArgs fa{};
+        fa.gate = gdiag;
+        fa.nblk_spec = h->nblk_spec;
+        fa.nblk_vg = h->nblk_vg;
+        fa.allkmax = h->allkmax;
+        fa.ndims = 3;
+        fa.nu = h->nu;
+        fa.npts = (double)h->npts;
+        fa.part_bins = h->part_bins;
+        fa.part_tot = h->part_tot;
+        fa.part_vg = h->part_vg;
+        fa.red = h->red;
+        fa.diag = h->ddiag + it;
+        fa.spec = h->dspec + (size_t)it * 3 * h->allkmax;
+        fa.status = h->status;
+        ProfScope ps(h, BASTR_K_FINALIZE);
+        LAUNCHCHK(launch_finalize(fa, h->stream));
+    }
+    if (h->p.timemethod == 1) {
+        // RK33D (solution.F90:233-270)
+        if ((rc = run_stage3<T>(h, 0, gstep, true))) return rc;
+        if ((rc = run_stage3<T>(h, 1, gstep, false))) return rc;
+        if ((rc = run_stage3<T>(h, 2, gstep, false))) return rc;
+    } else {
+        // CN3D (solution.F90:272-301)
+        for (int sweep = 1; sweep <= 6; ++sweep)
+            if ((rc = run_stage3<T>(h, sweep == 6 ? 4 : 3, gstep, sweep == 1))) return rc;
+    }
+    const int fm = h->p.forcemethod;
+    if (fm == 1 || fm == 2 || fm == 5) {
+        ProfScope ps(h, BASTR_K_FORCE);
+        if (fm == 1 || fm == 5) {
+            k3d::Energy3Args<T> e{};
+            e.g = geo3<T>(h);
+            e.gate = gstep;
+            for (int q = 0; q < 3; ++q) e.U[q] = C<T>(h->U[q]);
+            e.part_en = h->part_en;
+            LAUNCHCHK(k3d::launch_energy3<T>(e, h->nblk_en, h->stream));
+        }
+        k3d::Force3Args<T> f{};
+        f.g = geo3<T>(h);
+        f.gate = gstep;
+        for (int q = 0; q < 3; ++q) f.U[q] = C<T>(h->U[q]);
+        f.forcemethod = fm;
+        f.forcek = h->p.forcek;
+        f.target_energy = h->p.target_energy;
+        f.part_en = h->part_en;
+        f.nblk_en = h->nblk_en;
+        f.factor_out = h->dfactor;
+        f.diag = h->ddiag + it;
+        LAUNCHCHK(k3d::launch_force3<T>(f, h->stream));
+        if (fm == 1) {
+            k3d::Scale3Args<T> sc{};
+            sc.g = geo3<T>(h);
+            sc.gate = gstep;
+            for (int q = 0; q < 3; ++q) sc.U[q] = C<T>(h->U[q]);
+            sc.factor = h->dfactor;
+            LAUNCHCHK(k3d::launch_scale3<T>(sc, h->stream));
+        }
+    }
+    return 0;
+}
+
 int ensure_capacity(bastr_handle* h, int n) {
     if (n <= h->cap) return 0;
     if (h->ddiag) hipFree(h->ddiag);
@@ -441,7 +749,7 @@ template <typename T>
 int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     int rc;
     if (h->prologue_pending) {
-        if ((rc = run_prologue<T>(h))) return rc;
+        if ((rc = (h->nd == 3 ? run_prologue3<T>(h) : run_prologue<T>(h)))) return rc;
         h->prologue_pending = false;
     }
     if ((rc = ensure_capacity(h, nsteps))) return rc;
@@ -450,7 +758,7 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     hipLaunchKernelGGL(fill_diag_kernel, dim3((nsteps + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nsteps);
     HIPCHK(hipGetLastError());
     for (int it = 0; it < nsteps; ++it)
-        if ((rc = run_iteration<T>(h, it))) return rc;
+        if ((rc = (h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it)))) return rc;
     int status = INT_MAX;
     HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -487,9 +795,12 @@ const char* bastr_last_error(void) { return g_err; }
 int bastr_create(const bastr_params* p, bastr_handle** out) {
     if (!p || !out) return fail(BASTR_EINVAL, "bastr_create: NULL argument");
     *out = nullptr;
-    if (p->ka != 0) return fail(BASTR_EINVAL, "bastr_create: 3-D grids (ka=%d) are not supported by this build", p->ka);
-    if (!pow2_in(p->ia, 16, 4096) || !pow2_in(p->ja, 16, 4096))
+    const int nd = p->ka == 0 ? 2 : 3;
+    if (nd == 2 && (!pow2_in(p->ia, 16, 4096) || !pow2_in(p->ja, 16, 4096)))
         return fail(BASTR_EINVAL, "bastr_create: ia, ja must be powers of two in [16, 4096] (got %d, %d)", p->ia, p->ja);
+    if (nd == 3 && (!pow2_in(p->ia, 16, 512) || p->ja != p->ia || p->ka != p->ia))
+        return fail(BASTR_EINVAL, "bastr_create: 3-D grids must be cubic, a power of two in [16, 512] (got %d %d %d)",
+                    p->ia, p->ja, p->ka);
     if (p->timemethod != 1 && p->timemethod != 2)
         return fail(BASTR_EINVAL, "Undefined time method %d (1 = RK3, 2 = CN)", p->timemethod);
     if (p->forcemethod < 0 || p->forcemethod > 5 || p->forcemethod == 3 || p->forcemethod == 4)
@@ -504,9 +815,10 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
 
     bastr_handle* h = new bastr_handle();
     h->p = *p;
-    h->nd = 2;
+    h->nd = nd;
     h->n0 = p->ia;
     h->n1 = p->ja;
+    h->n2 = nd == 3 ? p->ka : 1;
     h->prec = p->precision;
     h->esz = p->precision == BASTR_PREC_F32 ? 4 : 8;
     h->allkmax = p->ia / 3 < p->kmax ? p->ia / 3 : p->kmax;
@@ -516,27 +828,45 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     h->nh = h->n0 / 2 + 1;
     const int cb = col_block_width(h->n1, h->prec);
     h->hp = (h->nh + cb - 1) / cb * cb;
-    h->npts = (size_t)h->n0 * h->n1;
-    h->nhalf = (size_t)h->hp * h->n1;
+    h->npts = (size_t)h->n0 * h->n1 * h->n2;
+    h->nhalf = (size_t)h->hp * h->n1 * h->n2;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(BASTR_EDEVICE, "bastr_create: hipStreamCreate failed");
     }
     bool ok = true;
     const size_t cbytes = h->nhalf * 2 * h->esz;
-    for (int c = 0; c < 2; ++c) {
-        h->U[c] = dalloc(cbytes, &ok);
-        h->UB[c] = dalloc(cbytes, &ok);
-        h->KA[c] = dalloc(cbytes, &ok);
-        h->NX[c] = dalloc(cbytes, &ok);
-        h->ureal[c] = dalloc(h->npts * h->esz, &ok);
+    if (nd == 2) {
+        for (int c = 0; c < 2; ++c) {
+            h->U[c] = dalloc(cbytes, &ok);
+            h->UB[c] = dalloc(cbytes, &ok);
+            h->KA[c] = dalloc(cbytes, &ok);
+            h->NX[c] = dalloc(cbytes, &ok);
+            h->ureal[c] = dalloc(h->npts * h->esz, &ok);
+        }
+        for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes, &ok);
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            h->U[c] = dalloc(cbytes, &ok);
+            h->UB[c] = dalloc(cbytes, &ok);
+            h->KA[c] = dalloc(cbytes, &ok);
+        }
+        for (int q = 0; q < 10; ++q) h->Z[q] = dalloc(cbytes, &ok);
+        for (int q = 0; q < 13; ++q) h->W[q] = dalloc(cbytes, &ok);
+        for (int c = 0; c < 3; ++c) {
+            h->NX[c] = h->Z[c];
+            h->NY[c] = h->Z[3 + c];
+            h->ureal[c] = h->W[c];  // a real n0*n1*n2 field fits in one half-spectrum buffer
+        }
     }
-    for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes, &ok);
     h->dstage = (double*)dalloc(h->npts * sizeof(double), &ok);
     const int nbx = (h->nh + 63) / 64;
-    const long ntask = (long)h->n1 * nbx;
+    const long ntask = (long)h->n1 * h->n2 * nbx;
     h->nblk_spec = (int)std::max(1L, std::min(512L, ntask / 8));
-    h->nblk_vg = velgrad_blocks(h->n0, h->n1, h->prec);
+    {
+        const int L = row_block_rows(h->n0, h->prec);
+        h->nblk_vg = (h->n1 * h->n2 + L - 1) / L;
+    }
     h->part_bins = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * (h->allkmax + 1) * 2, &ok);
     h->part_tot = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * 4, &ok);
     h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8, &ok);
@@ -559,17 +889,19 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     const int K = h->allkmax;
     h->kn.assign(K + 1, 0.0);
     h->counts.assign(K + 1, 0);
-    for (int j = 0; j < h->n1; ++j)
-        for (int i = 0; i < h->n0; ++i) {
-            const int k1 = i <= h->n0 / 2 ? i : i - h->n0;
-            const int k2 = j <= h->n1 / 2 ? j : j - h->n1;
-            const double kk = std::sqrt((double)k1 * k1 + (double)k2 * k2);
-            const int s = (int)std::lround(kk);
-            if (s <= K) {
-                h->counts[s] += 1;
-                h->kn[s] += kk;
+    for (int k = 0; k < h->n2; ++k)
+        for (int j = 0; j < h->n1; ++j)
+            for (int i = 0; i < h->n0; ++i) {
+                const int k1 = i <= h->n0 / 2 ? i : i - h->n0;
+                const int k2 = j <= h->n1 / 2 ? j : j - h->n1;
+                const int k3 = nd == 3 ? (k <= h->n2 / 2 ? k : k - h->n2) : 0;
+                const double kk = std::sqrt((double)k1 * k1 + (double)k2 * k2 + (double)k3 * k3);
+                const long sh = std::lround(kk);
+                if (sh <= K) {
+                    h->counts[sh] += 1;
+                    h->kn[sh] += kk;
+                }
             }
-        }
     for (int b = 1; b <= K; ++b) h->kn[b] = h->kn[b] / (double)h->counts[b];
     *out = h;
     return BASTR_OK;
@@ -578,14 +910,23 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
 int bastr_destroy(bastr_handle* h) {
     if (!h) return BASTR_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->U[0], h->U[1], h->UB[0], h->UB[1], h->KA[0], h->KA[1], h->NX[0], h->NX[1],
-                    h->ureal[0], h->ureal[1], h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf,
-                    h->part_bins, h->part_tot, h->part_vg, h->red, h->part_en, h->dfactor,
-                    h->status, h->ddiag, h->dspec};
+    void* bufs[] = {h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf, h->part_bins, h->part_tot,
+                    h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec};
     for (void* b : bufs)
         if (b) hipFree(b);
-    for (int q = 0; q < 7; ++q)
+    for (int c = 0; c < 3; ++c) {
+        if (h->U[c]) hipFree(h->U[c]);
+        if (h->UB[c]) hipFree(h->UB[c]);
+        if (h->KA[c]) hipFree(h->KA[c]);
+        if (h->nd == 2) {  // 3-D NX/ureal alias other buffers
+            if (h->NX[c]) hipFree(h->NX[c]);
+            if (h->ureal[c]) hipFree(h->ureal[c]);
+        }
+    }
+    for (int q = 0; q < 13; ++q)
         if (h->W[q]) hipFree(h->W[q]);
+    for (int q = 0; q < 10; ++q)
+        if (h->Z[q]) hipFree(h->Z[q]);
     for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -595,10 +936,9 @@ int bastr_destroy(bastr_handle* h) {
 int bastr_allkmax(const bastr_handle* h) { return h ? h->allkmax : -1; }
 
 int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const double* u3) {
-    (void)u3;
-    if (!h || !u1 || !u2) return fail(BASTR_EINVAL, "bastr_set_fields: NULL argument");
-    const double* src[2] = {u1, u2};
-    for (int c = 0; c < 2; ++c) {
+    if (!h || !u1 || !u2 || (h && h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_set_fields: NULL argument");
+    const double* src[3] = {u1, u2, u3};
+    for (int c = 0; c < h->nd; ++c) {
         if (h->prec == BASTR_PREC_F64) {
             HIPCHK(hipMemcpy(h->ureal[c], src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
         } else {
@@ -614,20 +954,24 @@ int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const 
     return BASTR_OK;
 }
 
+static int prologue_any(bastr_handle* h) {
+    if (h->nd == 3) return h->prec == BASTR_PREC_F32 ? run_prologue3<float>(h) : run_prologue3<double>(h);
+    return h->prec == BASTR_PREC_F32 ? run_prologue<float>(h) : run_prologue<double>(h);
+}
+
 int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3) {
-    (void)u3;
-    if (!h || !u1 || !u2) return fail(BASTR_EINVAL, "bastr_get_fields: NULL argument");
+    if (!h || !u1 || !u2 || (h && h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_get_fields: NULL argument");
     if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_get_fields: fields not set");
     int rc;
     if (h->prologue_pending) {
-        rc = h->prec == BASTR_PREC_F32 ? run_prologue<float>(h) : run_prologue<double>(h);
-        if (rc) return rc;
+        if ((rc = prologue_any(h))) return rc;
         h->prologue_pending = false;
     }
-    rc = h->prec == BASTR_PREC_F32 ? run_to_physical<float>(h) : run_to_physical<double>(h);
+    if (h->nd == 3) rc = h->prec == BASTR_PREC_F32 ? run_to_physical3<float>(h) : run_to_physical3<double>(h);
+    else rc = h->prec == BASTR_PREC_F32 ? run_to_physical<float>(h) : run_to_physical<double>(h);
     if (rc) return rc;
-    double* dst[2] = {u1, u2};
-    for (int c = 0; c < 2; ++c) {
+    double* dst[3] = {u1, u2, u3};
+    for (int c = 0; c < h->nd; ++c) {
         if (h->prec == BASTR_PREC_F64) {
             HIPCHK(hipMemcpyAsync(dst[c], h->ureal[c], sizeof(double) * h->npts, hipMemcpyDeviceToHost, h->stream));
         } else {
@@ -670,6 +1014,47 @@ int bastr_shell_counts(const bastr_handle* h, int64_t* counts) {
     return BASTR_OK;
 }
 
+static int fft_c2c3(bastr_handle* h, bool inverse, const cx<double>* tw) {
+    const Gate open{h->status, 0, 0};
+    cx<double>* z = (cx<double>*)h->cbuf;
+    Geo<double> g = geo<double>(h);
+    g.n1 = h->n1 * h->n2;  // rows of the x pass
+    RowArgs<double> r{};
+    r.g = g;
+    r.gate = open;
+    r.cin = z;
+    r.cout = z;
+    r.tw = tw;
+    k3d::ColXArgs<double> y{};
+    y.gate = open;
+    y.in[0] = z;
+    y.out[0] = z;
+    y.job[0] = job1(0, k3d::M_ONE, 0);
+    y.njob = 1;
+    y.pitch = h->n0;
+    y.ncols = h->n0;
+    y.plane_stride = (long)h->n0 * h->n1;
+    y.nplanes = h->n2;
+    y.scale = 1.0;
+    y.tw = tw;
+    k3d::ColXArgs<double> zc = y;
+    zc.pitch = h->n0 * h->n1;
+    zc.ncols = h->n0 * h->n1;
+    zc.plane_stride = 0;
+    zc.nplanes = 1;
+    if (!inverse) {
+        LAUNCHCHK(launch_row<double>(h->n0, R_C2C_FWD, r, h->n1 * h->n2, h->stream));
+        LAUNCHCHK(k3d::launch_colx<double>(h->n1, false, y, h->stream));
+        zc.scale = 1.0 / ((double)h->n0 * h->n1 * h->n2);  // fft3d normalisation (fftwlink.F90:347)
+        LAUNCHCHK(k3d::launch_colx<double>(h->n2, false, zc, h->stream));
+    } else {
+        LAUNCHCHK(k3d::launch_colx<double>(h->n2, true, zc, h->stream));
+        LAUNCHCHK(k3d::launch_colx<double>(h->n1, true, y, h->stream));
+        LAUNCHCHK(launch_row<double>(h->n0, R_C2C_INV, r, h->n1 * h->n2, h->stream));
+    }
+    return 0;
+}
+
 static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
     if (!h || !inout) return fail(BASTR_EINVAL, "bastr_fft: NULL argument");
     const size_t bytes = sizeof(double) * 2 * h->npts;
@@ -691,7 +1076,10 @@ static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
     const Gate open{h->status, 0, 0};
     Geo<double> g = geo<double>(h);
     cx<double>* z = (cx<double>*)h->cbuf;
-    if (!inverse) {
+    if (h->nd == 3) {
+        int rc = fft_c2c3(h, inverse, tw0);
+        if (rc) return rc;
+    } else if (!inverse) {
         RowArgs<double> r{};
         r.g = g;
         r.gate = open;

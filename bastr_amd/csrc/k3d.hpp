@@ -1,0 +1,146 @@
+// k3d.hpp -- 3-D pseudo-spectral step kernels (gfx950).
+//
+// Layout in HBM (x = index i of the reference, fastest):
+//   physical real field  u[z][y][x]              (Fortran u(i,j,k))
+//   half spectrum        S[kz][ky][kx], kx = 0..n0/2, row pitch hp >= n0/2+1
+// Passes:
+//   x  row kernels over the n1*n2 rows (pack two real fields per complex FFT)
+//   y  column passes inside each kz plane (pitch hp, n2 planes)
+//   z  column passes whose columns are all (ky, kx) of a plane (pitch hp*n1)
+#pragma once
+#include "fft_block.hpp"
+#include "k2d.hpp"
+
+namespace k3d {
+
+using k2d::ColCfg;
+using k2d::Gate;
+using k2d::RowCfg;
+
+// Multiplier of a column-pass term, a function of the signed wavenumber k of
+// the position along the pass (ko = k with the Nyquist entry zeroed for odd
+// powers, the Hermitian part of a c2c real-part inverse).
+enum Mult { M_ONE = 0, M_IK = 1, M_MK2 = 2, M_MIK3 = 3 };
+
+struct ColJob {
+    int nterm;
+    int in[4];
+    int mult[4];
+    int out;
+};
+
+template <typename T>
+struct ColXArgs {
+    Gate gate;
+    const cx<T>* in[10];
+    cx<T>* out[13];
+    ColJob job[13];
+    int njob;
+    int pitch;           // elements between consecutive positions along the pass
+    int ncols;           // columns per plane (multiple of CB)
+    long plane_stride;   // elements between planes
+    int nplanes;
+    double scale;        // applied to every output (1: none)
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct Geo3 {
+    int n0, n1, n2;
+    int nh, hp;
+    int cut2;
+    int lproject;
+    double nu, dt, inv_n;
+};
+
+enum Cf3Mode { C3_PLAIN = 0, C3_PROLOGUE = 1, C3_A = 2, C3_B = 3, C3_C = 4, C3_CN_MID = 5, C3_CN_LAST = 6 };
+
+// z-forward pass + spectral update of the three components
+template <typename T>
+struct Cf3Args {
+    Geo3<T> g;
+    Gate gate;
+    const cx<T>* nx[3];   // y-transformed inputs
+    cx<T>* U[3];
+    cx<T>* UB[3];
+    cx<T>* KA[3];
+    cx<T>* out[3];        // C3_PLAIN
+    int nfield;
+    int cn_first;
+    int pitch;            // hp*n1 (z pass) or n0*n1 (c2c)
+    int ncols;
+    const cx<T>* tw;
+};
+
+enum Row3Mode { R3_FWD_PACK = 0, R3_INV_PACK = 1, R3_STAGE = 2 };
+
+template <typename T>
+struct Row3Args {
+    Geo3<T> g;
+    Gate gate;
+    const cx<T>* hin[9];  // R3_STAGE: Y1 Y2 Y3, Y2_1 Y2_2 Y2_3, Y3_1 Y3_2 Y3_3 ; R3_INV_PACK: A1 A2 A3
+    cx<T>* hout[3];
+    const T* rin[3];
+    T* rout[3];
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct Vg3Args {
+    Geo3<T> g;
+    Gate gate;
+    const cx<T>* w[13];   // Y_c(3), Y2_c(3), Y3_c(3), L_c(3), D
+    double* part;         // [8][gridDim.x]
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct Spec3Args {
+    Geo3<T> g;
+    Gate gate;
+    const cx<T>* U[3];
+    int allkmax;
+    double* part_bins;    // [(K+1)*2][gridDim.x]
+    double* part_tot;     // [4][gridDim.x]
+};
+
+template <typename T>
+struct Force3Args {
+    Geo3<T> g;
+    Gate gate;
+    cx<T>* U[3];
+    int forcemethod;
+    int forcek;
+    double target_energy;
+    const double* part_en;
+    int nblk_en;
+    double* factor_out;
+    bastr_diag* diag;
+};
+
+template <typename T>
+struct Energy3Args {
+    Geo3<T> g;
+    Gate gate;
+    const cx<T>* U[3];
+    double* part_en;
+};
+
+template <typename T>
+struct Scale3Args {
+    Geo3<T> g;
+    Gate gate;
+    cx<T>* U[3];
+    const double* factor;
+};
+
+template <typename T> int launch_colx(int n, bool inverse, const ColXArgs<T>& a, hipStream_t s);
+template <typename T> int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s);
+template <typename T> int launch_row3(int n, int mode, const Row3Args<T>& a, int nrows, hipStream_t s);
+template <typename T> int launch_vg3(int n, const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk);
+template <typename T> int launch_spec3(const Spec3Args<T>& a, int nblk, hipStream_t s);
+template <typename T> int launch_force3(const Force3Args<T>& a, hipStream_t s);
+template <typename T> int launch_energy3(const Energy3Args<T>& a, int nblk, hipStream_t s);
+template <typename T> int launch_scale3(const Scale3Args<T>& a, hipStream_t s);
+
+}  // namespace k3d

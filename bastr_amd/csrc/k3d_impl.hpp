@@ -1,0 +1,784 @@
+// k3d_impl.hpp -- kernel bodies of the 3-D step (included once per precision).
+//
+// Reference correspondence (src/solution.F90, src/tool.F90):
+//   colx_kernel (inverse jobs)  spectral derivatives (compute_ut3D :874-885,
+//                               velgrad_calculate3D :1224-1237), z and y halves
+//                               of the inverse transforms
+//   row3_kernel<R3_STAGE>       x half of those transforms, -u_j d_j u_i (:903-908),
+//                               x half of fft3d (:912-914)
+//   colx_kernel (forward) +
+//   cf3_kernel<C3_A/B/C>        y and z halves of fft3d, normalisation, +nu*Lap(u),
+//                               dealiasing3D (tool.F90:196-217), projection3D
+//                               (:245-270), RK33D (:233-270) / CN3D (:272-301)
+//   spec3_kernel                spectra_compute3D (:1018-1108)
+//   vg3_kernel                  velgrad_calculate3D (:1188-1286)
+//   force3_kernel               forcing3D methods 1, 2, 5 (:542-814)
+#pragma once
+#include <climits>
+#include "k2d_impl.hpp"
+#include "k3d.hpp"
+
+namespace k3d {
+
+using k2d::ksigned;
+using k2d::pack_at;
+using k2d::unpack_store;
+using k2d::xcd_block;
+
+template <typename T>
+__device__ __forceinline__ cx<T> apply_mult(cx<T> u, int mult, T k, T ko) {
+    switch (mult) {
+        case M_IK: return cscale(cmuli(u), ko);
+        case M_MK2: return cscale(u, -(k * k));
+        case M_MIK3: return cscale(cmuli(u), -(ko * ko * ko));
+        default: return u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// generic column pass: out_j = FFT(sum_t mult_t(k) in_t) for each job j
+// ---------------------------------------------------------------------------
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void colx_kernel(ColXArgs<T> a) {
+    using C = ColCfg<T, N>;
+    constexpr int E = C::E, P = C::P, CB = C::CB;
+    if (a.gate.closed()) return;
+    __shared__ T lds[CB * lds_line(N)];
+    const int nbpp = a.ncols / CB;
+    const int blk = xcd_block(blockIdx.x, nbpp * a.nplanes);
+    if (blk < 0) return;
+    const int plane = blk / nbpp, cb = blk % nbpp;
+    const int tid = threadIdx.x;
+    const int l = tid % CB, j = tid / CB;
+    const size_t base = (size_t)plane * a.plane_stride + (size_t)cb * CB + l;
+    const size_t pitch = (size_t)a.pitch;
+    const LdsLine<T, true> L{lds + l, CB};
+    const T sc = (T)a.scale;
+    for (int q = 0; q < a.njob; ++q) {
+        const ColJob jb = a.job[q];
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = {(T)0, (T)0};
+        for (int t = 0; t < jb.nterm; ++t) {
+            const cx<T>* __restrict__ src = a.in[jb.in[t]];
+            const int mult = jb.mult[t];
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int m = j + k * P;
+                const T kk = (T)ksigned(m, N);
+                const T ko = (m == N / 2) ? (T)0 : kk;
+                v[k] = cadd(v[k], apply_mult(src[base + (size_t)m * pitch], mult, kk, ko));
+            }
+        }
+        line_fft<T, N, E, INV, true>(v, j, L, a.tw);
+        cx<T>* __restrict__ dst = a.out[jb.out];
+#pragma unroll
+        for (int k = 0; k < E; ++k) dst[base + (size_t)(j + k * P) * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z-forward pass + spectral update of three components
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void cf3_kernel(Cf3Args<T> a) {
+    using C = ColCfg<T, N>;
+    constexpr int E = C::E, P = C::P, CB = C::CB;
+    if (a.gate.closed()) return;
+    __shared__ T lds[CB * lds_line(N)];
+    const int blk = xcd_block(blockIdx.x, a.ncols / CB);
+    if (blk < 0) return;
+    const int tid = threadIdx.x;
+    const int l = tid % CB, j = tid / CB;
+    const int col = blk * CB + l;
+    const LdsLine<T, true> L{lds + l, CB};
+    const size_t pitch = (size_t)a.pitch;
+    const T sc = (T)a.g.inv_n;
+    if (MODE == C3_PLAIN) {
+        for (int f = 0; f < a.nfield; ++f) {
+            cx<T> v[E];
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = a.nx[f][(size_t)(j + k * P) * pitch + col];
+            line_fft<T, N, E, false, true>(v, j, L, a.tw);
+#pragma unroll
+            for (int k = 0; k < E; ++k) a.out[f][(size_t)(j + k * P) * pitch + col] = cscale(v[k], sc);
+        }
+        return;
+    }
+    cx<T> v[3][E];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[c][k] = a.nx[c][(size_t)(j + k * P) * pitch + col];
+        line_fft<T, N, E, false, true>(v[c], j, L, a.tw);
+    }
+    const int hp = a.g.hp;
+    const int kx = col % hp;
+    const int ky = ksigned(col / hp, a.g.n1);
+    const T dt = (T)a.g.dt;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        if (k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+        const int m = j + k * P;
+        const int kz = ksigned(m, N);
+        const int kk2 = kx * kx + ky * ky + kz * kz;
+        const size_t idx = (size_t)m * pitch + col;
+        cx<T> t[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = cscale(v[c][k], sc);
+        if (MODE == C3_PROLOGUE) {
+            // mainloop3D entry: u_hat = dealias(fft3d(u)) (solution.F90:111-116)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.U[c][idx] = (kk2 >= a.g.cut2) ? cx<T>{0, 0} : t[c];
+            continue;
+        }
+        cx<T> U[3], s[3], ka[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U[c] = a.U[c][idx];
+        if (MODE == C3_A) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s[c] = U[c];
+        } else if (MODE == C3_CN_MID || MODE == C3_CN_LAST) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s[c] = a.cn_first ? U[c] : a.UB[c][idx];
+        } else if (MODE == C3_B) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ka[c] = a.KA[c][idx];
+                s[c] = cadd(U[c], cscale(cscale(ka[c], dt), (T)0.5));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ka[c] = a.KA[c][idx];
+                s[c] = a.UB[c][idx];
+            }
+        }
+        const T vis = (T)(-a.g.nu * (double)kk2);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = cadd(t[c], cscale(s[c], vis));
+        if (kk2 >= a.g.cut2) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t[c] = {0, 0};
+        }
+        if (a.g.lproject) {
+            if (kk2 > 0) {
+                const T kkr = sqrt((T)kk2);
+                const T c1 = (T)kx / kkr, c2 = (T)ky / kkr, c3 = (T)kz / kkr;
+                const cx<T> ud = cadd(cadd(cscale(t[0], c1), cscale(t[1], c2)), cscale(t[2], c3));
+                t[0] = cscale(ud, c1);
+                t[1] = cscale(ud, c2);
+                t[2] = cscale(ud, c3);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) t[c] = {0, 0};
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (MODE == C3_A) {
+                a.KA[c][idx] = t[c];
+                a.UB[c][idx] = cadd(U[c], cscale(cscale(t[c], dt), (T)0.5));
+            } else if (MODE == C3_B) {
+                const T dt2 = (T)2 * dt;
+                a.UB[c][idx] = cadd(csub(U[c], cscale(ka[c], dt)), cscale(t[c], dt2));
+                a.KA[c][idx] = cadd(ka[c], cscale(t[c], (T)4));
+            } else if (MODE == C3_C) {
+                a.U[c][idx] = cadd(U[c], cscale(cscale(cadd(ka[c], t[c]), dt), (T)(1.0 / 6.0)));
+            } else if (MODE == C3_CN_MID) {
+                const cx<T> nw = cadd(U[c], cscale(t[c], dt));
+                a.UB[c][idx] = cscale(cadd(nw, U[c]), (T)0.5);
+            } else {  // C3_CN_LAST
+                a.U[c][idx] = cadd(U[c], cscale(t[c], dt));
+            }
+        }
+    }
+}
+
+// one real field as a packed row (second field zero)
+template <typename T, int N>
+__device__ __forceinline__ cx<T> pack1(const cx<T>* __restrict__ A, int m) {
+    if (m == 0 || m == N / 2) return {A[m].x, (T)0};
+    if (m < N / 2) return A[m];
+    return cconj(A[N - m]);
+}
+
+// pack (A, i*k1*A) for one half row A: value at m of A + i*(i k1 A)
+template <typename T, int N>
+__device__ __forceinline__ cx<T> pack_a_ika(const cx<T>* __restrict__ A, int m) {
+    const int h = m <= N / 2 ? m : N - m;
+    const T ko = (h == N / 2) ? (T)0 : (T)h;
+    const cx<T> a = A[h];
+    const cx<T> b = cscale(cmuli(a), ko);  // i k1 A at h
+    if (m == 0 || m == N / 2) return {a.x, b.x};
+    if (m < N / 2) return {a.x - b.y, a.y + b.x};
+    return {a.x + b.y, b.x - a.y};
+}
+
+// ---------------------------------------------------------------------------
+// x passes
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<T> a) {
+    using C = RowCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * lds_line(N)];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int nrows = a.g.n1 * a.g.n2;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < nrows;
+    const int row = valid ? row0 : nrows - 1;
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const size_t hp = (size_t)a.g.hp;
+    const size_t ro = (size_t)row * hp;
+
+    if (MODE == R3_FWD_PACK) {
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const size_t o = (size_t)row * N + j + k * P;
+            v[k] = {a.rin[0][o], a.rin[1][o]};
+        }
+        line_fft<T, N, E, false, false>(v, j, Ls, a.tw);
+        unpack_store<T, N, E>(v, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid);
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = {a.rin[2][(size_t)row * N + j + k * P], (T)0};
+        line_fft<T, N, E, false, false>(v, j, Ls, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int h = j + k * P;
+                if (h <= N / 2) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{v[k].x, (T)0} : v[k];
+            }
+        }
+        return;
+    }
+    if (MODE == R3_INV_PACK) {
+        cx<T> v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(a.hin[0] + ro, a.hin[1] + ro, j + k * P);
+        line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const size_t o = (size_t)row * N + j + k * P;
+                a.rout[0][o] = v[k].x;
+                a.rout[1][o] = v[k].y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = pack1<T, N>(a.hin[2] + ro, j + k * P);
+        line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) a.rout[2][(size_t)row * N + j + k * P] = v[k].x;
+        }
+        return;
+    }
+    // R3_STAGE: u, grad u from nine half rows, nonlinear term, x-forward
+    const cx<T>* Y1 = a.hin[0] + ro;
+    const cx<T>* Y2 = a.hin[1] + ro;
+    const cx<T>* Y3 = a.hin[2] + ro;
+    cx<T> z[E], w[E], n12[E];
+    T n3[E], u3[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(Y1, Y2, j + k * P);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        u3[k] = w[k].x;
+        n3[k] = -(z[k].x * w[k].y);
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
+        w[k] = cscale(cmuli(pack_at<T, N>(Y1, Y2, m)), kxs);
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
+#pragma unroll
+    for (int k = 0; k < E; ++k) n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[3] + ro, a.hin[4] + ro, j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u1, d2 u2)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        n12[k].x = n12[k].x - z[k].y * w[k].x;
+        n12[k].y = n12[k].y - z[k].y * w[k].y;
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[6] + ro, a.hin[7] + ro, j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d3 u1, d3 u2)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        n12[k].x = n12[k].x - u3[k] * w[k].x;
+        n12[k].y = n12[k].y - u3[k] * w[k].y;
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u3, d3 u3)
+#pragma unroll
+    for (int k = 0; k < E; ++k) n3[k] = n3[k] - z[k].y * w[k].x - u3[k] * w[k].y;
+    line_fft<T, N, E, false, false>(n12, j, Ls, a.tw);
+    unpack_store<T, N, E>(n12, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid);
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = {n3[k], (T)0};
+    line_fft<T, N, E, false, false>(w, j, Ls, a.tw);
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int h = j + k * P;
+            if (h <= N / 2) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{w[k].x, (T)0} : w[k];
+        }
+    }
+}
+
+// velocity-gradient moments, 3-D (velgrad_calculate3D, solution.F90:1188-1286)
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void vg3_kernel(Vg3Args<T> a) {
+    using C = RowCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ double red[C::THREADS / 64][8];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int nrows = a.g.n1 * a.g.n2;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < nrows;
+    const int row = valid ? row0 : nrows - 1;
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const size_t ro = (size_t)row * a.g.hp;
+    const cx<T>* W[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
+    const double nu = a.g.nu;
+    const double eta0 = (double)(2.0f * 3.14f);
+    // W: 0..2 Y_c, 3..5 Y2_c (d2), 6..8 Y3_c (d3), 9..11 L_c, 12 D
+    cx<T> z[E], w[E];
+    T u3[E], a31[E], a11[E], a21[E], a22[E], a23[E];
+    double e12[E], e13[E], c12[E], c13[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(W[2], j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, a31 = d1 u3
+#pragma unroll
+    for (int k = 0; k < E; ++k) { u3[k] = w[k].x; a31[k] = w[k].y; }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
+        w[k] = cscale(cmuli(pack_at<T, N>(W[0], W[1], m)), kxs);
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a11 = d1 u1, a21 = d1 u2
+#pragma unroll
+    for (int k = 0; k < E; ++k) { a11[k] = w[k].x; a21[k] = w[k].y; }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[3], W[4], j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a12 = d2 u1, a22 = d2 u2
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double a12 = w[k].x;
+        a22[k] = w[k].y;
+        c12[k] = 2.0 * a12 * (double)a21[k];
+        e12[k] = (a12 - (double)a21[k]) * (a12 - (double)a21[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[6], W[7], j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a13 = d3 u1, a23 = d3 u2
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double a13 = w[k].x;
+        a23[k] = w[k].y;
+        c13[k] = 2.0 * a13 * (double)a31[k];
+        e13[k] = (a13 - (double)a31[k]) * (a13 - (double)a31[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[5], W[8], j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a32 = d2 u3, a33 = d3 u3
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
+    double dv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
+        const double b11 = a11[k], b22 = a22[k], b33 = w[k].y, b23 = a23[k], b32 = w[k].x;
+        const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
+        const double div = b11 + b22 + b33;
+        dv[k] = div;
+        s_um2 += div * div * uu;
+        s_ijji += (b11 * b11 + b22 * b22 + b33 * b33 + c12[k] + c13[k] + 2.0 * b23 * b32) * uu;
+        s_u2t += div * uu;
+        s_eps += nu * e12[k] + nu * (b23 - b32) * (b23 - b32) + nu * e13[k] + nu * div * div;
+        eta = fmin(eta, sqrt(nu / fabs(div)));
+        s_urms += u1 * u1 + u2 * u2 + uu3 * uu3;
+        s_dudx += b11 * b11 + b22 * b22 + b33 * b33;
+    }
+    // Lap(u1), Lap(u2) = x-inverse of (-k1^2 Y_c + L_c)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1sq = (T)h * (T)h;
+        w[k] = cadd(cscale(pack_at<T, N>(W[0], W[1], m), -k1sq), pack_at<T, N>(W[9], W[10], m));
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        s_dis += 2.0 * (double)z[k].x * dv[k] * (double)w[k].x + 2.0 * (double)z[k].y * dv[k] * (double)w[k].y;
+    // (Lap(u3), Lap(theta)): Lap(u3) from (-k1^2 Y_3 + L_3); Lap(theta) from
+    // -k1^2 (i k1 Y_1 + Y2_2 + Y3_3) + i k1 L_1 + D
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1 = (T)h;
+        const T k1o = (h == N / 2) ? (T)0 : k1;
+        const cx<T> lu3 = cadd(cscale(W[2][h], -(k1 * k1)), W[11][h]);
+        const cx<T> th = cadd(cscale(cmuli(W[0][h]), k1o), cadd(W[4][h], W[8][h]));
+        const cx<T> lth = cadd(cadd(cscale(th, -(k1 * k1)), cscale(cmuli(W[9][h]), k1o)), W[12][h]);
+        if (m == 0 || m == N / 2) w[k] = {lu3.x, lth.x};
+        else if (m < N / 2) w[k] = {lu3.x - lth.y, lu3.y + lth.x};
+        else w[k] = {lu3.x + lth.y, lth.x - lu3.y};
+    }
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
+        const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
+        s_dis += 2.0 * uu3 * dv[k] * (double)w[k].x + uu * (double)w[k].y;
+    }
+    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
+    const int wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
+    vals[7] = wave_min(vals[7]);
+    __syncthreads();
+    if (lane == 0)
+        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
+    __syncthreads();
+    if (tid < 8) {
+        double acc = tid == 7 ? eta0 : 0.0;
+        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
+        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+// shell spectra, 3-D (spectra_compute3D :1018-1108): u_d = (u.k)/|k|,
+// u_s = u - u_d k/|k|; same deterministic per-wave binning as in 2-D.
+template <typename T>
+__global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
+    if (a.gate.closed()) return;
+    extern __shared__ double sh[];
+    const int K = a.allkmax, nb = K + 1;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    double* bins = sh;
+    double* tots = sh + 4 * nb * 2;
+    for (int i = tid; i < 4 * nb * 2; i += 256) bins[i] = 0;
+    __syncthreads();
+    const int n0 = a.g.n0, n1 = a.g.n1, n2 = a.g.n2, nh = a.g.nh;
+    const size_t hp = (size_t)a.g.hp;
+    const int nbx = (nh + 63) / 64;
+    const long ntask = (long)n1 * n2 * nbx;
+    const long tstride = 4L * gridDim.x;
+    double tot[4] = {0, 0, 0, 0};
+    for (long t = (long)blockIdx.x * 4 + wave; t < ntask; t += tstride) {
+        const long r = t / nbx;  // row = kz*n1 + ky
+        const int kx = (int)(t % nbx) * 64 + lane;
+        double es = 0, ed = 0, w = 0;
+        int s = INT_MAX;
+        if (kx < nh) {
+            const int ky = ksigned((int)(r % n1), n1), kz = ksigned((int)(r / n1), n2);
+            w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+            const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
+            if (kk > 0.5) {
+                const size_t o = (size_t)r * hp + kx;
+                const cx<T> u1 = a.U[0][o], u2 = a.U[1][o], u3 = a.U[2][o];
+                const double ik = 1.0 / kk;
+                const double c1 = kx * ik, c2 = ky * ik, c3 = kz * ik;
+                const double udx = (double)u1.x * c1 + (double)u2.x * c2 + (double)u3.x * c3;
+                const double udy = (double)u1.y * c1 + (double)u2.y * c2 + (double)u3.y * c3;
+                const double s1x = u1.x - udx * c1, s1y = u1.y - udy * c1;
+                const double s2x = u2.x - udx * c2, s2y = u2.y - udy * c2;
+                const double s3x = u3.x - udx * c3, s3y = u3.y - udy * c3;
+                es = (s1x * s1x + s1y * s1y) * 0.5 + (s2x * s2x + s2y * s2y) * 0.5 + (s3x * s3x + s3y * s3y) * 0.5;
+                ed = (udx * udx + udy * udy) * 0.5;
+                tot[3] += w * (ed * ik);
+            }
+            tot[0] += w * es;
+            tot[1] += w * ed;
+            tot[2] += w * (ed * (kk * kk));
+            const int sh_ = (int)(kk + 0.5);
+            if (sh_ <= K) s = sh_;
+        }
+        const int smin = wave_min_i(s);
+        if (smin != INT_MAX) {
+            const int smax = wave_max_i(s == INT_MAX ? -1 : s);
+            for (int b = smin; b <= smax; ++b) {
+                const double des = wave_sum(s == b ? w * es : 0.0);
+                const double ded = wave_sum(s == b ? w * ed : 0.0);
+                if (lane == 0) {
+                    bins[(wave * nb + b) * 2 + 0] += des;
+                    bins[(wave * nb + b) * 2 + 1] += ded;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tot[q] = wave_sum(tot[q]);
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) tots[wave * 4 + q] = tot[q];
+    __syncthreads();
+    for (int b = tid; b < nb; b += 256) {
+        for (int c = 0; c < 2; ++c) {
+            double acc = 0;
+            for (int wv = 0; wv < 4; ++wv) acc += bins[(wv * nb + b) * 2 + c];
+            a.part_bins[((size_t)b * 2 + c) * gridDim.x + blockIdx.x] = acc;
+        }
+    }
+    if (tid < 4) {
+        double acc = 0;
+        for (int wv = 0; wv < 4; ++wv) acc += tots[wv * 4 + tid];
+        a.part_tot[(size_t)tid * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void energy3_kernel(Energy3Args<T> a) {
+    if (a.gate.closed()) return;
+    __shared__ double red[4];
+    const int n0 = a.g.n0, nh = a.g.nh;
+    const size_t hp = (size_t)a.g.hp;
+    const size_t total = (size_t)nh * a.g.n1 * a.g.n2;
+    double e = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int kx = (int)(i % nh);
+        const size_t r = i / nh;
+        const double w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+        double s = 0;
+        for (int c = 0; c < 3; ++c) {
+            const cx<T> u = a.U[c][r * hp + kx];
+            s += (double)u.x * u.x + (double)u.y * u.y;
+        }
+        e += w * s;
+    }
+    e = wave_sum(e);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) a.part_en[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// forcing3D methods 1, 2, 5 (solution.F90:554-629, 754-813); the 3-D method 5
+// has no projected variant.
+template <typename T>
+__global__ __launch_bounds__(256) void force3_kernel(Force3Args<T> a) {
+    if (a.gate.closed()) return;
+    __shared__ double red[4];
+    __shared__ double fac;
+    const int tid = threadIdx.x;
+    const int fm = a.forcemethod;
+    double energy = 0;
+    if (fm == 1 || fm == 5) {
+        if (tid == 0) {
+            double s = 0;
+            for (int b = 0; b < a.nblk_en; ++b) s += a.part_en[b];
+            energy = fm == 1 ? s / 2 : s;
+        }
+    }
+    if (fm == 1) {
+        if (tid == 0) {
+            const double factor = sqrt(a.target_energy / energy);
+            *a.factor_out = factor;
+            a.diag->force_factor = factor;
+            a.diag->force_energy = energy;
+        }
+        return;
+    }
+    const int n0 = a.g.n0, n1 = a.g.n1, n2 = a.g.n2;
+    const size_t hp = (size_t)a.g.hp;
+    const int fk = a.forcek;
+    const int kxm = fk < n0 / 2 ? fk : n0 / 2;
+    const int kym = fk < n1 / 2 ? fk : n1 / 2 - 1;
+    const int kzm = fk < n2 / 2 ? fk : n2 / 2 - 1;
+    const int nx = kxm + 1, ny = 2 * kym + 1, nz = 2 * kzm + 1;
+    const int ntot = nx * ny * nz;
+    double e = 0;
+    for (int t = tid; t < ntot; t += 256) {
+        const int kx = t % nx, ky = (t / nx) % ny - kym, kz = t / (nx * ny) - kzm;
+        const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
+        if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
+            const size_t o = ((size_t)(kz >= 0 ? kz : n2 + kz) * n1 + (ky >= 0 ? ky : n1 + ky)) * hp + kx;
+            const double w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+            double s = 0;
+            for (int c = 0; c < 3; ++c) {
+                const cx<T> u = a.U[c][o];
+                s += (double)u.x * u.x + (double)u.y * u.y;
+            }
+            e += w * s;
+        }
+    }
+    e = wave_sum(e);
+    if (tid % 64 == 0) red[tid / 64] = e;
+    __syncthreads();
+    if (tid == 0) {
+        const double E = red[0] + red[1] + red[2] + red[3];
+        double factor;
+        if (fm == 2) factor = sqrt(a.target_energy / E) - 1.0;
+        else factor = a.target_energy > energy ? sqrt((a.target_energy - energy) / E + 1.0) - 1.0 : 0.0;
+        fac = factor;
+        a.diag->force_factor = factor;
+        a.diag->force_E = E;
+        if (fm == 5) a.diag->force_energy = energy;
+    }
+    __syncthreads();
+    const T f = (T)fac;
+    for (int t = tid; t < ntot; t += 256) {
+        const int kx = t % nx, ky = (t / nx) % ny - kym, kz = t / (nx * ny) - kzm;
+        const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
+        if ((kk - 0.5) < fk && (kk + 0.5) > fk) {
+            const size_t o = ((size_t)(kz >= 0 ? kz : n2 + kz) * n1 + (ky >= 0 ? ky : n1 + ky)) * hp + kx;
+            for (int c = 0; c < 3; ++c) {
+                const cx<T> u = a.U[c][o];
+                a.U[c][o] = cadd(u, cscale(u, f));
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale3_kernel(Scale3Args<T> a) {
+    if (a.gate.closed()) return;
+    const T f = (T)*a.factor;
+    const size_t total = (size_t)a.g.hp * a.g.n1 * a.g.n2;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
+        for (int c = 0; c < 3; ++c) a.U[c][i] = cscale(a.U[c][i], f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define K3D_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512)
+
+template <typename T, int N>
+static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
+    using C = ColCfg<T, N>;
+    if (a.ncols % C::CB || a.njob < 1 || a.njob > 13) return -1;
+    const int nb = (a.ncols / C::CB) * a.nplanes;
+    dim3 grid(k2d::xcd_grid_h(nb)), block(C::THREADS);
+    if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((colx_kernel<T, N, false>), grid, block, 0, s, a);
+    return 0;
+}
+
+template <typename T, int N>
+static int cf3_n(int mode, const Cf3Args<T>& a, hipStream_t s) {
+    using C = ColCfg<T, N>;
+    if (a.ncols % C::CB) return -1;
+    dim3 grid(k2d::xcd_grid_h(a.ncols / C::CB)), block(C::THREADS);
+    switch (mode) {
+#define M(MM) case MM: hipLaunchKernelGGL((cf3_kernel<T, N, MM>), grid, block, 0, s, a); break;
+        M(C3_PLAIN) M(C3_PROLOGUE) M(C3_A) M(C3_B) M(C3_C) M(C3_CN_MID) M(C3_CN_LAST)
+#undef M
+        default: return -1;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int row3_n(int mode, const Row3Args<T>& a, int nrows, hipStream_t s) {
+    using C = RowCfg<T, N>;
+    dim3 grid((nrows + C::L - 1) / C::L), block(C::THREADS);
+    switch (mode) {
+        case R3_FWD_PACK: hipLaunchKernelGGL((row3_kernel<T, N, R3_FWD_PACK>), grid, block, 0, s, a); break;
+        case R3_INV_PACK: hipLaunchKernelGGL((row3_kernel<T, N, R3_INV_PACK>), grid, block, 0, s, a); break;
+        case R3_STAGE: hipLaunchKernelGGL((row3_kernel<T, N, R3_STAGE>), grid, block, 0, s, a); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int vg3_n(const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
+    using C = RowCfg<T, N>;
+    const int g = (nrows + C::L - 1) / C::L;
+    if (g != nblk) return -1;
+    hipLaunchKernelGGL((vg3_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    return 0;
+}
+
+template <typename T>
+int launch_colx(int n, bool inverse, const ColXArgs<T>& a, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return colx_n<T, NN>(inverse, a, s);
+        K3D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return cf3_n<T, NN>(mode, a, s);
+        K3D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_row3(int n, int mode, const Row3Args<T>& a, int nrows, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return row3_n<T, NN>(mode, a, nrows, s);
+        K3D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_vg3(int n, const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
+    switch (n) {
+#define X(NN) case NN: return vg3_n<T, NN>(a, nrows, s, nblk);
+        K3D_SIZES(X)
+#undef X
+    }
+    return -1;
+}
+template <typename T>
+int launch_spec3(const Spec3Args<T>& a, int nblk, hipStream_t s) {
+    const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
+    hipLaunchKernelGGL((spec3_kernel<T>), dim3(nblk), dim3(256), shm, s, a);
+    return 0;
+}
+template <typename T>
+int launch_force3(const Force3Args<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((force3_kernel<T>), dim3(1), dim3(256), 0, s, a);
+    return 0;
+}
+template <typename T>
+int launch_energy3(const Energy3Args<T>& a, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL((energy3_kernel<T>), dim3(nblk), dim3(256), 0, s, a);
+    return 0;
+}
+template <typename T>
+int launch_scale3(const Scale3Args<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL((scale3_kernel<T>), dim3(2048), dim3(256), 0, s, a);
+    return 0;
+}
+
+#define K3D_INSTANTIATE(T)                                                             \
+    template int launch_colx<T>(int, bool, const ColXArgs<T>&, hipStream_t);           \
+    template int launch_cf3<T>(int, int, const Cf3Args<T>&, hipStream_t);              \
+    template int launch_row3<T>(int, int, const Row3Args<T>&, int, hipStream_t);       \
+    template int launch_vg3<T>(int, const Vg3Args<T>&, int, hipStream_t, int);         \
+    template int launch_spec3<T>(const Spec3Args<T>&, int, hipStream_t);               \
+    template int launch_force3<T>(const Force3Args<T>&, hipStream_t);                  \
+    template int launch_energy3<T>(const Energy3Args<T>&, int, hipStream_t);           \
+    template int launch_scale3<T>(const Scale3Args<T>&, hipStream_t);
+
+}  // namespace k3d
