@@ -18,6 +18,7 @@
 #include "../../include/bastr_prof.h"
 #include "k2d.hpp"
 #include "k3d.hpp"
+#include "init_rp.hpp"
 
 using namespace k2d;
 
@@ -985,6 +986,133 @@ int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3) {
     return BASTR_OK;
 }
 
+}  // extern "C"
+
+template <typename T>
+static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, double energy) {
+    const Gate open{h->status, 0, 0};
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
+    HIPCHK(hipGetLastError());
+    // 1. noise in the first physical field, zeros in the others
+    LAUNCHCHK(rp::launch_noise<T>(h->ureal[0], h->npts, seed, h->stream));
+    for (int c = 1; c < h->nd; ++c) LAUNCHCHK(rp::launch_zero<T>(h->ureal[c], h->npts, h->stream));
+    // 2. its normalised spectrum into U[0]
+    if (h->nd == 2) {
+        RowArgs<T> r{};
+        r.g = geo<T>(h);
+        r.gate = open;
+        r.rin[0] = (const T*)h->ureal[0];
+        r.rin[1] = (const T*)h->ureal[1];
+        r.hout[0] = C<T>(h->NX[0]);
+        r.hout[1] = C<T>(h->NX[1]);
+        r.tw = C<T>(h->tw0);
+        LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
+        ColFwdArgs<T> c{};
+        c.g = geo<T>(h);
+        c.gate = open;
+        c.nx[0] = C<T>(h->NX[0]);
+        c.out[0] = C<T>(h->U[0]);
+        c.nfield = 1;
+        c.pitch = h->hp;
+        c.tw = C<T>(h->tw1);
+        LAUNCHCHK(launch_col_fwd<T>(h->n1, CF_PLAIN, c, h->hp, h->stream));
+    } else {
+        k3d::Row3Args<T> r{};
+        r.g = geo3<T>(h);
+        r.gate = open;
+        for (int q = 0; q < 3; ++q) {
+            r.rin[q] = (const T*)h->ureal[q];
+            r.hout[q] = C<T>(h->NX[q]);
+        }
+        r.tw = C<T>(h->tw0);
+        LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
+        int rc = run_yfwd3<T>(h, open);
+        if (rc) return rc;
+        k3d::Cf3Args<T> c{};
+        c.g = geo3<T>(h);
+        c.gate = open;
+        c.nx[0] = C<T>(h->NY[0]);
+        c.out[0] = C<T>(h->U[0]);
+        c.nfield = 1;
+        c.pitch = h->hp * h->n1;
+        c.ncols = h->hp * h->n1;
+        c.tw = C<T>(h->tw1);
+        LAUNCHCHK(k3d::launch_cf3<T>(h->n2, k3d::C3_PLAIN, c, h->stream));
+    }
+    // 3. the spectrum law with the noise phases
+    rp::RpArgs b{};
+    for (int c = 0; c < 3; ++c) b.U[c] = h->U[c];
+    b.nd = h->nd;
+    b.n0 = h->n0;
+    b.n1 = h->n1;
+    b.n2 = h->n2;
+    b.nh = h->nh;
+    b.hp = h->hp;
+    b.k0 = k0;
+    LAUNCHCHK(rp::launch_build<T>(b, h->stream));
+    // 4. rescale to <sum u_i^2> = energy (Parseval over the half spectrum)
+    std::vector<double> part(h->nblk_en);
+    if (h->nd == 2) {
+        EnergyArgs<T> e{};
+        e.g = geo<T>(h);
+        e.gate = open;
+        e.U[0] = C<T>(h->U[0]);
+        e.U[1] = C<T>(h->U[1]);
+        e.part_en = h->part_en;
+        LAUNCHCHK(launch_energy<T>(e, h->nblk_en, h->stream));
+    } else {
+        k3d::Energy3Args<T> e{};
+        e.g = geo3<T>(h);
+        e.gate = open;
+        for (int q = 0; q < 3; ++q) e.U[q] = C<T>(h->U[q]);
+        e.part_en = h->part_en;
+        LAUNCHCHK(k3d::launch_energy3<T>(e, h->nblk_en, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(part.data(), h->part_en, sizeof(double) * h->nblk_en, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double e = 0;
+    for (double x : part) e += x;
+    if (!(e > 0)) return fail(BASTR_EINVAL, "bastr_init_random_phase: the generated field has zero energy");
+    const double factor = std::sqrt(energy / e);
+    HIPCHK(hipMemcpyAsync(h->dfactor, &factor, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->nd == 2) {
+        ScaleArgs<T> sc{};
+        sc.g = geo<T>(h);
+        sc.gate = open;
+        sc.U[0] = C<T>(h->U[0]);
+        sc.U[1] = C<T>(h->U[1]);
+        sc.factor = h->dfactor;
+        LAUNCHCHK(launch_scale<T>(sc, h->stream));
+    } else {
+        k3d::Scale3Args<T> sc{};
+        sc.g = geo3<T>(h);
+        sc.gate = open;
+        for (int q = 0; q < 3; ++q) sc.U[q] = C<T>(h->U[q]);
+        sc.factor = h->dfactor;
+        LAUNCHCHK(k3d::launch_scale3<T>(sc, h->stream));
+    }
+    // 5. physical field u = real(ifft(u_hat)): the file hitgen would write
+    int rc = h->nd == 3 ? run_to_physical3<T>(h) : run_to_physical<T>(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" {
+
+int bastr_init_random_phase(bastr_handle* h, double k0, uint64_t seed, double energy) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_init_random_phase: NULL handle");
+    if (!(k0 > 0) || !(energy > 0)) return fail(BASTR_EINVAL, "bastr_init_random_phase: k0 and energy must be > 0");
+    int rc = h->prec == BASTR_PREC_F32 ? init_random_phase_impl<float>(h, k0, seed, energy)
+                                       : init_random_phase_impl<double>(h, k0, seed, energy);
+    if (rc) return rc;
+    h->prologue_pending = true;
+    h->have_fields = true;
+    h->nstep = 0;
+    h->time = 0.0;
+    return BASTR_OK;
+}
+
 int bastr_set_time(bastr_handle* h, int64_t nstep, double time) {
     if (!h) return fail(BASTR_EINVAL, "bastr_set_time: NULL handle");
     h->nstep = nstep;
@@ -1135,9 +1263,12 @@ int bastr_prof_enable(bastr_handle* h, uint32_t mask) {
 
 int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* total_ms, double* bytes) {
     if (!h || c < 0 || c >= BASTR_K_NCLASS) return fail(BASTR_EINVAL, "bastr_prof_read: bad argument");
-    // compulsory traffic in units of one half-spectrum field (nh*n1 complex)
-    static const double units[BASTR_K_NCLASS] = {2, 9, 9, 0, 6, 6, 8, 10, 10, 0};
-    const double half = (double)h->nh * h->n1 * 2.0 * h->esz;
+    // compulsory traffic in units of one half-spectrum field (nh*n1*n2 complex);
+    // 3-D classes cover the z and y passes of one stage together
+    static const double units2[BASTR_K_NCLASS] = {2, 9, 9, 0, 6, 6, 8, 10, 10, 0};
+    static const double units3[BASTR_K_NCLASS] = {3, 36, 13, 0, 24, 12, 18, 21, 21, 0};
+    const double* units = h->nd == 3 ? units3 : units2;
+    const double half = (double)h->nh * h->n1 * h->n2 * 2.0 * h->esz;
     if (launches) *launches = h->prof_launch[c];
     if (total_ms) *total_ms = h->prof_ms[c];
     if (bytes) *bytes = units[c] * half;

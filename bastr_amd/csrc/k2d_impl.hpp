@@ -208,9 +208,18 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwd
             if (kk2 > 0) {
                 const T kkr = sqrt((T)kk2);
                 const T c1 = (T)kx / kkr, c2 = (T)ky / kkr;
-                const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
-                t0 = cscale(ud, c1);
-                t1 = cscale(ud, c2);
+                if (2 * m == N) {
+                    // y-Nyquist row (kept by the dealias when ja < 2 ia / 3): the reference
+                    // projects the full c2c spectrum, where the partner of (kx, +N/2) is
+                    // (-kx, +N/2), then keeps the real part of the inverse, so the two
+                    // projections' kx*ky cross terms cancel
+                    t0 = cscale(cscale(t0, c1), c1);
+                    t1 = cscale(cscale(t1, c2), c2);
+                } else {
+                    const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
+                    t0 = cscale(ud, c1);
+                    t1 = cscale(ud, c2);
+                }
             } else {
                 t0 = {0, 0}; t1 = {0, 0};
             }
@@ -602,12 +611,21 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
                 if (kk > 0.5) {
                     const double ik = 1.0 / kk;
                     const double c1 = kx * ik, c2 = ky * ik;
-                    const double usx = (double)u1[q].x * c2 - (double)u2[q].x * c1;
-                    const double usy = (double)u1[q].y * c2 - (double)u2[q].y * c1;
-                    const double udx = (double)u1[q].x * c1 + (double)u2[q].x * c2;
-                    const double udy = (double)u1[q].y * c1 + (double)u2[q].y * c2;
-                    es = (usx * usx + usy * usy) * 0.5;
-                    ed = (udx * udx + udy * udy) * 0.5;
+                    if (2 * r == n1 && kx > 0) {
+                        // y-Nyquist row: this entry stands for the reference's modes
+                        // (kx, +N/2) and (-kx, +N/2), whose kx*ky cross terms cancel
+                        const double a1 = (double)u1[q].x * (double)u1[q].x + (double)u1[q].y * (double)u1[q].y;
+                        const double a2 = (double)u2[q].x * (double)u2[q].x + (double)u2[q].y * (double)u2[q].y;
+                        es = (a1 * (c2 * c2) + a2 * (c1 * c1)) * 0.5;
+                        ed = (a1 * (c1 * c1) + a2 * (c2 * c2)) * 0.5;
+                    } else {
+                        const double usx = (double)u1[q].x * c2 - (double)u2[q].x * c1;
+                        const double usy = (double)u1[q].y * c2 - (double)u2[q].y * c1;
+                        const double udx = (double)u1[q].x * c1 + (double)u2[q].x * c2;
+                        const double udy = (double)u1[q].y * c1 + (double)u2[q].y * c2;
+                        es = (usx * usx + usy * usy) * 0.5;
+                        ed = (udx * udx + udy * udy) * 0.5;
+                    }
                     tot[3] += w * (ed * ik);
                 }
                 tot[0] += w * es;

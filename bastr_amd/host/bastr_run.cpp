@@ -13,8 +13,9 @@
 // C-ABI of include/bastr.h; this program only reads, writes and schedules.
 //
 // Extra command (the role of Bastrpp's hitgen2d/3d, Bastrpp.F90:601-808, with
-// a reproducible seed):  bastr_run hitgen <k0> <seed> [urms]  writes
-// datin/flowini{2,3}d.h5 for the grid of datin/input.dat.
+// a reproducible seed):  bastr_run hitgen [seed [k0 [energy]]]  writes
+// datin/flowini{2,3}d.h5 for the grid of datin/input.dat; k0 and energy
+// default to input.dat's forcek and targetenergy, as in Bastrpp.
 #include <dlfcn.h>
 #include <hdf5.h>  // types and prototypes only: the library is opened with dlopen below
 #include <sys/stat.h>
@@ -354,8 +355,10 @@ void writeflfed(Run& r, int filenumb, int64_t nstep, double time) {
     std::printf(" >> %s ... done\n", fn.c_str());
 }
 
-// hitgen: seeded random-phase dilatational field (Bastrpp.F90:601-808 with the
-// spectrum law E(k) = k^4 exp(-2 (k/k0)^2) and a SplitMix64 noise source).
+// hitgen: seeded random-phase dilatational field, create_initial_field_2d/3d
+// (Bastrpp.F90:601-808) step by step with a SplitMix64 noise source in place of
+// the clock-seeded random_number, on the library's fftwlink-compatible
+// transforms.
 uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -363,7 +366,7 @@ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-int hitgen(const Input& in, double k0, uint64_t seed, double urms) {
+int hitgen(const Input& in, double k0, uint64_t seed, double target) {
     const int nd = in.ka == 0 ? 2 : 3;
     const int nz = nd == 3 ? in.ka : 1;
     const size_t n = (size_t)in.ia * in.ja * nz;
@@ -391,7 +394,9 @@ int hitgen(const Input& in, double k0, uint64_t seed, double urms) {
                     const double k3 = nd == 3 ? (k <= in.ka / 2 ? k : k - in.ka) : 0.0;
                     const double kk = std::sqrt(k1 * k1 + k2 * k2 + k3 * k3);
                     if (kk == 0) { z[m] = 0; continue; }
-                    const double E = std::pow(kk, 4) * std::exp(-2.0 * (kk / k0) * (kk / k0));
+                    const double q = kk / k0, kk2 = kk * kk;
+                    const double E = kk >= k0 ? kk2 * kk2 * std::exp(-2.0 * (q * q))
+                                              : k0 * k0 * k0 * k0 * k0 * k0 / kk2 * std::exp(-2.0);
                     const double A = nd == 2 ? std::sqrt(E / (2 * M_PI * kk)) : std::sqrt(E / (4 * M_PI * kk * kk));
                     const double kc = c == 0 ? k1 : (c == 1 ? k2 : k3);
                     z[m] = A * kc / kk * std::complex<double>(std::sin(theta[m]), std::cos(theta[m]));
@@ -403,7 +408,7 @@ int hitgen(const Input& in, double k0, uint64_t seed, double urms) {
     for (size_t m = 0; m < n; ++m)
         for (int c = 0; c < nd; ++c) energy += u[c][m] * u[c][m];
     energy /= (double)n;
-    const double f = urms / std::sqrt(energy);
+    const double f = std::sqrt(target / energy);
     const std::string fn = nd == 2 ? "datin/flowini2d.h5" : "datin/flowini3d.h5";
     hid_t file = H5.fcreate(fn.c_str(), kAccTrunc, H5P_DEFAULT, H5P_DEFAULT);
     if (file < 0) stop_msg("cannot create " + fn, 2);
@@ -435,10 +440,11 @@ int main(int argc, char** argv) {
     Input in = readinput("datin/input.dat");
     std::printf(" ia: %d ja: %d ka: %d\n", in.ia, in.ja, in.ka);
     if (argc >= 2 && std::string(argv[1]) == "hitgen") {
-        const double k0 = argc >= 3 ? std::atof(argv[2]) : 4.0;
-        const uint64_t seed = argc >= 4 ? std::strtoull(argv[3], nullptr, 10) : 1;
-        const double urms = argc >= 5 ? std::atof(argv[4]) : 1.0;
-        return hitgen(in, k0, seed, urms);
+        const uint64_t seed = argc >= 3 ? std::strtoull(argv[2], nullptr, 10) : 1;
+        const double k0 = argc >= 4 ? std::atof(argv[3]) : (double)in.forcek;
+        const double energy = argc >= 5 ? std::atof(argv[4]) : in.target_energy;
+        if (!(k0 > 0) || !(energy > 0)) stop_msg("hitgen: k0 and energy must be > 0", 2);
+        return hitgen(in, k0, seed, energy);
     }
     if (in.ka == 0 && in.ja == 0) stop_msg("1D case is not implemented yet!", 1);
     if (in.timemethod != 1 && in.timemethod != 2) stop_msg("Undefined time method", 0);

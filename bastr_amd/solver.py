@@ -82,6 +82,11 @@ class Solver:
         ptrs = [_dptr(f) for f in fs] + [None] * (3 - self.nd)
         self._check(self._fn("set_fields")(self._h, *ptrs))
 
+    def init_random_phase(self, k0=4.0, seed=1, energy=1.0):
+        """Seeded random-phase field of Bastrpp's hitgen (bastr_init_random_phase):
+        spectrum peak k0 (the reference's forcek), <sum u_i^2> = energy."""
+        self._check(self._fn("init_random_phase")(self._h, float(k0), int(seed), float(energy)))
+
     def get_fields(self):
         fs = [np.empty(self.shape, dtype=np.float64, order="F") for _ in range(self.nd)]
         ptrs = [_dptr(f) for f in fs] + [None] * (3 - self.nd)

@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the Bastr pseudo-spectral time step on one MI355X.
 
-Workload (default): 2-D viscous Burgers on a 2048^2 periodic grid, FP64,
-Re = 5000, dt = 1e-4, Kutta RK3, band forcing at forcek = 5 with target
-energy 0.075, dilatational projection on, spectra to kmax = 200; initial
-field: seeded random-phase spectrum E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1.
-A "step" is one iteration of the reference main loop (solution.F90:54-97):
-spectra + velocity-gradient diagnostics, RK3 step, forcing.  The field stays
-resident in HBM for the whole timed region; bastr_step is called in chunks
-of 200 iterations (the spectra cadence of the configuration) and returns the
-per-iteration diagnostics to the host, as a log-writing driver would.
+Workload (default, --config c2): 2-D viscous Burgers on a 2048^2 periodic
+grid, FP64, Re = 5000, dt = 1e-4, Kutta RK3, band forcing at forcek = 5 with
+target energy 0.075, dilatational projection on, spectra to kmax = 200;
+initial field: Bastrpp's hitgen random-phase construction with k0 = 4
+(E(k) = k^4 exp(-2 (k/4)^2) from k = 4 up, 4^6 k^-2 e^-2 below), seed 1,
+rescaled to u_rms = 1, built on the device.  --config c3 is the 3-D 512^3 case,
+c2-fp32 / c3-fp32 the FP32 variants, c1 the 256^2 free-decay case.
+A "step" is one iteration of the reference main loop (solution.F90:54-97,
+124-169): spectra + velocity-gradient diagnostics, RK3 step, forcing.  The
+field stays resident in HBM for the whole timed region; bastr_step is called
+in chunks of 200 iterations (the spectra cadence of the configuration) and
+returns the per-iteration diagnostics to the host, as a log-writing driver
+would.
 
 Prints one JSON line.  `value` = grid points x steps / wall seconds.
-`roofline` is the dominant kernel: its compulsory HBM bytes per launch over
-its mean duration, timed with HIP events on the library's stream during the
-timed region.  `cpu_baseline` is the FP64 CPU restatement (oracle/) timed on
-this host's cores over a bounded sample of the same workload.
+`roofline` is the dominant kernel class: its compulsory HBM bytes per launch
+over its mean duration, timed with HIP events on the library's stream during
+the timed region.  `cpu_baseline` is the FP64 CPU restatement (oracle/) timed
+on this host's cores over a bounded sample of the same workload.
 """
 import argparse
 import ctypes as C
@@ -27,16 +31,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-import numpy as np  # noqa: E402
-
 METRIC = "grid-pt·steps/s at 2048² & 512³ (FP64 vs FP32), % HBM roofline, E(k) err"
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    # name: (ia, ja, precision, forcemethod, lproject, kmax)
-    "c2": (2048, 2048, 0, 2, True, 200),
-    "c2-fp32": (2048, 2048, 1, 2, True, 200),
-    "c1": (256, 256, 0, 0, False, 128),
+    # grid, precision, forcemethod, lproject, kmax, default timed steps, oracle sample steps
+    "c1": dict(grid=(256, 256), prec=0, force=0, proj=False, kmax=128, steps=2000, cpu=200),
+    "c2": dict(grid=(2048, 2048), prec=0, force=2, proj=True, kmax=200, steps=1000, cpu=10),
+    "c2-fp32": dict(grid=(2048, 2048), prec=1, force=2, proj=True, kmax=200, steps=1000, cpu=10),
+    "c3": dict(grid=(512, 512, 512), prec=0, force=2, proj=True, kmax=256, steps=100, cpu=1),
+    "c3-fp32": dict(grid=(512, 512, 512), prec=1, force=2, proj=True, kmax=256, steps=100, cpu=1),
 }
 
 
@@ -74,15 +78,17 @@ KCLASS_SYMBOL = {
 }
 
 
-def traffic_from_profile(kclass, prec, ia, ja):
+def traffic_from_profile(kclass, prec, grid):
     """HBM bytes per launch of the kernel from the committed counter summary
     (profiles/hbm_traffic_2d_<n>_<prec>.txt), or None if not profiled."""
-    T = "float" if prec == 1 else "double"
-    tag = f"2d_{ia}_{'fp32' if prec == 1 else 'fp64'}" if ia == ja else None
-    sym = KCLASS_SYMBOL.get(kclass)
-    if tag is None or sym is None:
+    if len(grid) != 2 or grid[0] != grid[1]:
         return None
-    sym = sym.format(T=T, N0=ia, N1=ja)
+    T = "float" if prec == 1 else "double"
+    tag = f"2d_{grid[0]}_{'fp32' if prec == 1 else 'fp64'}"
+    sym = KCLASS_SYMBOL.get(kclass)
+    if sym is None:
+        return None
+    sym = sym.format(T=T, N0=grid[0], N1=grid[1])
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
     if not os.path.exists(path):
         return None
@@ -107,13 +113,15 @@ def run_steps(s, n, chunk):
 
 
 def cpu_baseline(params, fields, sample_steps):
+    """The oracle on this host's cores: one untimed iteration (it carries the
+    entry dealias), then `sample_steps` timed iterations of the same workload."""
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = min(threads, 16)
     oracle.set_threads(threads)
     o = oracle.solver(params)
     o.set_fields(*fields)
-    o.step(1)  # prologue + one iteration (untimed)
+    o.step(1)
     t0 = time.perf_counter()
     o.step(sample_steps)
     dt = time.perf_counter() - t0
@@ -124,11 +132,11 @@ def cpu_baseline(params, fields, sample_steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--chunk", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,70 +145,80 @@ def main():
         sys.exit(2)
 
     import bastr_amd
-    from bastr_amd.fields import hitgen
     from bastr_amd.solver import make_params
 
-    ia, ja, prec, force, proj, kmax = CONFIGS[args.config]
-    params = make_params(ia, ja, 0, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
-                         target_energy=0.075, lproject=proj, kmax=kmax, precision=prec)
-    fields = hitgen((ia, ja), k0=4.0, seed=1, urms=1.0)
+    cfg = CONFIGS[args.config]
+    grid, prec, force, proj, kmax = cfg["grid"], cfg["prec"], cfg["force"], cfg["proj"], cfg["kmax"]
+    steps = args.steps if args.steps is not None else cfg["steps"]
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else cfg["cpu"]
+    ia, ja = grid[0], grid[1]
+    ka = grid[2] if len(grid) == 3 else 0
+    dims = "x".join(map(str, grid))
+
+    def params(precision):
+        return make_params(ia, ja, ka, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
+                           target_energy=0.075, lproject=proj, kmax=kmax, precision=precision)
+
     lib = bastr_amd.load()
     prof_api(lib)
-    s = bastr_amd.solver(params)
-    s.set_fields(*fields)
+    s = bastr_amd.solver(params(prec))
+    s.init_random_phase(k0=4.0, seed=1, energy=1.0)
 
     # warmup, with every kernel class timed to find the dominant one
     lib.bastr_prof_enable(s._h, 0x3FF)
     run_steps(s, max(args.warmup, 1), args.chunk)
     warm = read_prof(lib, s._h)
-    dom = max(warm.items(), key=lambda kv: kv[1][2])
-    dom_name, (dom_cls, _, _, _) = dom
+    dom_name, (dom_cls, _, _, _) = max(warm.items(), key=lambda kv: kv[1][2])
     for name, (c, n, ms, by) in sorted(warm.items(), key=lambda kv: -kv[1][2]):
         if n:
             print(f"  warmup {name:12s} launches={n:6d} mean={ms / n * 1e3:9.2f} us "
-                  f"bytes/launch={by / 1e6:8.2f} MB  {by / (ms / n * 1e-3) / 1e9:8.1f} GB/s", file=sys.stderr)
+                  f"bytes/launch={by / 1e6:8.2f} MB  {by / (ms / n * 1e-3) / 1e9:8.1f} GB/s",
+                  file=sys.stderr, flush=True)
 
-    # timed region: only the dominant kernel is bracketed by events
+    # timed region: only the dominant kernel class is bracketed by events
     lib.bastr_prof_enable(s._h, 1 << dom_cls)
     t0 = time.perf_counter()
-    run_steps(s, args.steps, args.chunk)
+    run_steps(s, steps, args.chunk)
     wall = time.perf_counter() - t0
+    print(f"  timed {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
     prof = read_prof(lib, s._h)
     _, n_l, ms_l, by_l = prof[dom_name]
     mean_s = ms_l / n_l * 1e-3
     achieved = by_l / mean_s / 1e9
+    s.close()
 
-    npts = ia * ja
-    value = npts * args.steps / wall
+    value = ia * ja * max(ka, 1) * steps / wall
     rd = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, ia, ja),
-          "kernel": dom_name,
-          "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)}
+          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, grid),
+          "kernel": dom_name, "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)}
     cpu = None
     if not args.no_cpu:
-        p64 = make_params(ia, ja, 0, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
-                          target_energy=0.075, lproject=proj, kmax=kmax, precision=0)
-        v, thr, dt = cpu_baseline(p64, fields, args.cpu_steps)
+        # the same initial field, from the library's FP64 generator, on the CPU restatement
+        g0 = bastr_amd.solver(params(0))
+        g0.init_random_phase(k0=4.0, seed=1, energy=1.0)
+        fields = g0.get_fields()
+        g0.close()
+        v, thr, dt = cpu_baseline(params(0), fields, cpu_steps)
         cpu = {"value": v, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
-               "sample": f"{args.cpu_steps} loop iterations of the same {ia}x{ja} FP64 workload "
-                         f"({dt:.1f} s), FP64 C restatement with its own radix-2 FFT, OpenMP"}
+               "sample": f"{cpu_steps} loop iteration(s) of the same {dims} FP64 workload ({dt:.1f} s), "
+                         f"FP64 C restatement with its own radix-2 FFT, OpenMP"}
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "grid-pt·steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == 1 else "f64",
-        "data": "synthetic: seeded random-phase field, E(k)~k^4 exp(-2(k/4)^2), u_rms=1",
-        "config": {"workload": f"2-D viscous Burgers {ia}x{ja}, Re=5000, dt=1e-4, RK3, "
+        "data": "synthetic: seeded random-phase field (hitgen law, k0=4, seed 1), E(k)~k^4 exp(-2(k/4)^2), u_rms=1",
+        "config": {"workload": f"{len(grid)}-D viscous Burgers {dims}, Re=5000, dt=1e-4, RK3, "
                                f"{'band forcing k=5 E_t=0.075' if force == 2 else 'free decay'}, "
                                f"lproject={'t' if proj else 'f'}, spectra kmax={kmax}",
-                   "grid": [ia, ja], "precision": "fp32" if prec == 1 else "fp64"},
+                   "grid": list(grid), "precision": "fp32" if prec == 1 else "fp64"},
         "roofline": rd,
         "cpu_baseline": cpu,
     }

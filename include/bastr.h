@@ -88,7 +88,10 @@ const char* bastr_last_error(void);
 /*
  * Create a solver: replaces mpisizedis_fftw + allocation + prepareplan_fftw_plan
  * + quantity_prepare (fftwlink.F90:34,370,284; solution.F90:1288).
- * Supported grids: ia, ja, ka powers of two in [16, 4096]; 3-D needs ia==ja==ka.
+ * Supported grids: 2-D ia, ja powers of two in [16, 4096]; 3-D ia == ja == ka,
+ * a power of two in [16, 512].  timemethod 1 or 2; forcemethod 0, 1, 2 or 5
+ * (methods 3 and 4 draw unseeded random forcing in the reference and are
+ * rejected with BASTR_EINVAL).
  */
 int bastr_create(const bastr_params* p, bastr_handle** out);
 int bastr_destroy(bastr_handle* h);
@@ -103,6 +106,24 @@ int bastr_allkmax(const bastr_handle* h);
  * (solution.F90:41-52 / 105-122), as every reference run does.
  */
 int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const double* u3);
+/*
+ * Seeded random-phase initial field, built on the device: the construction of
+ * Bastrpp's hitgen2d/hitgen3d (create_initial_field_2d/3d, Bastrpp.F90:601-808)
+ * with k0 in the role of forcek, `energy` in the role of target_energy and a
+ * reproducible noise source in place of the reference's clock-seeded
+ * random_number (Bastrpp.F90:626-637):
+ *   noise(m) = (splitmix64(seed * 0x100000001B3 + m) >> 11) * 2^-53, m the
+ *   Fortran-order linear index; theta(k) = atan2 of fft(noise)(k);
+ *   u_hat_i(k) = A(|k|) k_i/|k| (sin theta + i cos theta), u_hat(0) = 0,
+ *   A = sqrt(E/(2 pi k)) (2-D) or sqrt(E/(4 pi k^2)) (3-D),
+ *   E(k) = k^4 exp(-2 (k/k0)^2) for k >= k0, k0^6 k^-2 e^-2 below;
+ *   u = real(ifft(u_hat)) (unnormalised), rescaled to <sum_i u_i^2> = energy
+ *   (energy = 1 is u_rms = 1).
+ * The field is then loaded as by bastr_set_fields (entry dealias pending),
+ * with nstep = 0, time = 0.  energy and k0 must be > 0.
+ */
+int bastr_init_random_phase(bastr_handle* h, double k0, uint64_t seed, double energy);
+
 /* Physical velocity field (Fortran order) at the current nstep. */
 int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3);
 

@@ -828,3 +828,66 @@ static int fft_common(bo_state* s, double* inout, int dir) {
 }
 int bo_fft_forward(bo_state* s, double* inout) { return fft_common(s, inout, -1); }
 int bo_fft_backward(bo_state* s, double* inout) { return fft_common(s, inout, +1); }
+
+/* ------------------------------------------------------------------------- */
+/* seeded random-phase initial field                                          */
+/* ------------------------------------------------------------------------- */
+
+/* SplitMix64 (Steele, Lea, Flood 2014), the noise source that stands in for
+ * the reference's clock-seeded random_number (Bastrpp.F90:626-637). */
+static uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+/* create_initial_field_2d / _3d (Bastrpp.F90:601-696, 698-808), statement by
+ * statement, with k0 in the role of forcek and `energy` in the role of
+ * target_energy.  Loads the field like bo_set_fields. */
+int bo_init_random_phase(bo_state* s, double k0, uint64_t seed, double energy) {
+    const double PI = 3.14159265358979323846;
+    const size_t n = s->n;
+    if (!(k0 > 0) || !(energy > 0)) return fail(BASTR_EINVAL, "bo_init_random_phase: k0 and energy must be > 0");
+    /* random_angle = random_number(); random_complex = cmplx(random_angle, 0); fft */
+    cplx* rc = s->tA[0];
+    for (size_t m = 0; m < n; ++m)
+        rc[m] = (double)(splitmix64(seed * 0x100000001B3ull + m) >> 11) * 0x1.0p-53;
+    fftf(s, rc);
+    for (size_t m = 0; m < n; ++m) {
+        const double kx = s->kw[0][m], ky = s->kw[1][m], kz = s->nd == 3 ? s->kw[2][m] : 0.0;
+        const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+        if (kx == 0 && ky == 0 && kz == 0) {
+            for (int c = 0; c < s->nd; ++c) s->spe[c][m] = 0.0;
+            continue;
+        }
+        const double q = kk / k0, kk2 = kk * kk;
+        double var1;
+        if (kk >= k0) var1 = kk2 * kk2 * exp(-2.0 * (q * q));
+        else var1 = k0 * k0 * k0 * k0 * k0 * k0 / kk2 * exp(-2.0);
+        const double var2 = s->nd == 2 ? sqrt(var1 / 2.0 / PI / kk) : sqrt(var1 / 4.0 / PI / (kk * kk));
+        const double th = atan2(cimag(rc[m]), creal(rc[m]));
+        const cplx ph = sin(th) + I * cos(th);
+        const double kc[3] = {kx, ky, kz};
+        for (int c = 0; c < s->nd; ++c) s->spe[c][m] = var2 * kc[c] / kk * ph;
+    }
+    double e = 0.0;
+    for (int c = 0; c < s->nd; ++c) {
+        ffti(s, s->spe[c]);
+        for (size_t m = 0; m < n; ++m) s->u[c][m] = creal(s->spe[c][m]);
+    }
+    for (size_t m = 0; m < n; ++m) {
+        double q = 0.0;
+        for (int c = 0; c < s->nd; ++c) q += s->u[c][m] * s->u[c][m];
+        e += q;
+    }
+    e /= (double)n;
+    const double factor = sqrt(energy / e);
+    for (int c = 0; c < s->nd; ++c)
+        for (size_t m = 0; m < n; ++m) s->u[c][m] *= factor;
+    s->prologue_pending = 1;
+    s->have_fields = 1;
+    s->nstep = 0;
+    s->time = 0.0;
+    return BASTR_OK;
+}

@@ -12,6 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from bastr_amd.fields import hitgen
 from bastr_amd.solver import Solver, make_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -103,9 +104,12 @@ def test_driver_run_dumps_and_restart(tmp_path):
     print("case driver 64x64", flush=True)
     n, maxstep = 64, 30
     write_input(tmp_path, n, n, 0, maxstep, "1.d-4", True, 10, True, 10, 128, 1, 2, 5, "0.075d0", True, 1)
-    assert run(tmp_path, "hitgen", "4", "7").returncode == 0
+    assert run(tmp_path, "hitgen", "7", "4", "1.0").returncode == 0
     u0 = [h5read(tmp_path / "datin" / "flowini2d.h5", v, (n, n)) for v in ("u1", "u2")]
     assert abs(np.mean(u0[0] ** 2 + u0[1] ** 2) - 1.0) < 1e-12  # u_rms = 1
+    # the same construction as the numpy generator and the device generator
+    ref = hitgen((n, n), k0=4, seed=7, energy=1.0)
+    assert max(np.abs(a - b).max() for a, b in zip(u0, ref)) < 1e-12 * max(np.abs(b).max() for b in ref)
     r = run(tmp_path)
     assert r.returncode == 0, r.stdout
     logs = tmp_path / "log"

@@ -204,3 +204,53 @@ def test_nan_stop_matches_reference_semantics(hip, orc):
     assert [d.nan_flag for d in dg[:2]] == [0, 1] == [d.nan_flag for d in do[:2]]
     assert np.isinf(dg[0].force_factor) and np.isinf(do[0].force_factor)
     assert all(np.isnan(x).all() for x in g.get_fields())
+
+
+@pytest.mark.parametrize("shape,tm,force", [((128, 64), 1, 2), ((256, 64), 1, 5), ((128, 64), 2, 0)])
+def test_nyquist_row_parity(hip, orc, shape, tm, force):
+    """Grids with ja < 2 ia / 3 keep the y-Nyquist row through the dealias
+    (cutoff floor(ia/3), tool.F90:149-179).  The reference works on the full
+    c2c spectrum with k2 = +ja/2 on that row and takes real parts, which
+    cancels the kx*ky cross terms of the projection (tool.F90:219-243) and of
+    the Es/Ed split (solution.F90:957-972) there.  A broadband field puts
+    energy on that row."""
+    print(f"case nyquist_row {shape} tm={tm} fm={force}", flush=True)
+    p = make_params(*shape, timemethod=tm, forcemethod=force, lproject=True, kmax=128,
+                    target_energy=0.9 if force == 5 else 0.075)
+    u = mixed_field(shape, seed=31, k0=20)
+    g, o, [((rg, dg, sg), (ro, do, so))] = run_pair(hip, orc, p, u, 3)
+    assert rg == ro == 0
+    err = rel_l2(g.get_fields(), o.get_fields())
+    print("rel-L2", err)
+    assert err < 1e-11
+    check_diags(dg, do)
+    # every shell, the ones fed by the Nyquist row included
+    assert np.abs(sg - so).max() / np.abs(so[:, 1:, :]).max() < 1e-11
+
+
+@pytest.mark.parametrize("shape,k0,seed,energy", [((64, 64), 4.0, 1, 1.0), ((128, 64), 6.0, 9, 0.075),
+                                                   ((256, 256), 4.0, 1, 1.0)])
+def test_device_random_phase_matches_oracle(hip, orc, shape, k0, seed, energy):
+    """bastr_init_random_phase (Bastrpp hitgen construction on the device) equals
+    the oracle's restatement with the same seed, after the entry dealias both go
+    through, and a step from it matches too."""
+    print(f"case init_rp {shape}", flush=True)
+    p = make_params(*shape, forcemethod=2, lproject=True)
+    g = Solver(hip, "bastr", p)
+    g.init_random_phase(k0=k0, seed=seed, energy=energy)
+    o = orc.solver(p)
+    o.init_random_phase(k0=k0, seed=seed, energy=energy)
+    assert rel_l2(g.get_fields(), o.get_fields()) < 1e-12
+    assert g.get_time() == (0, 0.0)
+    g.step(3)
+    o.step(3)
+    assert rel_l2(g.get_fields(), o.get_fields()) < 1e-11
+
+
+def test_device_random_phase_fp32(hip, orc):
+    print("case init_rp fp32 128", flush=True)
+    g = Solver(hip, "bastr", make_params(128, 128, precision=1))
+    g.init_random_phase(k0=4.0, seed=3, energy=1.0)
+    o = orc.solver(make_params(128, 128))
+    o.init_random_phase(k0=4.0, seed=3, energy=1.0)
+    assert rel_l2(g.get_fields(), o.get_fields()) < 1e-5

@@ -41,6 +41,8 @@ def test_fft3d_compat_matches_numpy(hip, n):
     (32, 2, 0, False, 3),
     (32, 1, 1, False, 4),
     (32, 1, 5, True, 4),
+    (128, 1, 2, True, 2),
+    (256, 1, 2, True, 1),
 ])
 def test_step3d_parity(hip, orc, n, tm, force, proj, nsteps):
     print(f"case step3d {n} tm={tm} fm={force} proj={proj}", flush=True)
@@ -99,3 +101,14 @@ def test_nan_stop_3d(hip, orc):
     ro, do, _ = o.step(4)
     assert rg == ro == abi.BASTR_STOP_NAN
     assert g.get_time() == o.get_time() == (1, 1e-4)
+
+
+def test_device_random_phase_3d(hip, orc):
+    print("case init_rp 3d 32", flush=True)
+    n = 32
+    p = make_params(n, n, ka=n)
+    g = Solver(hip, "bastr", p)
+    g.init_random_phase(k0=4.0, seed=5, energy=1.0)
+    o = orc.solver(p)
+    o.init_random_phase(k0=4.0, seed=5, energy=1.0)
+    assert rel_l2(g.get_fields(), o.get_fields()) < 1e-12

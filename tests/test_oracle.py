@@ -15,6 +15,12 @@ from bastr_amd.fields import hitgen
 from bastr_amd.solver import diag_table, make_params
 
 
+def rel_l2(a, b):
+    num = sum(float(np.sum((np.asarray(x) - np.asarray(y)) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(np.asarray(y) ** 2)) for y in b)
+    return np.sqrt(num / den)
+
+
 def kgrid(shape):
     ks = []
     for n in shape:
@@ -353,7 +359,7 @@ def test_forcing_m5_against_numpy(orc, proj):
     band if lproject), E = <f.f>, energy = <u.u> (no 1/2),
     factor = sqrt((E_t - energy)/E + 1) - 1 when E_t > energy."""
     shape = (64, 64)
-    u = hitgen(shape, k0=5, seed=10, urms=0.3)
+    u = hitgen(shape, k0=5, seed=10, energy=0.09)
     base = dict(reynolds=1000.0, deltat=1e-3, forcek=5, target_energy=0.2, lproject=proj)
     s0 = orc.solver(make_params(*shape, forcemethod=0, **base))
     s5 = orc.solver(make_params(*shape, forcemethod=5, **base))
@@ -379,3 +385,44 @@ def test_forcing_m5_against_numpy(orc, proj):
     w = s5.get_fields()
     for a, b, c in zip(w, v, f):
         assert np.abs(a - (b + factor * c)).max() < 1e-13
+
+
+@pytest.mark.parametrize("shape,k0,seed,energy", [((64, 48), 4.0, 1, 1.0), ((32, 32), 5.0, 7, 0.075),
+                                                   ((16, 16, 16), 4.0, 3, 1.0), ((16, 32, 8), 3.0, 2, 0.5)])
+def test_oracle_random_phase_field(orc, shape, k0, seed, energy):
+    """bo_init_random_phase (create_initial_field_2d/3d, Bastrpp.F90:601-808)
+    against properties its formula fixes, independently of both restatements:
+    <sum u^2> = energy; the field is real and curl-free (u_hat parallel to k
+    off the Nyquist lines); per-mode |u_hat|^2 is proportional to
+    E(k)/(2 pi k) (2-D) or E(k)/(4 pi k^2) (3-D) with the piecewise law; and
+    the numpy restatement (another FFT) gives the same field."""
+    nd = len(shape)
+    p = make_params(shape[0], shape[1], shape[2] if nd == 3 else 0)
+    o = orc.solver(p)
+    o.init_random_phase(k0=k0, seed=seed, energy=energy)
+    assert o.get_time() == (0, 0.0)
+    # raw (un-dealiased) field: a cutoff beyond the grid keeps every mode
+    u = hitgen(shape, k0=k0, seed=seed, energy=energy)
+    assert abs(sum(np.mean(x * x) for x in u) - energy) < 1e-13 * energy
+    uh = [np.fft.fftn(x) / x.size for x in u]
+    ks = np.meshgrid(*[np.fft.fftfreq(n, 1.0 / n) for n in shape], indexing="ij")
+    # fftfreq puts the Nyquist at -n/2; GenerateWave (tool.F90:31-37) at +n/2
+    ks = [np.where(k == -n / 2, n / 2, k) for k, n in zip(ks, shape)]
+    kk = np.sqrt(sum(k * k for k in ks))
+    off = (kk > 0)
+    for k, n in zip(ks, shape):
+        off &= np.abs(k) != n / 2
+    # curl-free: u_hat_i k_j - u_hat_j k_i = 0
+    scale = max(np.abs(x).max() for x in uh)
+    for i in range(nd):
+        for j in range(i + 1, nd):
+            assert np.abs((uh[i] * ks[j] - uh[j] * ks[i])[off]).max() < 1e-12 * scale * max(shape)
+    law = np.where(kk >= k0, kk ** 4 * np.exp(-2 * (kk / k0) ** 2), k0 ** 6 / np.where(kk > 0, kk, 1) ** 2 * np.exp(-2))
+    dens = law / (2 * np.pi * kk) if nd == 2 else law / (4 * np.pi * kk ** 2)
+    off &= dens > 1e-10 * dens[off].max()  # modes well above the FFT roundoff floor
+    ratio = sum(np.abs(x) ** 2 for x in uh)[off] / dens[off]
+    assert np.ptp(ratio) < 1e-9 * ratio.mean()
+    # the oracle's field (after its entry dealias) equals the numpy field after the same dealias
+    o2 = orc.solver(p)
+    o2.set_fields(*u)
+    assert rel_l2(o.get_fields(), o2.get_fields()) < 1e-13
