@@ -18,8 +18,13 @@ would.
 Prints one JSON line.  `value` = grid points x steps / wall seconds.
 `roofline` is the dominant kernel class: its compulsory HBM bytes per launch
 over its mean duration, timed with HIP events on the library's stream during
-the timed region.  `cpu_baseline` is the FP64 CPU restatement (oracle/) timed
-on this host's cores over a bounded sample of the same workload.
+the timed region.  `step_roofline` prices the whole step at the reference
+algorithm's transform count (one compulsory real round trip, 2*s*N^d bytes,
+per whole-field transform).  `cpu_baseline` is the FP64 CPU restatement
+(oracle/) timed on this host's cores over a bounded sample of the same
+workload.  `accuracy` gives the E(k) and field error of the FP64 device path
+against that oracle sample and, for FP32 configurations, of the FP32 path
+against the FP64 device path.
 """
 import argparse
 import ctypes as C
@@ -113,20 +118,54 @@ def run_steps(s, n, chunk):
 
 
 def cpu_baseline(params, fields, sample_steps):
-    """The oracle on this host's cores: one untimed iteration (it carries the
-    entry dealias), then `sample_steps` timed iterations of the same workload."""
+    """The oracle on this host's cores, timed over `sample_steps` iterations of
+    the same workload (its entry dealias runs first, untimed).  Returns the
+    rate, thread count, seconds, and the oracle's last-iteration spectrum and
+    final fields for the accuracy check."""
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = min(threads, 16)
     oracle.set_threads(threads)
     o = oracle.solver(params)
     o.set_fields(*fields)
-    o.step(1)
+    o.get_fields()  # runs the entry dealias
     t0 = time.perf_counter()
-    o.step(sample_steps)
+    _, _, spec = o.step(sample_steps)
     dt = time.perf_counter() - t0
     npts = params.ia * params.ja * max(params.ka, 1)
-    return npts * sample_steps / dt, threads, dt
+    out = o.get_fields()
+    o.close()
+    return npts * sample_steps / dt, threads, dt, spec[-1], out
+
+
+def ek_err(test, ref):
+    """E(k) = Es(k) + Ed(k) over shells 1..allkmax, from one iteration's
+    (kn, Es, Ed): ||E_test - E_ref||_2 / ||E_ref||_2 over the shells, and the
+    largest per-shell relative error over shells holding at least 1e-12 of the
+    peak."""
+    import numpy as np
+    a, b = test[1] + test[2], ref[1] + ref[2]
+    keep = np.abs(b) >= 1e-12 * np.abs(b).max()
+    return (float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b))),
+            float((np.abs(a - b)[keep] / np.abs(b)[keep]).max()))
+
+
+def reference_transforms(nd, timemethod, forcemethod):
+    """Whole-field transforms the reference performs per loop iteration
+    (solution.F90): top-of-loop FFT, velgrad, the RHS evaluations, the stage
+    FFTs, and forcing's transform pair."""
+    top, vg, rhs, stage_ffts, force = (2, 7, 12, 4, 4) if nd == 2 else (3, 13, 21, 6, 6)
+    if timemethod == 2:
+        evals, stage_ffts = 6, stage_ffts // 2 * 5
+    else:
+        evals = 3
+    return top + vg + evals * rhs + stage_ffts + (force if forcemethod in (2, 5) else 0)
+
+
+def rel_l2(a, b):
+    import numpy as np
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    return float(np.sqrt(num / sum(float(np.sum(y * y)) for y in b)))
 
 
 def main():
@@ -187,21 +226,51 @@ def main():
     achieved = by_l / mean_s / 1e9
     s.close()
 
-    value = ia * ja * max(ka, 1) * steps / wall
+    npts = ia * ja * max(ka, 1)
+    value = npts * steps / wall
+    # the whole step against the reference algorithm's transform count, each
+    # transform one compulsory real round trip of 2 * s * N^d bytes
+    T = reference_transforms(len(grid), 1, force)
+    b_step = T * 2 * (4 if prec == 1 else 8) * npts
+    step_rd = {"reference_transforms": T, "bytes_per_step": b_step,
+               "achieved": round(b_step * steps / wall / 1e9, 1), "unit": "GB/s",
+               "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4)}
     rd = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, grid),
           "kernel": dom_name, "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)}
+    # accuracy of E(k): an FP32 run against the FP64 device path over the same
+    # iterations (the FP64 path itself is checked against the oracle below)
+    accuracy = {}
+    if prec == 1:
+        nchk = min(steps, 200 if ka == 0 else 20)
+        res = []
+        for pr in (1, 0):
+            g = bastr_amd.solver(params(pr))
+            g.init_random_phase(k0=4.0, seed=1, energy=1.0)
+            _, _, sp = g.step(nchk)
+            res.append((sp[-1], g.get_fields()))
+            g.close()
+        e_abs, e_shell = ek_err(res[0][0], res[1][0])
+        accuracy["fp32_vs_fp64_device"] = {"iterations": nchk, "ek_rel_l2": e_abs,
+                                           "ek_err_max_per_shell": e_shell,
+                                           "fields_rel_l2": rel_l2(res[0][1], res[1][1])}
+        print(f"  fp32 vs fp64: {accuracy['fp32_vs_fp64_device']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         # the same initial field, from the library's FP64 generator, on the CPU restatement
         g0 = bastr_amd.solver(params(0))
         g0.init_random_phase(k0=4.0, seed=1, energy=1.0)
         fields = g0.get_fields()
+        _, _, sp0 = g0.step(cpu_steps)
+        dev = (sp0[-1], g0.get_fields())
         g0.close()
-        v, thr, dt = cpu_baseline(params(0), fields, cpu_steps)
+        v, thr, dt, osp, ofields = cpu_baseline(params(0), fields, cpu_steps)
         cpu = {"value": v, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
                "sample": f"{cpu_steps} loop iteration(s) of the same {dims} FP64 workload ({dt:.1f} s), "
                          f"FP64 C restatement with its own radix-2 FFT, OpenMP"}
+        e_abs, e_shell = ek_err(dev[0], osp)
+        accuracy["fp64_device_vs_oracle"] = {"iterations": cpu_steps, "ek_rel_l2": e_abs,
+                                             "ek_err_max_per_shell": e_shell, "fields_rel_l2": rel_l2(dev[1], ofields)}
     out = {
         "metric": METRIC,
         "value": value,
@@ -221,6 +290,8 @@ def main():
                    "grid": list(grid), "precision": "fp32" if prec == 1 else "fp64"},
         "roofline": rd,
         "cpu_baseline": cpu,
+        "step_roofline": step_rd,
+        "accuracy": accuracy,
     }
     print(json.dumps(out))
 
