@@ -207,6 +207,90 @@ Geo<T> geo(const bastr_handle* h) {
 template <typename T>
 cx<T>* C(void* p) { return (cx<T>*)p; }
 
+k3d::ColJob job1(int in, int mult, int out) {
+    k3d::ColJob j{};
+    j.nterm = 1;
+    j.in[0] = in;
+    j.mult[0] = mult;
+    j.out = out;
+    return j;
+}
+
+// 2-D y-inverse of the half spectra src[0..1] into W:
+//   mode 0: W0 = u1, W1 = u2
+//   mode 1: + W2 = i k2 u1, W3 = i k2 u2                 (compute_ut2D :825-830)
+//   mode 2: + W4 = -k2^2 u1, W5 = -k2^2 u2, W6 = -i k2^3 u2   (velgrad_calculate2D)
+template <typename T>
+int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    c.in[0] = C<T>((void*)src[0]);
+    c.in[1] = C<T>((void*)src[1]);
+    for (int q = 0; q < 7; ++q) c.out[q] = C<T>(h->W[q]);
+    int nj = 0;
+    c.job[nj++] = job1(0, k3d::M_ONE, 0);
+    c.job[nj++] = job1(1, k3d::M_ONE, 1);
+    if (mode >= 1) {
+        c.job[nj++] = job1(0, k3d::M_IK, 2);
+        c.job[nj++] = job1(1, k3d::M_IK, 3);
+    }
+    if (mode == 2) {
+        c.job[nj++] = job1(0, k3d::M_MK2, 4);
+        c.job[nj++] = job1(1, k3d::M_MK2, 5);
+        c.job[nj++] = job1(1, k3d::M_MIK3, 6);
+    }
+    c.njob = nj;
+    c.pitch = h->hp;
+    c.ncols = h->hp;
+    c.plane_stride = 0;
+    c.nplanes = 1;
+    c.scale = 1.0;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(h->n1, true, c, h->stream));
+    return 0;
+}
+
+// 2-D y-forward of nf half-spectrum fields src -> dst, times `scale`
+template <typename T>
+int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst, int nf, double scale, int pitch,
+              const void* tw) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < nf; ++q) {
+        c.in[q] = C<T>(src[q]);
+        c.out[q] = C<T>(dst[q]);
+        c.job[q] = job1(q, k3d::M_ONE, q);
+    }
+    c.njob = nf;
+    c.pitch = pitch;
+    c.ncols = pitch;
+    c.plane_stride = 0;
+    c.nplanes = 1;
+    c.scale = scale;
+    c.tw = (const cx<T>*)tw;
+    LAUNCHCHK(k3d::launch_colx<T>(h->n1, false, c, h->stream));
+    return 0;
+}
+
+// spectral update (dealias/stage combination) from the y-forward outputs in W[0..1]
+template <typename T>
+int run_update2(bastr_handle* h, const Gate& g, int mode, bool first) {
+    ColFwdArgs<T> c{};
+    c.g = geo<T>(h);
+    c.gate = g;
+    c.nx[0] = C<T>(h->W[0]);
+    c.nx[1] = C<T>(h->W[1]);
+    for (int q = 0; q < 2; ++q) {
+        c.U[q] = C<T>(h->U[q]);
+        c.UB[q] = C<T>(h->UB[q]);
+        c.KA[q] = C<T>(h->KA[q]);
+    }
+    c.pitch = h->hp;
+    c.cn_first = first ? 1 : 0;
+    LAUNCHCHK(launch_update<T>(mode, c, h->stream));
+    return 0;
+}
+
 // u_hat = dealias(fft2d(u)) from the real fields in ureal (mainloop2D :41-52)
 template <typename T>
 int run_prologue(bastr_handle* h) {
@@ -222,17 +306,11 @@ int run_prologue(bastr_handle* h) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
     HIPCHK(hipGetLastError());
     LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
-    ColFwdArgs<T> c{};
-    c.g = geo<T>(h);
-    c.gate = open;
-    c.nx[0] = C<T>(h->NX[0]);
-    c.nx[1] = C<T>(h->NX[1]);
-    c.U[0] = C<T>(h->U[0]);
-    c.U[1] = C<T>(h->U[1]);
-    c.pitch = h->hp;
-    c.tw = C<T>(h->tw1);
-    LAUNCHCHK(launch_col_fwd<T>(h->n1, CF_PROLOGUE, c, h->hp, h->stream));
-    return 0;
+    void* nx[2] = {h->NX[0], h->NX[1]};
+    void* w[2] = {h->W[0], h->W[1]};
+    int rc = run_yfwd2<T>(h, open, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1);
+    if (rc) return rc;
+    return run_update2<T>(h, open, UP_PROLOGUE, false);
 }
 
 // u = ifft2d(u_hat) into ureal
@@ -241,17 +319,9 @@ int run_to_physical(bastr_handle* h) {
     const Gate open{h->status, 0, 0};
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
     HIPCHK(hipGetLastError());
-    ColInvArgs<T> c{};
-    c.g = geo<T>(h);
-    c.gate = open;
-    c.in[0] = C<T>(h->U[0]);
-    c.in[1] = C<T>(h->U[1]);
-    c.out[0] = C<T>(h->W[0]);
-    c.out[1] = C<T>(h->W[1]);
-    c.nfield = 2;
-    c.pitch = h->hp;
-    c.tw = C<T>(h->tw1);
-    LAUNCHCHK(launch_col_inv<T>(h->n1, CI_PLAIN, c, h->hp, h->stream));
+    const void* Us[2] = {h->U[0], h->U[1]};
+    int rc = run_yinv2<T>(h, open, Us, 0);
+    if (rc) return rc;
     RowArgs<T> r{};
     r.g = geo<T>(h);
     r.gate = open;
@@ -270,17 +340,10 @@ int run_to_physical(bastr_handle* h) {
 // in the diagnostic phase (shared with velgrad).
 template <typename T>
 int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* Uin, bool first = false) {
-    ColInvArgs<T> ci{};
-    ci.g = geo<T>(h);
-    ci.gate = gstep;
-    ci.in[0] = C<T>((void*)Uin[0]);
-    ci.in[1] = C<T>((void*)Uin[1]);
-    for (int q = 0; q < 4; ++q) ci.out[q] = C<T>(h->W[q]);
-    ci.pitch = h->hp;
-    ci.tw = C<T>(h->tw1);
     if (!first) {
         ProfScope ps(h, BASTR_K_COL_INV);
-        LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE, ci, h->hp, h->stream));
+        const int rc = run_yinv2<T>(h, gstep, Uin, 1);
+        if (rc) return rc;
     }
     // (stage A's y-inverse ran in the diagnostic phase with the velgrad outputs)
     RowArgs<T> r{};
@@ -294,23 +357,14 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
         ProfScope ps(h, BASTR_K_ROW_STAGE);
         LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
     }
-    ColFwdArgs<T> cf{};
-    cf.g = geo<T>(h);
-    cf.gate = gstep;
-    cf.nx[0] = C<T>(h->NX[0]);
-    cf.nx[1] = C<T>(h->NX[1]);
-    for (int c = 0; c < 2; ++c) {
-        cf.U[c] = C<T>(h->U[c]);
-        cf.UB[c] = C<T>(h->UB[c]);
-        cf.KA[c] = C<T>(h->KA[c]);
-    }
-    cf.pitch = h->hp;
-    cf.cn_first = first ? 1 : 0;
-    cf.tw = C<T>(h->tw1);
     {
-        static const int modes[5] = {CF_A, CF_B, CF_C, CF_CN_MID, CF_CN_LAST};
+        static const int modes[5] = {UP_A, UP_B, UP_C, UP_CN_MID, UP_CN_LAST};
         ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
-        LAUNCHCHK(launch_col_fwd<T>(h->n1, modes[stage], cf, h->hp, h->stream));
+        void* nx[2] = {h->NX[0], h->NX[1]};
+        void* w[2] = {h->W[0], h->W[1]};
+        int rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1);
+        if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -334,17 +388,11 @@ int run_iteration(bastr_handle* h, int it) {
         LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
     }
     // velgrad_calculate2D: derivative fields shared with RK stage A
-    ColInvArgs<T> ci{};
-    ci.g = geo<T>(h);
-    ci.gate = gdiag;
-    ci.in[0] = C<T>(h->U[0]);
-    ci.in[1] = C<T>(h->U[1]);
-    for (int q = 0; q < 7; ++q) ci.out[q] = C<T>(h->W[q]);
-    ci.pitch = h->hp;
-    ci.tw = C<T>(h->tw1);
     {
         ProfScope ps(h, BASTR_K_COL_INV_VG);
-        LAUNCHCHK(launch_col_inv<T>(h->n1, CI_STAGE_VG, ci, h->hp, h->stream));
+        const void* Us[2] = {h->U[0], h->U[1]};
+        const int rc = run_yinv2<T>(h, gdiag, Us, 2);
+        if (rc) return rc;
     }
     VgArgs<T> va{};
     va.g = geo<T>(h);
@@ -448,15 +496,6 @@ k3d::Geo3<T> geo3(const bastr_handle* h) {
     return g;
 }
 
-k3d::ColJob job1(int in, int mult, int out) {
-    k3d::ColJob j{};
-    j.nterm = 1;
-    j.in[0] = in;
-    j.mult[0] = mult;
-    j.out = out;
-    return j;
-}
-
 // z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: Z33_c = -k3^2, Z333 = -i k3^3 of u3)
 template <typename T>
 int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/) {
@@ -547,6 +586,28 @@ int run_yfwd3(bastr_handle* h, const Gate& g) {
     return 0;
 }
 
+// 3-D z-forward of nf fields src -> dst, times 1/(ia*ja*ka) (the fft3d normalisation)
+template <typename T>
+int run_zfwd3(bastr_handle* h, const Gate& g, void* const* src, void* const* dst, int nf) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < nf; ++q) {
+        c.in[q] = C<T>(src[q]);
+        c.out[q] = C<T>(dst[q]);
+        c.job[q] = job1(q, k3d::M_ONE, q);
+    }
+    c.njob = nf;
+    c.pitch = h->hp * h->n1;
+    c.ncols = h->hp * h->n1;
+    c.plane_stride = 0;
+    c.nplanes = 1;
+    c.scale = 1.0 / ((double)h->n0 * (double)h->n1 * (double)h->n2);
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(h->n2, false, c, h->stream));
+    return 0;
+}
+
+// z-forward of NY fused with the spectral update of the state
 template <typename T>
 int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first) {
     k3d::Cf3Args<T> c{};
@@ -559,8 +620,6 @@ int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first) {
         c.KA[q] = C<T>(h->KA[q]);
     }
     c.cn_first = first ? 1 : 0;
-    c.pitch = h->hp * h->n1;
-    c.ncols = h->hp * h->n1;
     c.tw = C<T>(h->tw1);
     LAUNCHCHK(k3d::launch_cf3<T>(h->n2, mode, c, h->stream));
     return 0;
@@ -865,7 +924,7 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     const long ntask = (long)h->n1 * h->n2 * nbx;
     h->nblk_spec = (int)std::max(1L, std::min(512L, ntask / 8));
     {
-        const int L = row_block_rows(h->n0, h->prec);
+        const int L = vg_block_rows(h->n0, h->prec, nd);
         h->nblk_vg = (h->n1 * h->n2 + L - 1) / L;
     }
     h->part_bins = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * (h->allkmax + 1) * 2, &ok);
@@ -1007,15 +1066,10 @@ static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, dou
         r.hout[1] = C<T>(h->NX[1]);
         r.tw = C<T>(h->tw0);
         LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
-        ColFwdArgs<T> c{};
-        c.g = geo<T>(h);
-        c.gate = open;
-        c.nx[0] = C<T>(h->NX[0]);
-        c.out[0] = C<T>(h->U[0]);
-        c.nfield = 1;
-        c.pitch = h->hp;
-        c.tw = C<T>(h->tw1);
-        LAUNCHCHK(launch_col_fwd<T>(h->n1, CF_PLAIN, c, h->hp, h->stream));
+        void* src[1] = {h->NX[0]};
+        void* dst[1] = {h->U[0]};
+        int rc = run_yfwd2<T>(h, open, src, dst, 1, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1);
+        if (rc) return rc;
     } else {
         k3d::Row3Args<T> r{};
         r.g = geo3<T>(h);
@@ -1028,16 +1082,9 @@ static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, dou
         LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
         int rc = run_yfwd3<T>(h, open);
         if (rc) return rc;
-        k3d::Cf3Args<T> c{};
-        c.g = geo3<T>(h);
-        c.gate = open;
-        c.nx[0] = C<T>(h->NY[0]);
-        c.out[0] = C<T>(h->U[0]);
-        c.nfield = 1;
-        c.pitch = h->hp * h->n1;
-        c.ncols = h->hp * h->n1;
-        c.tw = C<T>(h->tw1);
-        LAUNCHCHK(k3d::launch_cf3<T>(h->n2, k3d::C3_PLAIN, c, h->stream));
+        void* src[1] = {h->NY[0]};
+        void* dst[1] = {h->U[0]};
+        if ((rc = run_zfwd3<T>(h, open, src, dst, 1))) return rc;
     }
     // 3. the spectrum law with the noise phases
     rp::RpArgs b{};
@@ -1215,27 +1262,25 @@ static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
         r.cout = z;
         r.tw = tw0;
         LAUNCHCHK(launch_row<double>(h->n0, R_C2C_FWD, r, h->n1, h->stream));
-        ColFwdArgs<double> c{};
-        c.g = g;
-        c.gate = open;
-        c.nx[0] = z;
-        c.out[0] = z;
-        c.nfield = 1;
-        c.pitch = h->n0;
-        c.tw = tw1;
         if (col_block_width(h->n1, BASTR_PREC_F64) > h->n0) return fail(BASTR_EINVAL, "bastr_fft: ia too small for ja");
-        LAUNCHCHK(launch_col_fwd<double>(h->n1, CF_PLAIN, c, h->n0, h->stream));
+        void* zz[1] = {z};
+        const int rc = run_yfwd2<double>(h, open, zz, zz, 1, 1.0 / ((double)h->n0 * (double)h->n1), h->n0, tw1);
+        if (rc) return rc;
     } else {
-        ColInvArgs<double> c{};
-        c.g = g;
+        k3d::ColXArgs<double> c{};
         c.gate = open;
         c.in[0] = z;
         c.out[0] = z;
-        c.nfield = 1;
+        c.job[0] = job1(0, k3d::M_ONE, 0);
+        c.njob = 1;
         c.pitch = h->n0;
+        c.ncols = h->n0;
+        c.plane_stride = 0;
+        c.nplanes = 1;
+        c.scale = 1.0;
         c.tw = tw1;
         if (col_block_width(h->n1, BASTR_PREC_F64) > h->n0) return fail(BASTR_EINVAL, "bastr_fft: ia too small for ja");
-        LAUNCHCHK(launch_col_inv<double>(h->n1, CI_PLAIN, c, h->n0, h->stream));
+        LAUNCHCHK(k3d::launch_colx<double>(h->n1, true, c, h->stream));
         RowArgs<double> r{};
         r.g = g;
         r.gate = open;

@@ -17,28 +17,45 @@
 
 namespace k2d {
 
-// Column-pass launch shape for (T, N1): CB adjacent columns per workgroup
-// (CB*sizeof(complex) >= 64 B contiguous per row), E points per thread,
-// CB*N1/E threads (<= 512 in FP64 so a thread may hold 256 VGPRs without
-// spilling), CB*N1*sizeof(T) bytes of LDS (split exchange, <= 64 KiB).
-template <typename T, int N>
+// Column-pass launch shape for (T, N): CB adjacent columns per workgroup,
+// E points per thread (EMAX from N = 256 up, else 8), P = N/E threads per
+// column, about 256 threads per workgroup.  CB*sizeof(complex) >= 32 B keeps
+// each row access to whole 32-byte sectors (the neighbouring column blocks,
+// run on the same XCD, read the rest of each line from its L2); CB <= 16 so
+// CB divides every supported row length; LDS is CB*N*sizeof(T) (+1/16
+// padding) for the split real/imaginary exchange.  Small workgroups let
+// several of them share a CU, so one block's loads overlap another's
+// transform.
+template <typename T, int N, int EMAX = 16>
 struct ColCfg {
-    static constexpr int CBMIN = 64 / (int)sizeof(cx<T>);
-    static constexpr int LDS_CB = 65536 / (N * (int)sizeof(T));  // (+1/16 padding, <= 68 KiB)
-    static constexpr int CBW = (256 / (N / 8)) > CBMIN ? (256 / (N / 8)) : CBMIN;
-    static constexpr int CB2 = CBW > 16 ? 16 : CBW;
-    static constexpr int CB = CB2 < LDS_CB ? CB2 : LDS_CB;
-    static constexpr int MAXT = sizeof(T) == 8 ? 512 : 1024;
-    static constexpr int E = (CB * (N / 8) <= MAXT) ? 8 : 16;
+    static constexpr int E = N >= 256 ? EMAX : 8;
     static constexpr int P = N / E;
+    static constexpr int CBMIN = 32 / (int)sizeof(cx<T>);
+    static constexpr int CBW = (256 / P) > CBMIN ? (256 / P) : CBMIN;
+    static constexpr int CB = CBW > 16 ? 16 : CBW;
     static constexpr int THREADS = CB * P;
-    static_assert(CB >= 1 && THREADS <= MAXT && N >= E, "column config");
+    static_assert(CB >= 1 && THREADS <= 1024 && N >= E, "column config");
+    static_assert(EMAX == 8 || EMAX == 16, "column config");
+    static_assert(CB * (N + N / 16) * (int)sizeof(T) <= 81920, "column LDS");
 };
 
 // Row-pass launch shape for (T, N0): L rows per workgroup.
 template <typename T, int N>
 struct RowCfg {
     static constexpr int E = 8;
+    static constexpr int P = N / E;
+    static constexpr int L0 = P >= 256 ? 1 : 256 / P;
+    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/16 padding)
+    static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
+    static constexpr int THREADS = L * P;
+};
+
+// Row-pass shape of the velocity-gradient kernels, which hold several
+// gradient components per point across their transforms: 8 points per thread
+// in 2-D, 4 in the 3-D kernel (three components) to stay within 256 registers.
+template <typename T, int N, int EE = 8>
+struct VgCfg {
+    static constexpr int E = EE;
     static constexpr int P = N / E;
     static constexpr int L0 = P >= 256 ? 1 : 256 / P;
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/16 padding)
@@ -69,34 +86,20 @@ struct Gate {
     }
 };
 
-enum ColInvMode { CI_PLAIN = 0, CI_STAGE = 1, CI_STAGE_VG = 2 };
-enum ColFwdMode { CF_PLAIN = 0, CF_PROLOGUE = 1, CF_A = 2, CF_B = 3, CF_C = 4, CF_CN_MID = 5, CF_CN_LAST = 6 };
+enum UpdateMode { UP_PROLOGUE = 1, UP_A = 2, UP_B = 3, UP_C = 4, UP_CN_MID = 5, UP_CN_LAST = 6 };
 enum RowMode { R_FWD_PACK = 0, R_INV_PACK = 1, R_STAGE = 2, R_C2C_FWD = 3, R_C2C_INV = 4 };
 
-template <typename T>
-struct ColInvArgs {
-    Geo<T> g;
-    Gate gate;
-    const cx<T>* in[2];
-    cx<T>* out[7];
-    int nfield;  // CI_PLAIN: number of fields (in[f] -> out[f])
-    int pitch;   // row pitch of in/out (hp, or n0 for full c2c arrays)
-    const cx<T>* tw;
-};
 
 template <typename T>
 struct ColFwdArgs {
     Geo<T> g;
     Gate gate;
-    const cx<T>* nx[2];  // row-transformed inputs
-    cx<T>* U[2];         // state (read; written by CF_C / CF_PROLOGUE)
-    cx<T>* UB[2];        // stage-B input (CF_A writes), stage-C input (CF_B writes, CF_C reads)
-    cx<T>* KA[2];        // k_A (CF_A writes), k_A + 4 k_B (CF_B writes, CF_C reads)
-    cx<T>* out[2];       // CF_PLAIN output
-    int nfield;
+    const cx<T>* nx[2];  // y-transformed, normalised tendencies (UP_PROLOGUE: the field)
+    cx<T>* U[2];         // state (read; written by UP_C / UP_PROLOGUE / UP_CN_LAST)
+    cx<T>* UB[2];        // stage-B input (UP_A writes), stage-C input (UP_B writes, UP_C reads)
+    cx<T>* KA[2];        // k_A (UP_A writes), k_A + 4 k_B (UP_B writes, UP_C reads)
     int pitch;
-    int cn_first;        // CF_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
-    const cx<T>* tw;
+    int cn_first;        // UP_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
 };
 
 template <typename T>
@@ -176,8 +179,7 @@ struct ScaleArgs {
 };
 
 // launchers (k2d.hip); n is the transform length along the pass
-template <typename T> int launch_col_inv(int n, int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s);
-template <typename T> int launch_col_fwd(int n, int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s);
+template <typename T> int launch_update(int mode, const ColFwdArgs<T>& a, hipStream_t s);
 template <typename T> int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s);
 template <typename T> int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk);
 template <typename T> int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s);
@@ -187,6 +189,6 @@ template <typename T> int launch_energy(const EnergyArgs<T>& a, int nblk, hipStr
 template <typename T> int launch_scale(const ScaleArgs<T>& a, hipStream_t s);
 int col_block_width(int n, int prec);  // CB of the column pass for (n, precision)
 int row_block_rows(int n, int prec);   // L of the row pass
-int velgrad_blocks(int n0, int n1, int prec);
+int vg_block_rows(int n, int prec, int nd);  // L of the velocity-gradient row pass
 
 }  // namespace k2d

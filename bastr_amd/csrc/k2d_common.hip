@@ -107,9 +107,18 @@ static int l_of(int n) {
 
 int col_block_width(int n, int prec) { return prec == BASTR_PREC_F32 ? cb_of<float>(n) : cb_of<double>(n); }
 int row_block_rows(int n, int prec) { return prec == BASTR_PREC_F32 ? l_of<float>(n) : l_of<double>(n); }
-int velgrad_blocks(int n0, int n1, int prec) {
-    const int L = row_block_rows(n0, prec);
-    return L > 0 ? (n1 + L - 1) / L : -1;
+template <typename T, int EE>
+static int lvg_of(int n) {
+    switch (n) {
+#define X(NN) case NN: return VgCfg<T, NN, EE>::L;
+        X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#undef X
+    }
+    return -1;
+}
+int vg_block_rows(int n, int prec, int nd) {
+    if (nd == 3) return prec == BASTR_PREC_F32 ? lvg_of<float, 4>(n) : lvg_of<double, 4>(n);
+    return prec == BASTR_PREC_F32 ? lvg_of<float, 8>(n) : lvg_of<double, 8>(n);
 }
 
 }  // namespace k2d

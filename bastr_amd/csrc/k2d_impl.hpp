@@ -1,17 +1,18 @@
 // k2d_impl.hpp -- kernel bodies of the 2-D step (included once per precision).
 //
 // Reference correspondence (src/solution.F90, src/tool.F90):
-//   col_inv_kernel<CI_STAGE>   spectral derivatives u_hat * i k_y (compute_ut2D :825-830),
-//                              y half of the inverse transforms (:832-839)
-//   row_kernel<R_STAGE>        x half of those inverse transforms, the nonlinear
-//                              term -u_j d_j u_i (:841-846), x half of fft2d (:850-851)
-//   col_fwd_kernel<CF_A/B/C>   y half of fft2d, normalisation (fftwlink.F90:334),
-//                              + nu*Lap(u) in spectral space, dealiasing (:853-854),
-//                              projection (:856-858), Kutta RK3 combination (:183-200)
-//   spectra_kernel             spectra_compute2D (:930-1016)
-//   col_inv<CI_STAGE_VG> +
-//   velgrad_kernel             velgrad_calculate2D (:1110-1186)
-//   force_band_kernel          forcing2D method 2 (:336-378)
+//   k3d::colx_kernel (inverse)  spectral derivatives u_hat * i k_y (compute_ut2D :825-830),
+//                               y half of the inverse transforms (:832-839); the
+//                               same job-table column pass serves 2-D and 3-D
+//   row_kernel<R_STAGE>         x half of those inverse transforms, the nonlinear
+//                               term -u_j d_j u_i (:841-846), x half of fft2d (:850-851)
+//   k3d::colx_kernel (forward)  y half of fft2d, normalisation (fftwlink.F90:334)
+//   update_kernel<UP_A/B/C>     + nu*Lap(u) in spectral space, dealiasing (:853-854),
+//                               projection (:856-858), Kutta RK3 combination (:183-200)
+//   spectra_kernel              spectra_compute2D (:930-1016)
+//   colx (velgrad jobs) +
+//   row_stage_vg_kernel         velgrad_calculate2D (:1110-1186) fused with stage A's x pass
+//   force_band_kernel           forcing2D methods 1, 2, 5 (:303-540)
 #pragma once
 #include <climits>
 #include "k2d.hpp"
@@ -78,175 +79,95 @@ __device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, f
 }
 
 // ---------------------------------------------------------------------------
-// y-direction inverse pass
+// spectral update after the y-forward column pass (k3d::colx_kernel, which
+// also applies the 1/(ia*ja) of fft2d): one thread per mode, rows streamed
+// contiguously.
+//   UP_PROLOGUE  u_hat = dealias(fft2d(u))                        (mainloop2D :41-47)
+//   UP_A/B/C     r = P(D(N + nu Lap s)), Kutta RK3 combination    (RK32D :183-200)
+//   UP_CN_*      u = uold + dt r, next midpoint 0.5 (u + uold)    (CN2D :215-225)
+// s is the stage input (the viscous term of compute_ut2D :841-846).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
-__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_inv_kernel(ColInvArgs<T> a) {
-    using C = ColCfg<T, N>;
-    constexpr int E = C::E, P = C::P, CB = C::CB;
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void update_kernel(ColFwdArgs<T> a) {
     if (a.gate.closed()) return;
-    __shared__ T lds[CB * lds_line(N)];
-    const int blk = xcd_block(blockIdx.x, a.pitch / CB);
-    if (blk < 0) return;
-    const int tid = threadIdx.x;
-    const int l = tid % CB, j = tid / CB;
-    const int col = blk * CB + l;
-    const LdsLine<T, true> L{lds + l, CB};
-    const size_t pitch = (size_t)a.pitch;
-    const int nf = MODE == CI_PLAIN ? a.nfield : 2;
-    for (int f = 0; f < nf; ++f) {
-        const cx<T>* __restrict__ src = a.in[f];
-        const int njob = MODE == CI_PLAIN ? 1 : (MODE == CI_STAGE ? 2 : (f == 0 ? 3 : 4));
-        for (int job = 0; job < njob; ++job) {
-            // the column is re-read per job (an L2 hit after the first) rather
-            // than kept in registers across the transforms
-            cx<T> v[E];
-#pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int m = j + k * P;
-                const cx<T> u = src[(size_t)m * pitch + col];
-                const T ky = (T)ksigned(m, N);
-                const T kyo = (m == N / 2) ? (T)0 : ky;  // odd powers vanish at the Nyquist row
-                if (job == 0) v[k] = u;
-                else if (job == 1) v[k] = cscale(cmuli(u), kyo);                 // i k2 u
-                else if (job == 2) v[k] = cscale(u, -(ky * ky));                 // -k2^2 u
-                else v[k] = cscale(cmuli(u), -(kyo * kyo * kyo));                 // -i k2^3 u
-            }
-            line_fft<T, N, E, true, true>(v, j, L, a.tw);
-            cx<T>* __restrict__ dst;
-            if (MODE == CI_PLAIN) dst = a.out[f];
-            else if (job == 0) dst = a.out[f];
-            else if (job == 1) dst = a.out[2 + f];
-            else if (job == 2) dst = a.out[4 + f];
-            else dst = a.out[6];
-#pragma unroll
-            for (int k = 0; k < E; ++k) dst[(size_t)(j + k * P) * pitch + col] = v[k];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// y-direction forward pass + spectral update
-// ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
-__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void col_fwd_kernel(ColFwdArgs<T> a) {
-    using C = ColCfg<T, N>;
-    constexpr int E = C::E, P = C::P, CB = C::CB;
-    if (a.gate.closed()) return;
-    __shared__ T lds[CB * lds_line(N)];
-    const int blk = xcd_block(blockIdx.x, a.pitch / CB);
-    if (blk < 0) return;
-    const int tid = threadIdx.x;
-    const int l = tid % CB, j = tid / CB;
-    const int col = blk * CB + l;
-    const LdsLine<T, true> L{lds + l, CB};
-    const size_t pitch = (size_t)a.pitch;
-    const T sc = (T)a.g.inv_n;
-
-    if (MODE == CF_PLAIN) {
-        for (int f = 0; f < a.nfield; ++f) {
-            cx<T> v[E];
-#pragma unroll
-            for (int k = 0; k < E; ++k) v[k] = a.nx[f][(size_t)(j + k * P) * pitch + col];
-            line_fft<T, N, E, false, true>(v, j, L, a.tw);
-#pragma unroll
-            for (int k = 0; k < E; ++k) a.out[f][(size_t)(j + k * P) * pitch + col] = cscale(v[k], sc);
-        }
+    const int kx = blockIdx.x * 256 + threadIdx.x;
+    const int m = blockIdx.y;
+    if (kx >= a.g.nh) return;
+    const int N = a.g.n1;
+    const int ky = ksigned(m, N);
+    const int kk2 = kx * kx + ky * ky;
+    const size_t idx = (size_t)m * a.pitch + kx;
+    cx<T> t0 = a.nx[0][idx], t1 = a.nx[1][idx];
+    if (MODE == UP_PROLOGUE) {
+        if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
+        a.U[0][idx] = t0;
+        a.U[1][idx] = t1;
         return;
     }
-
-    cx<T> v0[E], v1[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) v0[k] = a.nx[0][(size_t)(j + k * P) * pitch + col];
-    line_fft<T, N, E, false, true>(v0, j, L, a.tw);
-#pragma unroll
-    for (int k = 0; k < E; ++k) v1[k] = a.nx[1][(size_t)(j + k * P) * pitch + col];
-    line_fft<T, N, E, false, true>(v1, j, L, a.tw);
-
-    const int kx = col;
     const T dt = (T)a.g.dt;
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        // groups of 4 modes: keeps the compiler from hoisting every state load
-        // of the thread ahead of the arithmetic (register pressure)
-        if (k % 4 == 0) __builtin_amdgcn_sched_barrier(0);
-        const int m = j + k * P;
-        const int ky = ksigned(m, N);
-        const int kk2 = kx * kx + ky * ky;
-        const size_t idx = (size_t)m * pitch + col;
-        cx<T> t0 = cscale(v0[k], sc), t1 = cscale(v1[k], sc);
-        if (MODE == CF_PROLOGUE) {
-            // mainloop2D entry: u_hat = dealias(fft2d(u)) (solution.F90:41-47)
-            if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
-            a.U[0][idx] = t0;
-            a.U[1][idx] = t1;
-            continue;
-        }
-        const cx<T> U0 = a.U[0][idx], U1 = a.U[1][idx];
-        cx<T> s0, s1;  // this stage's input spectrum (for the viscous term)
-        cx<T> ka0, ka1;
-        if (MODE == CF_A) {
-            s0 = U0; s1 = U1;
-        } else if (MODE == CF_CN_MID || MODE == CF_CN_LAST) {
-            if (a.cn_first) { s0 = U0; s1 = U1; }                  // first sweep: u_hat
-            else { s0 = a.UB[0][idx]; s1 = a.UB[1][idx]; }         // this sweep's midpoint
-        } else if (MODE == CF_B) {
-            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
-            s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
-            s1 = cadd(U1, cscale(cscale(ka1, dt), (T)0.5));
-        } else {
-            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
-            s0 = a.UB[0][idx]; s1 = a.UB[1][idx];
-        }
-        // r_i = N_i + nu * Lap(u_i)   (compute_ut2D :841-846)
-        const T vis = (T)(-a.g.nu * (double)kk2);
-        t0 = cadd(t0, cscale(s0, vis));
-        t1 = cadd(t1, cscale(s1, vis));
-        if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
-        if (a.g.lproject) {
-            // projection2D (tool.F90:219-243)
-            if (kk2 > 0) {
-                const T kkr = sqrt((T)kk2);
-                const T c1 = (T)kx / kkr, c2 = (T)ky / kkr;
-                if (2 * m == N) {
-                    // y-Nyquist row (kept by the dealias when ja < 2 ia / 3): the reference
-                    // projects the full c2c spectrum, where the partner of (kx, +N/2) is
-                    // (-kx, +N/2), then keeps the real part of the inverse, so the two
-                    // projections' kx*ky cross terms cancel
-                    t0 = cscale(cscale(t0, c1), c1);
-                    t1 = cscale(cscale(t1, c2), c2);
-                } else {
-                    const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
-                    t0 = cscale(ud, c1);
-                    t1 = cscale(ud, c2);
-                }
+    const cx<T> U0 = a.U[0][idx], U1 = a.U[1][idx];
+    cx<T> s0, s1;  // this stage's input spectrum (for the viscous term)
+    cx<T> ka0, ka1;
+    if (MODE == UP_A) {
+        s0 = U0; s1 = U1;
+    } else if (MODE == UP_CN_MID || MODE == UP_CN_LAST) {
+        if (a.cn_first) { s0 = U0; s1 = U1; }                  // first sweep: u_hat
+        else { s0 = a.UB[0][idx]; s1 = a.UB[1][idx]; }         // this sweep's midpoint
+    } else if (MODE == UP_B) {
+        ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+        s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
+        s1 = cadd(U1, cscale(cscale(ka1, dt), (T)0.5));
+    } else {
+        ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+        s0 = a.UB[0][idx]; s1 = a.UB[1][idx];
+    }
+    // r_i = N_i + nu * Lap(u_i)   (compute_ut2D :841-846)
+    const T vis = (T)(-a.g.nu * (double)kk2);
+    t0 = cadd(t0, cscale(s0, vis));
+    t1 = cadd(t1, cscale(s1, vis));
+    if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
+    if (a.g.lproject) {
+        // projection2D (tool.F90:219-243)
+        if (kk2 > 0) {
+            const T kkr = sqrt((T)kk2);
+            const T c1 = (T)kx / kkr, c2 = (T)ky / kkr;
+            if (2 * m == N) {
+                // y-Nyquist row (kept by the dealias when ja < 2 ia / 3): the reference
+                // projects the full c2c spectrum, where the partner of (kx, +N/2) is
+                // (-kx, +N/2), then keeps the real part of the inverse, so the two
+                // projections' kx*ky cross terms cancel
+                t0 = cscale(cscale(t0, c1), c1);
+                t1 = cscale(cscale(t1, c2), c2);
             } else {
-                t0 = {0, 0}; t1 = {0, 0};
+                const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
+                t0 = cscale(ud, c1);
+                t1 = cscale(ud, c2);
             }
-        }
-        if (MODE == CF_CN_MID) {
-            // CN2D (solution.F90:215-225): u = uold + dt*r; next midpoint 0.5*(u + uold)
-            const cx<T> n0 = cadd(U0, cscale(t0, dt)), n1 = cadd(U1, cscale(t1, dt));
-            a.UB[0][idx] = cscale(cadd(n0, U0), (T)0.5);
-            a.UB[1][idx] = cscale(cadd(n1, U1), (T)0.5);
-        } else if (MODE == CF_CN_LAST) {
-            a.U[0][idx] = cadd(U0, cscale(t0, dt));
-            a.U[1][idx] = cadd(U1, cscale(t1, dt));
-        } else if (MODE == CF_A) {
-            a.KA[0][idx] = t0; a.KA[1][idx] = t1;
-            a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
-            a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
-        } else if (MODE == CF_B) {
-            const T dt2 = (T)2 * dt;
-            a.UB[0][idx] = cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2));
-            a.UB[1][idx] = cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2));
-            a.KA[0][idx] = cadd(ka0, cscale(t0, (T)4));
-            a.KA[1][idx] = cadd(ka1, cscale(t1, (T)4));
         } else {
-            const T r6 = (T)(1.0 / 6.0);
-            a.U[0][idx] = cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6));
-            a.U[1][idx] = cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6));
+            t0 = {0, 0}; t1 = {0, 0};
         }
+    }
+    if (MODE == UP_CN_MID) {
+        const cx<T> n0 = cadd(U0, cscale(t0, dt)), n1 = cadd(U1, cscale(t1, dt));
+        a.UB[0][idx] = cscale(cadd(n0, U0), (T)0.5);
+        a.UB[1][idx] = cscale(cadd(n1, U1), (T)0.5);
+    } else if (MODE == UP_CN_LAST) {
+        a.U[0][idx] = cadd(U0, cscale(t0, dt));
+        a.U[1][idx] = cadd(U1, cscale(t1, dt));
+    } else if (MODE == UP_A) {
+        a.KA[0][idx] = t0; a.KA[1][idx] = t1;
+        a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
+        a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
+    } else if (MODE == UP_B) {
+        const T dt2 = (T)2 * dt;
+        a.UB[0][idx] = cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2));
+        a.UB[1][idx] = cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2));
+        a.KA[0][idx] = cadd(ka0, cscale(t0, (T)4));
+        a.KA[1][idx] = cadd(ka1, cscale(t1, (T)4));
+    } else {
+        const T r6 = (T)(1.0 / 6.0);
+        a.U[0][idx] = cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6));
+        a.U[1][idx] = cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6));
     }
 }
 
@@ -339,8 +260,9 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
 // pass over A1 A2 B1 B2 C1 C2 D produces both the x-forward nonlinear term
 // and the 8 moments' partial sums (part[q * gridDim.x + block]).
 template <typename T, int N>
-__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_stage_vg_kernel(VgArgs<T> a) {
-    using C = RowCfg<T, N>;
+__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void row_stage_vg_kernel(VgArgs<T> a) {
+    using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
     __shared__ cx<T> lds[LR * lds_line(N)];
@@ -397,6 +319,10 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_stage_vg_kernel(V
         s_urms += u1 * u1 + u2 * u2;
         s_dudx += b11 * b11 + b22 * b22;
     }
+    // nonlinear term: x-forward and split into the two half rows (done here, so
+    // its registers are free for the Laplacian transforms below)
+    line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
     // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -429,9 +355,6 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_stage_vg_kernel(V
         const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
         s_dis += uu * (double)w[k].x;
     }
-    // nonlinear term: x-forward and split into the two half rows
-    line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
-    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
 
     if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = eta0; }
     double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
@@ -447,115 +370,6 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_stage_vg_kernel(V
         double acc = tid == 7 ? eta0 : 0.0;
         for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
         a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;
-    }
-}
-
-// velocity-gradient moments (velgrad_calculate2D, solution.F90:1110-1186)
-template <typename T, int N>
-__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void velgrad_kernel(VgArgs<T> a) {
-    using C = RowCfg<T, N>;
-    constexpr int E = C::E, P = C::P, LR = C::L;
-    if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
-    __shared__ double red[C::THREADS / 64][8];
-    const int tid = threadIdx.x;
-    const int l = tid / P, j = tid % P;
-    const int row0 = blockIdx.x * LR + l;
-    const bool valid = row0 < a.g.n1;
-    const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
-    const size_t hp = (size_t)a.g.hp;
-    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
-    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
-    const cx<T>* B1 = a.w[2] + (size_t)row * hp;
-    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
-    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
-    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
-    const cx<T>* D = a.w[6] + (size_t)row * hp;
-    const double nu = a.g.nu;
-
-    cx<T> z[E], w[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        z[k] = pack_at<T, N>(A1, A2, m);
-        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
-        w[k] = cscale(cmuli(z[k]), kxs);
-    }
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a11 = d1u1, a21 = d1u2
-    double a11[E], a21[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) { a11[k] = w[k].x; a21[k] = w[k].y; }
-#pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(B1, B2, j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a12 = d2u1, a22 = d2u2
-
-    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0;
-    double eta = (double)(2.0f * 3.14f);
-    double dv[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const double u1 = z[k].x, u2 = z[k].y;
-        const double b11 = a11[k], b21 = a21[k], b12 = w[k].x, b22 = w[k].y;
-        const double div = b11 + b22;
-        const double uu = u1 * u1 + u2 * u2;
-        dv[k] = div;
-        s_um2 += div * div * uu;
-        s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
-        s_u2t += div * uu;
-        s_eps += nu * (b12 - b21) * (b12 - b21) + nu * div * div;
-        eta = fmin(eta, sqrt(nu / fabs(div)));
-        s_urms += u1 * u1 + u2 * u2;
-        s_dudx += b11 * b11 + b22 * b22;
-    }
-    // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int kx = m <= N / 2 ? m : N - m;
-        const T k1sq = (T)kx * (T)kx;
-        const cx<T> pa = pack_at<T, N>(A1, A2, m);
-        const cx<T> pc = pack_at<T, N>(C1, C2, m);
-        w[k] = cadd(cscale(pa, -k1sq), pc);
-    }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
-#pragma unroll
-    for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * dv[k] * (double)w[k].x + 2.0 * (double)z[k].y * dv[k] * (double)w[k].y;
-    // Lap(theta) = x-inverse of (-i k1^3 A1 + i k1 C1 - k1^2 B2 + D), a real field
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T k1 = (T)h;
-        const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> th = cadd(cadd(cscale(cmuli(A1[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
-                              cadd(cscale(B2[h], -(k1 * k1)), D[h]));
-        if (m == 0 || m == N / 2) w[k] = {th.x, (T)0};
-        else if (m < N / 2) w[k] = th;
-        else w[k] = cconj(th);
-    }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
-        s_dis += uu * (double)w[k].x;
-    }
-    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = (double)(2.0f * 3.14f); }
-    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
-    const int wave = tid / 64, lane = tid % 64;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
-    vals[7] = wave_min(vals[7]);
-    __syncthreads();
-    if (lane == 0)
-        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
-    __syncthreads();
-    if (tid < 8) {
-        double acc = tid == 7 ? (double)(2.0f * 3.14f) : 0.0;
-        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
-        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;  // [quantity][block]
     }
 }
 
@@ -800,38 +614,6 @@ static inline int xcd_grid_h(int nb) { return ((nb + 7) / 8) * 8; }
 #define K2D_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 template <typename T, int N>
-static int col_inv_n(int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s) {
-    using C = ColCfg<T, N>;
-    if (ncols % C::CB) return -1;
-    dim3 grid(xcd_grid_h(ncols / C::CB)), block(C::THREADS);
-    switch (mode) {
-        case CI_PLAIN: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_PLAIN>), grid, block, 0, s, a); break;
-        case CI_STAGE: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_STAGE>), grid, block, 0, s, a); break;
-        case CI_STAGE_VG: hipLaunchKernelGGL((col_inv_kernel<T, N, CI_STAGE_VG>), grid, block, 0, s, a); break;
-        default: return -1;
-    }
-    return 0;
-}
-
-template <typename T, int N>
-static int col_fwd_n(int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s) {
-    using C = ColCfg<T, N>;
-    if (ncols % C::CB) return -1;
-    dim3 grid(xcd_grid_h(ncols / C::CB)), block(C::THREADS);
-    switch (mode) {
-        case CF_PLAIN: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PLAIN>), grid, block, 0, s, a); break;
-        case CF_PROLOGUE: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_PROLOGUE>), grid, block, 0, s, a); break;
-        case CF_A: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_A>), grid, block, 0, s, a); break;
-        case CF_B: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_B>), grid, block, 0, s, a); break;
-        case CF_C: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_C>), grid, block, 0, s, a); break;
-        case CF_CN_MID: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_CN_MID>), grid, block, 0, s, a); break;
-        case CF_CN_LAST: hipLaunchKernelGGL((col_fwd_kernel<T, N, CF_CN_LAST>), grid, block, 0, s, a); break;
-        default: return -1;
-    }
-    return 0;
-}
-
-template <typename T, int N>
 static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
     using C = RowCfg<T, N>;
     dim3 grid((nrows + C::L - 1) / C::L), block(C::THREADS);
@@ -848,31 +630,24 @@ static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
 
 template <typename T, int N>
 static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
-    using C = RowCfg<T, N>;
+    using C = VgCfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
-    if (a.hout[0]) hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
-    else hipLaunchKernelGGL((velgrad_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    if (!a.hout[0] || !a.hout[1]) return -1;
+    hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
 template <typename T>
-int launch_col_inv(int n, int mode, const ColInvArgs<T>& a, int ncols, hipStream_t s) {
-    switch (n) {
-#define X(NN) case NN: return col_inv_n<T, NN>(mode, a, ncols, s);
-        K2D_SIZES(X)
-#undef X
+int launch_update(int mode, const ColFwdArgs<T>& a, hipStream_t s) {
+    dim3 grid((a.g.nh + 255) / 256, a.g.n1), block(256);
+    switch (mode) {
+#define M(MM) case MM: hipLaunchKernelGGL((update_kernel<T, MM>), grid, block, 0, s, a); break;
+        M(UP_PROLOGUE) M(UP_A) M(UP_B) M(UP_C) M(UP_CN_MID) M(UP_CN_LAST)
+#undef M
+        default: return -1;
     }
-    return -1;
-}
-template <typename T>
-int launch_col_fwd(int n, int mode, const ColFwdArgs<T>& a, int ncols, hipStream_t s) {
-    switch (n) {
-#define X(NN) case NN: return col_fwd_n<T, NN>(mode, a, ncols, s);
-        K2D_SIZES(X)
-#undef X
-    }
-    return -1;
+    return 0;
 }
 template <typename T>
 int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
@@ -915,8 +690,7 @@ int launch_scale(const ScaleArgs<T>& a, hipStream_t s) {
 }
 
 #define K2D_INSTANTIATE(T)                                                                     \
-    template int launch_col_inv<T>(int, int, const ColInvArgs<T>&, int, hipStream_t);          \
-    template int launch_col_fwd<T>(int, int, const ColFwdArgs<T>&, int, hipStream_t);          \
+    template int launch_update<T>(int, const ColFwdArgs<T>&, hipStream_t);                      \
     template int launch_row<T>(int, int, const RowArgs<T>&, int, hipStream_t);                 \
     template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*);             \
     template int launch_spectra<T>(const SpecArgs<T>&, int, hipStream_t);                      \

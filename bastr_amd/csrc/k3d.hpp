@@ -53,7 +53,7 @@ struct Geo3 {
     double nu, dt, inv_n;
 };
 
-enum Cf3Mode { C3_PLAIN = 0, C3_PROLOGUE = 1, C3_A = 2, C3_B = 3, C3_C = 4, C3_CN_MID = 5, C3_CN_LAST = 6 };
+enum Cf3Mode { C3_PROLOGUE = 1, C3_A = 2, C3_B = 3, C3_C = 4, C3_CN_MID = 5, C3_CN_LAST = 6 };
 
 // z-forward pass + spectral update of the three components
 template <typename T>
@@ -64,11 +64,7 @@ struct Cf3Args {
     cx<T>* U[3];
     cx<T>* UB[3];
     cx<T>* KA[3];
-    cx<T>* out[3];        // C3_PLAIN
-    int nfield;
-    int cn_first;
-    int pitch;            // hp*n1 (z pass) or n0*n1 (c2c)
-    int ncols;
+    int cn_first;         // C3_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
     const cx<T>* tw;
 };
 

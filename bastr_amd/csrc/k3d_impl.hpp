@@ -25,38 +25,57 @@ using k2d::pack_at;
 using k2d::unpack_store;
 using k2d::xcd_block;
 
+// ---------------------------------------------------------------------------
+// generic column pass: out_j = FFT(sum_t mult_t(k) in_t), one job per
+// workgroup.  The jobs of one column block are consecutive in the
+// XCD-contiguous block order, so they run together on one XCD and the
+// inputs they share come from its L2.  MT = 1 serves the single-term jobs
+// (most of them) with a lean load phase; MT = 4 the sums.
+// ---------------------------------------------------------------------------
+// multiplier of `mult` at signed wavenumber k as a complex factor (re, im)
 template <typename T>
-__device__ __forceinline__ cx<T> apply_mult(cx<T> u, int mult, T k, T ko) {
+__device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko) {
     switch (mult) {
-        case M_IK: return cscale(cmuli(u), ko);
-        case M_MK2: return cscale(u, -(k * k));
-        case M_MIK3: return cscale(cmuli(u), -(ko * ko * ko));
-        default: return u;
+        case M_IK: return {(T)0, ko};
+        case M_MK2: return {-(k * k), (T)0};
+        case M_MIK3: return {(T)0, -(ko * ko * ko)};
+        default: return {(T)1, (T)0};
     }
 }
 
-// ---------------------------------------------------------------------------
-// generic column pass: out_j = FFT(sum_t mult_t(k) in_t) for each job j
-// ---------------------------------------------------------------------------
-template <typename T, int N, bool INV>
-__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void colx_kernel(ColXArgs<T> a) {
+template <typename T, int N, bool INV, int MT>
+__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
+void colx_kernel(ColXArgs<T> a) {
     using C = ColCfg<T, N>;
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
     const int nbpp = a.ncols / CB;
-    const int blk = xcd_block(blockIdx.x, nbpp * a.nplanes);
-    if (blk < 0) return;
+    const int vb = xcd_block(blockIdx.x, a.njob * nbpp * a.nplanes);
+    if (vb < 0) return;
+    const int q = vb % a.njob, blk = vb / a.njob;
     const int plane = blk / nbpp, cb = blk % nbpp;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
     const size_t base = (size_t)plane * a.plane_stride + (size_t)cb * CB + l;
     const size_t pitch = (size_t)a.pitch;
     const LdsLine<T, true> L{lds + l, CB};
-    const T sc = (T)a.scale;
-    for (int q = 0; q < a.njob; ++q) {
-        const ColJob jb = a.job[q];
-        cx<T> v[E];
+    const ColJob& jb = a.job[q];
+    cx<T> v[E];
+    if (MT == 1) {
+        const cx<T>* __restrict__ src = a.in[jb.in[0]];
+        const int mult = jb.mult[0];
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = src[base + (size_t)(j + k * P) * pitch];
+        if (mult != M_ONE) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int m = j + k * P;
+                const T kk = (T)ksigned(m, N);
+                v[k] = cmul(v[k], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk));
+            }
+        }
+    } else {
 #pragma unroll
         for (int k = 0; k < E; ++k) v[k] = {(T)0, (T)0};
         for (int t = 0; t < jb.nterm; ++t) {
@@ -66,45 +85,46 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void colx_kernel(ColXArgs<
             for (int k = 0; k < E; ++k) {
                 const int m = j + k * P;
                 const T kk = (T)ksigned(m, N);
-                const T ko = (m == N / 2) ? (T)0 : kk;
-                v[k] = cadd(v[k], apply_mult(src[base + (size_t)m * pitch], mult, kk, ko));
+                v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk)));
             }
         }
-        line_fft<T, N, E, INV, true>(v, j, L, a.tw);
-        cx<T>* __restrict__ dst = a.out[jb.out];
-#pragma unroll
-        for (int k = 0; k < E; ++k) dst[base + (size_t)(j + k * P) * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
     }
+    line_fft<T, N, E, INV, true>(v, j, L, a.tw);
+    cx<T>* __restrict__ dst = a.out[jb.out];
+    const T sc = (T)a.scale;
+#pragma unroll
+    for (int k = 0; k < E; ++k) dst[base + (size_t)(j + k * P) * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
 }
 
 // ---------------------------------------------------------------------------
-// z-forward pass + spectral update of three components
+// z-forward pass fused with the spectral update of the three components.
+// The projection couples the components, so a workgroup transforms all
+// three columns of its block and updates the modes in registers:
+//   C3_PROLOGUE  u_hat = dealias(fft3d(u))                       (mainloop3D :111-116)
+//   C3_A/B/C     r = P(D(N + nu Lap s)), Kutta RK3 combination   (RK33D :233-270)
+//   C3_CN_*      u = uold + dt r, next midpoint 0.5 (u + uold)   (CN3D :272-301)
+// with compute_ut3D's viscous term (:903-908), dealiasing3D (tool.F90:196-217)
+// and projection3D (:245-270); s is the stage input.
 // ---------------------------------------------------------------------------
+template <typename T, int N>
+using Fwd3Cfg = ColCfg<T, N, sizeof(T) == 8 ? 8 : 16>;
+
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void cf3_kernel(Cf3Args<T> a) {
-    using C = ColCfg<T, N>;
+__global__ __launch_bounds__((Fwd3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void cf3_kernel(Cf3Args<T> a) {
+    using C = Fwd3Cfg<T, N>;
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
-    const int blk = xcd_block(blockIdx.x, a.ncols / CB);
+    const int ncols = a.g.hp * a.g.n1;
+    const int blk = xcd_block(blockIdx.x, ncols / CB);
     if (blk < 0) return;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
     const int col = blk * CB + l;
     const LdsLine<T, true> L{lds + l, CB};
-    const size_t pitch = (size_t)a.pitch;
+    const size_t pitch = (size_t)ncols;
     const T sc = (T)a.g.inv_n;
-    if (MODE == C3_PLAIN) {
-        for (int f = 0; f < a.nfield; ++f) {
-            cx<T> v[E];
-#pragma unroll
-            for (int k = 0; k < E; ++k) v[k] = a.nx[f][(size_t)(j + k * P) * pitch + col];
-            line_fft<T, N, E, false, true>(v, j, L, a.tw);
-#pragma unroll
-            for (int k = 0; k < E; ++k) a.out[f][(size_t)(j + k * P) * pitch + col] = cscale(v[k], sc);
-        }
-        return;
-    }
     cx<T> v[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -112,9 +132,8 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void cf3_kernel(Cf3Args<T>
         for (int k = 0; k < E; ++k) v[c][k] = a.nx[c][(size_t)(j + k * P) * pitch + col];
         line_fft<T, N, E, false, true>(v[c], j, L, a.tw);
     }
-    const int hp = a.g.hp;
-    const int kx = col % hp;
-    const int ky = ksigned(col / hp, a.g.n1);
+    const int kx = col % a.g.hp;
+    const int ky = ksigned(col / a.g.hp, a.g.n1);
     const T dt = (T)a.g.dt;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -127,7 +146,6 @@ __global__ __launch_bounds__((ColCfg<T, N>::THREADS)) void cf3_kernel(Cf3Args<T>
 #pragma unroll
         for (int c = 0; c < 3; ++c) t[c] = cscale(v[c][k], sc);
         if (MODE == C3_PROLOGUE) {
-            // mainloop3D entry: u_hat = dealias(fft3d(u)) (solution.F90:111-116)
 #pragma unroll
             for (int c = 0; c < 3; ++c) a.U[c][idx] = (kk2 >= a.g.cut2) ? cx<T>{0, 0} : t[c];
             continue;
@@ -338,10 +356,15 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     }
 }
 
-// velocity-gradient moments, 3-D (velgrad_calculate3D, solution.F90:1188-1286)
 template <typename T, int N>
-__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void vg3_kernel(Vg3Args<T> a) {
-    using C = RowCfg<T, N>;
+using Vg3Cfg = k2d::VgCfg<T, N, 4>;
+
+// velocity-gradient moments, 3-D (velgrad_calculate3D, solution.F90:1188-1286)
+
+template <typename T, int N>
+__global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void vg3_kernel(Vg3Args<T> a) {
+    using C = Vg3Cfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
     __shared__ cx<T> lds[LR * lds_line(N)];
@@ -360,67 +383,77 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void vg3_kernel(Vg3Args<T>
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
     // W: 0..2 Y_c, 3..5 Y2_c (d2), 6..8 Y3_c (d3), 9..11 L_c, 12 D
+    // The gradient products enter the moments as soon as both factors are
+    // known (|u|^2 is known after the first two transforms), so at most three
+    // gradient components are held per point at any time.
     cx<T> z[E], w[E];
-    T u3[E], a31[E], a11[E], a21[E], a22[E], a23[E];
-    double e12[E], e13[E], c12[E], c13[E];
+    T u3[E], b31[E], b11[E], b21[E], b22[E], b23[E];
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
     for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(W[2], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, a31 = d1 u3
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u3
 #pragma unroll
-    for (int k = 0; k < E; ++k) { u3[k] = w[k].x; a31[k] = w[k].y; }
+    for (int k = 0; k < E; ++k) { u3[k] = w[k].x; b31[k] = w[k].y; }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         w[k] = cscale(cmuli(pack_at<T, N>(W[0], W[1], m)), kxs);
     }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a11 = d1 u1, a21 = d1 u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d1 u1, d1 u2
 #pragma unroll
-    for (int k = 0; k < E; ++k) { a11[k] = w[k].x; a21[k] = w[k].y; }
+    for (int k = 0; k < E; ++k) { b11[k] = w[k].x; b21[k] = w[k].y; }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[3], W[4], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a12 = d2 u1, a22 = d2 u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u1, d2 u2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        const double a12 = w[k].x;
-        a22[k] = w[k].y;
-        c12[k] = 2.0 * a12 * (double)a21[k];
-        e12[k] = (a12 - (double)a21[k]) * (a12 - (double)a21[k]);
+        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y + (double)u3[k] * u3[k];
+        const double b12 = w[k].x;
+        b22[k] = w[k].y;
+        s_ijji += 2.0 * b12 * (double)b21[k] * uu;
+        s_eps += nu * ((b12 - (double)b21[k]) * (b12 - (double)b21[k]));
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[6], W[7], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a13 = d3 u1, a23 = d3 u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d3 u1, d3 u2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        const double a13 = w[k].x;
-        a23[k] = w[k].y;
-        c13[k] = 2.0 * a13 * (double)a31[k];
-        e13[k] = (a13 - (double)a31[k]) * (a13 - (double)a31[k]);
+        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y + (double)u3[k] * u3[k];
+        const double b13 = w[k].x;
+        b23[k] = w[k].y;
+        s_ijji += 2.0 * b13 * (double)b31[k] * uu;
+        s_eps += nu * ((b13 - (double)b31[k]) * (b13 - (double)b31[k]));
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[5], W[8], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // a32 = d2 u3, a33 = d3 u3
-    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u3, d3 u3
     double dv[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
-        const double b11 = a11[k], b22 = a22[k], b33 = w[k].y, b23 = a23[k], b32 = w[k].x;
+        const double b11k = b11[k], b22k = b22[k], b33 = w[k].y, b23k = b23[k], b32 = w[k].x;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-        const double div = b11 + b22 + b33;
+        const double div = b11k + b22k + b33;
         dv[k] = div;
         s_um2 += div * div * uu;
-        s_ijji += (b11 * b11 + b22 * b22 + b33 * b33 + c12[k] + c13[k] + 2.0 * b23 * b32) * uu;
+        s_ijji += (b11k * b11k + b22k * b22k + b33 * b33 + 2.0 * b23k * b32) * uu;
         s_u2t += div * uu;
-        s_eps += nu * e12[k] + nu * (b23 - b32) * (b23 - b32) + nu * e13[k] + nu * div * div;
+        s_eps += nu * (b23k - b32) * (b23k - b32) + nu * div * div;
         eta = fmin(eta, sqrt(nu / fabs(div)));
         s_urms += u1 * u1 + u2 * u2 + uu3 * uu3;
-        s_dudx += b11 * b11 + b22 * b22 + b33 * b33;
+        s_dudx += b11k * b11k + b22k * b22k + b33 * b33;
     }
     // Lap(u1), Lap(u2) = x-inverse of (-k1^2 Y_c + L_c)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
@@ -434,8 +467,11 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void vg3_kernel(Vg3Args<T>
         s_dis += 2.0 * (double)z[k].x * dv[k] * (double)w[k].x + 2.0 * (double)z[k].y * dv[k] * (double)w[k].y;
     // (Lap(u3), Lap(theta)): Lap(u3) from (-k1^2 Y_3 + L_3); Lap(theta) from
     // -k1^2 (i k1 Y_1 + Y2_2 + Y3_3) + i k1 L_1 + D
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
+        // seven inputs per mode: at most two modes' loads in flight
+        if (k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
         const T k1 = (T)h;
@@ -670,23 +706,27 @@ template <typename T, int N>
 static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     using C = ColCfg<T, N>;
     if (a.ncols % C::CB || a.njob < 1 || a.njob > 13) return -1;
-    const int nb = (a.ncols / C::CB) * a.nplanes;
-    dim3 grid(k2d::xcd_grid_h(nb)), block(C::THREADS);
-    if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((colx_kernel<T, N, false>), grid, block, 0, s, a);
-    return 0;
-}
-
-template <typename T, int N>
-static int cf3_n(int mode, const Cf3Args<T>& a, hipStream_t s) {
-    using C = ColCfg<T, N>;
-    if (a.ncols % C::CB) return -1;
-    dim3 grid(k2d::xcd_grid_h(a.ncols / C::CB)), block(C::THREADS);
-    switch (mode) {
-#define M(MM) case MM: hipLaunchKernelGGL((cf3_kernel<T, N, MM>), grid, block, 0, s, a); break;
-        M(C3_PLAIN) M(C3_PROLOGUE) M(C3_A) M(C3_B) M(C3_C) M(C3_CN_MID) M(C3_CN_LAST)
-#undef M
-        default: return -1;
+    // single-term and multi-term jobs go to the two kernel variants
+    ColXArgs<T> part[2] = {a, a};
+    part[0].njob = part[1].njob = 0;
+    for (int q = 0; q < a.njob; ++q) {
+        if (a.job[q].nterm < 1 || a.job[q].nterm > 4) return -1;
+        ColXArgs<T>& d = part[a.job[q].nterm > 1 ? 1 : 0];
+        d.job[d.njob++] = a.job[q];
+    }
+    for (int v = 0; v < 2; ++v) {
+        const ColXArgs<T>& b = part[v];
+        if (!b.njob) continue;
+        const long nbl = (long)b.njob * (b.ncols / C::CB) * b.nplanes;
+        if (nbl > INT_MAX - 8) return -1;
+        dim3 grid(k2d::xcd_grid_h((int)nbl)), block(C::THREADS);
+        if (v == 0) {
+            if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1>), grid, block, 0, s, b);
+            else hipLaunchKernelGGL((colx_kernel<T, N, false, 1>), grid, block, 0, s, b);
+        } else {
+            if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 4>), grid, block, 0, s, b);
+            else hipLaunchKernelGGL((colx_kernel<T, N, false, 4>), grid, block, 0, s, b);
+        }
     }
     return 0;
 }
@@ -706,21 +746,37 @@ static int row3_n(int mode, const Row3Args<T>& a, int nrows, hipStream_t s) {
 
 template <typename T, int N>
 static int vg3_n(const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
-    using C = RowCfg<T, N>;
+    using C = Vg3Cfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (g != nblk) return -1;
     hipLaunchKernelGGL((vg3_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
+// the column pass also serves the 2-D step, up to 4096 rows
+#define COLX_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 template <typename T>
 int launch_colx(int n, bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     switch (n) {
 #define X(NN) case NN: return colx_n<T, NN>(inverse, a, s);
-        K3D_SIZES(X)
+        COLX_SIZES(X)
 #undef X
     }
     return -1;
+}
+template <typename T, int N>
+static int cf3_n(int mode, const Cf3Args<T>& a, hipStream_t s) {
+    using C = Fwd3Cfg<T, N>;
+    const int ncols = a.g.hp * a.g.n1;
+    if (ncols % C::CB) return -1;
+    dim3 grid(k2d::xcd_grid_h(ncols / C::CB)), block(C::THREADS);
+    switch (mode) {
+#define M(MM) case MM: hipLaunchKernelGGL((cf3_kernel<T, N, MM>), grid, block, 0, s, a); break;
+        M(C3_PROLOGUE) M(C3_A) M(C3_B) M(C3_C) M(C3_CN_MID) M(C3_CN_LAST)
+#undef M
+        default: return -1;
+    }
+    return 0;
 }
 template <typename T>
 int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s) {

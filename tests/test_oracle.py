@@ -418,7 +418,8 @@ def test_oracle_random_phase_field(orc, shape, k0, seed, energy):
         for j in range(i + 1, nd):
             assert np.abs((uh[i] * ks[j] - uh[j] * ks[i])[off]).max() < 1e-12 * scale * max(shape)
     law = np.where(kk >= k0, kk ** 4 * np.exp(-2 * (kk / k0) ** 2), k0 ** 6 / np.where(kk > 0, kk, 1) ** 2 * np.exp(-2))
-    dens = law / (2 * np.pi * kk) if nd == 2 else law / (4 * np.pi * kk ** 2)
+    safe = np.where(kk > 0, kk, 1.0)
+    dens = law / (2 * np.pi * safe) if nd == 2 else law / (4 * np.pi * safe ** 2)
     off &= dens > 1e-10 * dens[off].max()  # modes well above the FFT roundoff floor
     ratio = sum(np.abs(x) ** 2 for x in uh)[off] / dens[off]
     assert np.ptp(ratio) < 1e-9 * ratio.mean()
