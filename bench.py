@@ -70,40 +70,24 @@ def read_prof(lib, h, ncls=10):
     return out
 
 
-# kernel class -> symbol template of the kernel it launches (2-D path)
-KCLASS_SYMBOL = {
-    "row_stage": "void k2d::row_kernel<{T}, {N0}, 2>(",
-    "col_inv": "void k2d::col_inv_kernel<{T}, {N1}, 1>(",
-    "col_inv_vg": "void k2d::col_inv_kernel<{T}, {N1}, 2>(",
-    "velgrad": "void k2d::velgrad_kernel<{T}, {N0}>(",
-    "col_fwd_a": "void k2d::col_fwd_kernel<{T}, {N1}, 2>(",
-    "col_fwd_b": "void k2d::col_fwd_kernel<{T}, {N1}, 3>(",
-    "col_fwd_c": "void k2d::col_fwd_kernel<{T}, {N1}, 4>(",
-    "spectra": "void k2d::spectra_kernel<{T}>(",
-}
-
-
 def traffic_from_profile(kclass, prec, grid):
-    """HBM bytes per launch of the kernel from the committed counter summary
-    (profiles/hbm_traffic_2d_<n>_<prec>.txt), or None if not profiled."""
-    if len(grid) != 2 or grid[0] != grid[1]:
+    """HBM bytes of one invocation of the kernel class, from the committed
+    counter summary of this workload (profiles/hbm_traffic_<tag>.txt, the
+    `class:` lines: FETCH_SIZE + WRITE_SIZE calibrated against a device copy),
+    or None if the workload was not profiled."""
+    if grid == (2048, 2048):
+        tag = "2d_2048"
+    elif grid == (512, 512, 512):
+        tag = "3d_512"
+    else:
         return None
-    T = "float" if prec == 1 else "double"
-    tag = f"2d_{grid[0]}_{'fp32' if prec == 1 else 'fp64'}"
-    sym = KCLASS_SYMBOL.get(kclass)
-    if sym is None:
-        return None
-    sym = sym.format(T=T, N0=grid[0], N1=grid[1])
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}_{'fp32' if prec == 1 else 'fp64'}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith(sym[:60]):
-            parts = line.split()
-            try:
-                return float(parts[-3]) * 1e6  # total_MB column
-            except (ValueError, IndexError):
-                return None
+        parts = line.split()
+        if parts and parts[0] == f"class:{kclass}":
+            return float(parts[3]) * 1e6  # total_MB
     return None
 
 
