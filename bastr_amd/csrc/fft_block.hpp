@@ -17,6 +17,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <climits>
 
 template <typename T>
 struct cx {
@@ -249,13 +250,27 @@ __device__ __forceinline__ double wave_min(double x) {
     for (int o = 32; o >= 1; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
     return x;
 }
-__device__ __forceinline__ int wave_min_i(int x) {
+// Deterministic shell binning for one wavefront.  The 64 lanes hold the
+// modes kx = kx0 + lane of one row, whose shells s are non-decreasing in the
+// lane index (|k| grows with kx >= 0), so every shell is one contiguous run of
+// lanes (INT_MAX marks modes beyond the last shell, a run at the end).  A
+// segmented inclusive scan (Hillis-Steele, fixed order) sums each run and the
+// last lane of the run adds the sums to bins[2*s], bins[2*s + 1].
+__device__ __forceinline__ void bin_runs(double* bins, int s, double a, double b, int lane) {
+    if (__all(s == INT_MAX)) return;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-    return x;
-}
-__device__ __forceinline__ int wave_max_i(int x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
-    return x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int so = __shfl_up(s, o, 64);
+        const double ao = __shfl_up(a, o, 64);
+        const double bo = __shfl_up(b, o, 64);
+        if (lane >= o && so == s) {
+            a += ao;
+            b += bo;
+        }
+    }
+    const int sn = __shfl_down(s, 1, 64);
+    if (s != INT_MAX && (lane == 63 || sn != s)) {
+        bins[2 * s] += a;
+        bins[2 * s + 1] += b;
+    }
 }

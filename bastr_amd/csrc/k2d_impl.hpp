@@ -448,18 +448,7 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
                 const int sh_ = (int)(kk + 0.5);
                 if (sh_ <= K) s = sh_;
             }
-            const int smin = wave_min_i(s);
-            if (smin != INT_MAX) {
-                const int smax = wave_max_i(s == INT_MAX ? -1 : s);
-                for (int b = smin; b <= smax; ++b) {
-                    const double des = wave_sum(s == b ? w * es : 0.0);
-                    const double ded = wave_sum(s == b ? w * ed : 0.0);
-                    if (lane == 0) {
-                        bins[(wave * nb + b) * 2 + 0] += des;
-                        bins[(wave * nb + b) * 2 + 1] += ded;
-                    }
-                }
-            }
+            bin_runs(bins + wave * nb * 2, s, w * es, w * ed, lane);
         }
     }
 #pragma unroll

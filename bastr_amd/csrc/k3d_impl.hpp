@@ -525,46 +525,51 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
     const long ntask = (long)n1 * n2 * nbx;
     const long tstride = 4L * gridDim.x;
     double tot[4] = {0, 0, 0, 0};
-    for (long t = (long)blockIdx.x * 4 + wave; t < ntask; t += tstride) {
-        const long r = t / nbx;  // row = kz*n1 + ky
-        const int kx = (int)(t % nbx) * 64 + lane;
-        double es = 0, ed = 0, w = 0;
-        int s = INT_MAX;
-        if (kx < nh) {
-            const int ky = ksigned((int)(r % n1), n1), kz = ksigned((int)(r / n1), n2);
-            w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
-            const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
-            if (kk > 0.5) {
-                const size_t o = (size_t)r * hp + kx;
-                const cx<T> u1 = a.U[0][o], u2 = a.U[1][o], u3 = a.U[2][o];
-                const double ik = 1.0 / kk;
-                const double c1 = kx * ik, c2 = ky * ik, c3 = kz * ik;
-                const double udx = (double)u1.x * c1 + (double)u2.x * c2 + (double)u3.x * c3;
-                const double udy = (double)u1.y * c1 + (double)u2.y * c2 + (double)u3.y * c3;
-                const double s1x = u1.x - udx * c1, s1y = u1.y - udy * c1;
-                const double s2x = u2.x - udx * c2, s2y = u2.y - udy * c2;
-                const double s3x = u3.x - udx * c3, s3y = u3.y - udy * c3;
-                es = (s1x * s1x + s1y * s1y) * 0.5 + (s2x * s2x + s2y * s2y) * 0.5 + (s3x * s3x + s3y * s3y) * 0.5;
-                ed = (udx * udx + udy * udy) * 0.5;
-                tot[3] += w * (ed * ik);
-            }
-            tot[0] += w * es;
-            tot[1] += w * ed;
-            tot[2] += w * (ed * (kk * kk));
-            const int sh_ = (int)(kk + 0.5);
-            if (sh_ <= K) s = sh_;
+    constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
+    for (long t = (long)blockIdx.x * 4 + wave; t < ntask; t += tstride * TB) {
+        cx<T> u[TB][3];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const long tq = t + tstride * q;
+            const long r = tq / nbx;  // row = kz*n1 + ky
+            const int kx = (int)(tq % nbx) * 64 + lane;
+            const bool ok = tq < ntask && kx < nh;
+            const size_t o = ok ? (size_t)r * hp + kx : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) u[q][c] = ok ? a.U[c][o] : cx<T>{(T)0, (T)0};
         }
-        const int smin = wave_min_i(s);
-        if (smin != INT_MAX) {
-            const int smax = wave_max_i(s == INT_MAX ? -1 : s);
-            for (int b = smin; b <= smax; ++b) {
-                const double des = wave_sum(s == b ? w * es : 0.0);
-                const double ded = wave_sum(s == b ? w * ed : 0.0);
-                if (lane == 0) {
-                    bins[(wave * nb + b) * 2 + 0] += des;
-                    bins[(wave * nb + b) * 2 + 1] += ded;
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const long tq = t + tstride * q;
+            if (tq >= ntask) break;  // wave-uniform
+            const long r = tq / nbx;
+            const int kx = (int)(tq % nbx) * 64 + lane;
+            double es = 0, ed = 0, w = 0;
+            int s = INT_MAX;
+            if (kx < nh) {
+                const int ky = ksigned((int)(r % n1), n1), kz = ksigned((int)(r / n1), n2);
+                w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
+                const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
+                if (kk > 0.5) {
+                    const cx<T> u1 = u[q][0], u2 = u[q][1], u3 = u[q][2];
+                    const double ik = 1.0 / kk;
+                    const double c1 = kx * ik, c2 = ky * ik, c3 = kz * ik;
+                    const double udx = (double)u1.x * c1 + (double)u2.x * c2 + (double)u3.x * c3;
+                    const double udy = (double)u1.y * c1 + (double)u2.y * c2 + (double)u3.y * c3;
+                    const double s1x = u1.x - udx * c1, s1y = u1.y - udy * c1;
+                    const double s2x = u2.x - udx * c2, s2y = u2.y - udy * c2;
+                    const double s3x = u3.x - udx * c3, s3y = u3.y - udy * c3;
+                    es = (s1x * s1x + s1y * s1y) * 0.5 + (s2x * s2x + s2y * s2y) * 0.5 + (s3x * s3x + s3y * s3y) * 0.5;
+                    ed = (udx * udx + udy * udy) * 0.5;
+                    tot[3] += w * (ed * ik);
                 }
+                tot[0] += w * es;
+                tot[1] += w * ed;
+                tot[2] += w * (ed * (kk * kk));
+                const int sh_ = (int)(kk + 0.5);
+                if (sh_ <= K) s = sh_;
             }
+            bin_runs(bins + wave * nb * 2, s, w * es, w * ed, lane);
         }
     }
 #pragma unroll
