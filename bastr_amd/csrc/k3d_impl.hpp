@@ -43,10 +43,14 @@ __device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko) {
     }
 }
 
+// the multi-term variant runs 8 points per thread in FP64 (register pressure)
+template <typename T, int N, int MT>
+using ColxCfg = ColCfg<T, N, (MT == 1 || sizeof(T) == 4) ? 16 : 8>;
+
 template <typename T, int N, bool INV, int MT>
-__global__ __launch_bounds__((ColCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
+__global__ __launch_bounds__((ColxCfg<T, N, MT>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
 void colx_kernel(ColXArgs<T> a) {
-    using C = ColCfg<T, N>;
+    using C = ColxCfg<T, N, MT>;
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
@@ -79,6 +83,9 @@ void colx_kernel(ColXArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < E; ++k) v[k] = {(T)0, (T)0};
         for (int t = 0; t < jb.nterm; ++t) {
+            // one term's loads at a time: keeps the next term's loads from
+            // being hoisted over this one's accumulation (register pressure)
+            __builtin_amdgcn_sched_barrier(0);
             const cx<T>* __restrict__ src = a.in[jb.in[t]];
             const int mult = jb.mult[t];
 #pragma unroll
@@ -88,6 +95,7 @@ void colx_kernel(ColXArgs<T> a) {
                 v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk)));
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
     }
     line_fft<T, N, E, INV, true>(v, j, L, a.tw);
     cx<T>* __restrict__ dst = a.out[jb.out];
@@ -709,8 +717,9 @@ __global__ __launch_bounds__(256) void scale3_kernel(Scale3Args<T> a) {
 
 template <typename T, int N>
 static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
-    using C = ColCfg<T, N>;
-    if (a.ncols % C::CB || a.njob < 1 || a.njob > 13) return -1;
+    using C = ColxCfg<T, N, 1>;
+    using C4 = ColxCfg<T, N, 4>;
+    if (a.ncols % C::CB || a.ncols % C4::CB || a.njob < 1 || a.njob > 13) return -1;
     // single-term and multi-term jobs go to the two kernel variants
     ColXArgs<T> part[2] = {a, a};
     part[0].njob = part[1].njob = 0;
@@ -722,9 +731,10 @@ static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     for (int v = 0; v < 2; ++v) {
         const ColXArgs<T>& b = part[v];
         if (!b.njob) continue;
-        const long nbl = (long)b.njob * (b.ncols / C::CB) * b.nplanes;
+        const int cb = v == 0 ? C::CB : C4::CB;
+        const long nbl = (long)b.njob * (b.ncols / cb) * b.nplanes;
         if (nbl > INT_MAX - 8) return -1;
-        dim3 grid(k2d::xcd_grid_h((int)nbl)), block(C::THREADS);
+        dim3 grid(k2d::xcd_grid_h((int)nbl)), block(v == 0 ? C::THREADS : C4::THREADS);
         if (v == 0) {
             if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1>), grid, block, 0, s, b);
             else hipLaunchKernelGGL((colx_kernel<T, N, false, 1>), grid, block, 0, s, b);
