@@ -66,6 +66,8 @@ SIGNATURES = [
     ("step", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
     ("shell_counts", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("init_random_phase", C.c_int, [_P, C.c_double, C.c_uint64, C.c_double]),
+    ("pp_spectra", C.c_int, [_P, _D, _D, _D, _D, _D]),
+    ("pp_transfer", C.c_int, [_P, _D, _D, _D, _D]),
     ("fft_forward", C.c_int, [_P, _D]),
     ("fft_backward", C.c_int, [_P, _D]),
 ]

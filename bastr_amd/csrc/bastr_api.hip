@@ -19,6 +19,7 @@
 #include "k2d.hpp"
 #include "k3d.hpp"
 #include "init_rp.hpp"
+#include "pp.hpp"
 
 using namespace k2d;
 
@@ -122,6 +123,11 @@ struct bastr_handle {
     std::vector<std::pair<int, size_t>> ev_pending;
     int64_t prof_launch[BASTR_K_NCLASS] = {};
     double prof_ms[BASTR_K_NCLASS] = {};
+    // post-processing arena (bastr_pp_*), allocated on first use
+    void* pp[30] = {};
+    int* pp_status = nullptr;  // always INT_MAX: the post-processing gate stays open
+    double* pp_part = nullptr;
+    int pp_nblk = 0;
 };
 
 namespace {
@@ -145,13 +151,13 @@ struct ProfScope {
             hipEvent_t e = pool_event(h);
             if (!e) { on = false; return; }
             h->ev_pending.push_back({cls, h->ev_used - 1});
-            hipEventRecord(e, h->stream);
+            (void)hipEventRecord(e, h->stream);
         }
     }
     ~ProfScope() {
         if (on) {
             hipEvent_t e = pool_event(h);
-            if (e) hipEventRecord(e, h->stream);
+            if (e) (void)hipEventRecord(e, h->stream);
             else h->ev_pending.pop_back();
         }
     }
@@ -794,8 +800,8 @@ int run_iteration3(bastr_handle* h, int it) {
 
 int ensure_capacity(bastr_handle* h, int n) {
     if (n <= h->cap) return 0;
-    if (h->ddiag) hipFree(h->ddiag);
-    if (h->dspec) hipFree(h->dspec);
+    if (h->ddiag) (void)hipFree(h->ddiag);
+    if (h->dspec) (void)hipFree(h->dspec);
     h->ddiag = nullptr;
     h->dspec = nullptr;
     HIPCHK(hipMalloc(&h->ddiag, sizeof(bastr_diag) * n));
@@ -969,26 +975,30 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
 
 int bastr_destroy(bastr_handle* h) {
     if (!h) return BASTR_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf, h->part_bins, h->part_tot,
                     h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec};
     for (void* b : bufs)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     for (int c = 0; c < 3; ++c) {
-        if (h->U[c]) hipFree(h->U[c]);
-        if (h->UB[c]) hipFree(h->UB[c]);
-        if (h->KA[c]) hipFree(h->KA[c]);
+        if (h->U[c]) (void)hipFree(h->U[c]);
+        if (h->UB[c]) (void)hipFree(h->UB[c]);
+        if (h->KA[c]) (void)hipFree(h->KA[c]);
         if (h->nd == 2) {  // 3-D NX/ureal alias other buffers
-            if (h->NX[c]) hipFree(h->NX[c]);
-            if (h->ureal[c]) hipFree(h->ureal[c]);
+            if (h->NX[c]) (void)hipFree(h->NX[c]);
+            if (h->ureal[c]) (void)hipFree(h->ureal[c]);
         }
     }
     for (int q = 0; q < 13; ++q)
-        if (h->W[q]) hipFree(h->W[q]);
+        if (h->W[q]) (void)hipFree(h->W[q]);
     for (int q = 0; q < 10; ++q)
-        if (h->Z[q]) hipFree(h->Z[q]);
-    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (h->Z[q]) (void)hipFree(h->Z[q]);
+    for (int q = 0; q < 30; ++q)
+        if (h->pp[q]) (void)hipFree(h->pp[q]);
+    if (h->pp_status) (void)hipFree(h->pp_status);
+    if (h->pp_part) (void)hipFree(h->pp_part);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return BASTR_OK;
 }
@@ -1326,5 +1336,262 @@ const char* bastr_prof_name(int c) {
     return (c >= 0 && c < BASTR_K_NCLASS) ? names[c] : "";
 }
 int bastr_fft_backward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, true); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// post-processing of a field: Bastrpp's spectra2d and transfer2d/3d
+// (Bastrpp.F90:174-599).  A separate arena of half-spectrum buffers, so the
+// solver's state, its scratch and a pending prologue stay as they are:
+//   2-D: [0,1] u_hat, [2..5] y-inverse outputs, [6..13] products
+//   3-D: [0..2] u_hat, [3..8] z-inverse outputs, [9..17] y-inverse outputs,
+//        [18..29] products
+// The real input fields are staged in the product buffers.
+// ---------------------------------------------------------------------------
+namespace {
+
+int pp_base(const bastr_handle* h) { return h->nd == 2 ? 6 : 18; }
+
+int pp_arena(bastr_handle* h) {
+    if (h->pp_status) return 0;
+    const int nbuf = h->nd == 2 ? 14 : 30;
+    const size_t cbytes = h->nhalf * 2 * h->esz;
+    const long nbx = (h->nh + 63) / 64;
+    const long ntask = (long)h->n1 * h->n2 * nbx;
+    h->pp_nblk = (int)std::max(1L, std::min(1024L, ntask / 4));
+    const size_t K1 = (size_t)h->allkmax + 1;
+    const size_t npart = std::max((size_t)h->pp_nblk * K1 * 3, (size_t)h->nblk_spec * (K1 * 2 + 4));
+    bool ok = true;
+    for (int q = 0; q < nbuf && ok; ++q) h->pp[q] = dalloc(cbytes, &ok);
+    if (ok) h->pp_part = (double*)dalloc(sizeof(double) * npart, &ok);
+    int* st = ok ? (int*)dalloc(sizeof(int), &ok) : nullptr;
+    const int open = INT_MAX;
+    if (ok && hipMemcpy(st, &open, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ok = false;
+    if (!ok) {
+        for (int q = 0; q < 30; ++q) {
+            if (h->pp[q]) (void)hipFree(h->pp[q]);
+            h->pp[q] = nullptr;
+        }
+        if (h->pp_part) (void)hipFree(h->pp_part);
+        if (st) (void)hipFree(st);
+        h->pp_part = nullptr;
+        return fail(BASTR_ENOMEM, "post-processing: device allocation of %d half fields failed", nbuf);
+    }
+    h->pp_status = st;
+    return 0;
+}
+
+// column transforms along y (axis 1) or z (axis 2) of the arena buffers
+template <typename T>
+int pp_cols(bastr_handle* h, const Gate& g, const int* in, int nin, const int* out, int nout,
+            const k3d::ColJob* jobs, int nj, bool inverse, int axis, double scale) {
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    for (int q = 0; q < nin; ++q) c.in[q] = C<T>(h->pp[in[q]]);
+    for (int q = 0; q < nout; ++q) c.out[q] = C<T>(h->pp[out[q]]);
+    for (int q = 0; q < nj; ++q) c.job[q] = jobs[q];
+    c.njob = nj;
+    if (axis == 1) {
+        c.pitch = h->hp;
+        c.ncols = h->hp;
+        c.plane_stride = (long)h->hp * h->n1;
+        c.nplanes = h->n2;
+    } else {
+        c.pitch = h->hp * h->n1;
+        c.ncols = h->hp * h->n1;
+        c.plane_stride = 0;
+        c.nplanes = 1;
+    }
+    c.scale = scale;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_colx<T>(axis == 1 ? h->n1 : h->n2, inverse, c, h->stream));
+    return 0;
+}
+
+// plain forward transforms, in place, of arena buffers first..first+nf-1
+template <typename T>
+int pp_fwd_inplace(bastr_handle* h, const Gate& g, int first, int nf, int axis, double scale) {
+    for (int b = 0; b < nf; b += 6) {
+        const int m = std::min(6, nf - b);
+        int io[6];
+        k3d::ColJob j[6];
+        for (int q = 0; q < m; ++q) {
+            io[q] = first + b + q;
+            j[q] = job1(q, k3d::M_ONE, q);
+        }
+        int rc = pp_cols<T>(h, g, io, m, io, m, j, m, false, axis, scale);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// u_hat = fft(u) (normalised, no dealias) into arena [0..nd-1]
+template <typename T>
+int pp_load(bastr_handle* h, const double* const* src) {
+    const int rb = pp_base(h);
+    for (int c = 0; c < h->nd; ++c) {
+        if (h->prec == BASTR_PREC_F64) {
+            HIPCHK(hipMemcpy(h->pp[rb + c], src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemcpy(h->dstage, src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3(1024), dim3(256), 0, h->stream, h->dstage,
+                               (float*)h->pp[rb + c], h->npts);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+    }
+    const Gate open{h->pp_status, 0, 0};
+    const double inv_n = 1.0 / ((double)h->n0 * (double)h->n1 * (double)h->n2);
+    k3d::ColJob j[3] = {job1(0, k3d::M_ONE, 0), job1(1, k3d::M_ONE, 1), job1(2, k3d::M_ONE, 2)};
+    if (h->nd == 2) {
+        RowArgs<T> r{};
+        r.g = geo<T>(h);
+        r.gate = open;
+        r.rin[0] = (const T*)h->pp[rb];
+        r.rin[1] = (const T*)h->pp[rb + 1];
+        r.hout[0] = C<T>(h->pp[2]);
+        r.hout[1] = C<T>(h->pp[3]);
+        r.tw = C<T>(h->tw0);
+        LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
+        const int in[2] = {2, 3}, out[2] = {0, 1};
+        return pp_cols<T>(h, open, in, 2, out, 2, j, 2, false, 1, inv_n);
+    }
+    k3d::Row3Args<T> r{};
+    r.g = geo3<T>(h);
+    r.gate = open;
+    for (int c = 0; c < 3; ++c) {
+        r.rin[c] = (const T*)h->pp[rb + c];
+        r.hout[c] = C<T>(h->pp[9 + c]);
+    }
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
+    const int y[3] = {9, 10, 11}, z[3] = {3, 4, 5}, u[3] = {0, 1, 2};
+    int rc = pp_cols<T>(h, open, y, 3, z, 3, j, 3, false, 1, 1.0);
+    if (rc) return rc;
+    return pp_cols<T>(h, open, z, 3, u, 3, j, 3, false, 2, inv_n);
+}
+
+template <typename T>
+int pp_spectra_impl(bastr_handle* h, const double* const* src, double* shells, double* totals) {
+    int rc = pp_load<T>(h, src);
+    if (rc) return rc;
+    const int nb = h->nblk_spec, K = h->allkmax;
+    SpecArgs<T> sa{};
+    sa.g = geo<T>(h);
+    sa.gate = Gate{h->pp_status, 0, 0};
+    sa.U[0] = C<T>(h->pp[0]);
+    sa.U[1] = C<T>(h->pp[1]);
+    sa.allkmax = K;
+    sa.part_bins = h->pp_part;
+    sa.part_tot = h->pp_part + (size_t)nb * (K + 1) * 2;
+    LAUNCHCHK(launch_spectra<T>(sa, nb, h->stream));
+    std::vector<double> part((size_t)nb * ((K + 1) * 2 + 4));
+    HIPCHK(hipMemcpyAsync(part.data(), h->pp_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the device bins carry spectra_compute2D's 1/2; spectra2d has none
+    for (int b = 1; b <= K; ++b)
+        for (int c = 0; c < 2; ++c) {
+            double acc = 0;
+            for (int k = 0; k < nb; ++k) acc += part[((size_t)b * 2 + c) * nb + k];
+            shells[(size_t)c * K + b - 1] = 2.0 * acc;
+        }
+    const double* tot = part.data() + (size_t)nb * (K + 1) * 2;
+    for (int q = 0; q < 4; ++q) {
+        double acc = 0;
+        for (int k = 0; k < nb; ++k) acc += tot[(size_t)q * nb + k];
+        totals[q] = 2.0 * acc;
+    }
+    return BASTR_OK;
+}
+
+template <typename T>
+int pp_transfer_impl(bastr_handle* h, const double* const* src, double* out) {
+    int rc = pp_load<T>(h, src);
+    if (rc) return rc;
+    const Gate open{h->pp_status, 0, 0};
+    const int nd = h->nd, rb = pp_base(h), K = h->allkmax;
+    const double inv_n = 1.0 / ((double)h->n0 * (double)h->n1 * (double)h->n2);
+    pp::RowArgs<T> ra{};
+    ra.g = geo<T>(h);
+    ra.n2 = h->n2;
+    ra.gate = open;
+    ra.tw = C<T>(h->tw0);
+    if (nd == 2) {
+        // y-inverse: u1, u2, i k2 u1, i k2 u2
+        const int in[2] = {0, 1}, yo[4] = {2, 3, 4, 5};
+        const k3d::ColJob j[4] = {job1(0, k3d::M_ONE, 0), job1(1, k3d::M_ONE, 1), job1(0, k3d::M_IK, 2),
+                                  job1(1, k3d::M_IK, 3)};
+        if ((rc = pp_cols<T>(h, open, in, 2, yo, 4, j, 4, true, 1, 1.0))) return rc;
+        for (int q = 0; q < 4; ++q) ra.hin[q] = C<T>(h->pp[2 + q]);
+        for (int q = 0; q < 8; ++q) ra.hout[q] = C<T>(h->pp[rb + q]);
+        LAUNCHCHK(pp::launch_row2<T>(h->n0, ra, h->stream));
+        if ((rc = pp_fwd_inplace<T>(h, open, rb, 7, 1, inv_n))) return rc;
+    } else {
+        // z-inverse: u_c, i k3 u_c; y-inverse: u_c, i k2 u_c, (i k3 u_c)
+        const int in[3] = {0, 1, 2}, zo[6] = {3, 4, 5, 6, 7, 8};
+        k3d::ColJob j[9];
+        for (int q = 0; q < 3; ++q) {
+            j[q] = job1(q, k3d::M_ONE, q);
+            j[3 + q] = job1(q, k3d::M_IK, 3 + q);
+        }
+        if ((rc = pp_cols<T>(h, open, in, 3, zo, 6, j, 6, true, 2, 1.0))) return rc;
+        const int yo[9] = {9, 10, 11, 12, 13, 14, 15, 16, 17};
+        for (int q = 0; q < 3; ++q) j[6 + q] = job1(3 + q, k3d::M_ONE, 6 + q);
+        if ((rc = pp_cols<T>(h, open, zo, 6, yo, 9, j, 9, true, 1, 1.0))) return rc;
+        for (int q = 0; q < 9; ++q) ra.hin[q] = C<T>(h->pp[9 + q]);
+        for (int q = 0; q < 12; ++q) ra.hout[q] = C<T>(h->pp[rb + q]);
+        LAUNCHCHK(pp::launch_row3<T>(h->n0, ra, h->stream));
+        if ((rc = pp_fwd_inplace<T>(h, open, rb, 12, 1, 1.0))) return rc;
+        if ((rc = pp_fwd_inplace<T>(h, open, rb, 12, 2, inv_n))) return rc;
+    }
+    pp::BinArgs<T> ba{};
+    ba.nd = nd;
+    ba.n0 = h->n0;
+    ba.n1 = h->n1;
+    ba.n2 = h->n2;
+    ba.nh = h->nh;
+    ba.hp = h->hp;
+    ba.gate = open;
+    for (int c = 0; c < nd; ++c) ba.U[c] = C<T>(h->pp[c]);
+    for (int q = 0; q < (nd == 2 ? 8 : 12); ++q) ba.F[q] = C<T>(h->pp[rb + q]);
+    ba.allkmax = K;
+    ba.part = h->pp_part;
+    LAUNCHCHK(pp::launch_bins<T>(ba, h->pp_nblk, h->stream));
+    const int nb = h->pp_nblk;
+    std::vector<double> part((size_t)nb * (K + 1) * 3);
+    HIPCHK(hipMemcpyAsync(part.data(), h->pp_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int b = 1; b <= K; ++b)
+        for (int q = 0; q < 3; ++q) {
+            double acc = 0;
+            for (int k = 0; k < nb; ++k) acc += part[((size_t)b * 3 + q) * nb + k];
+            out[(size_t)q * K + b - 1] = acc;
+        }
+    return BASTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* shells,
+                     double* totals) {
+    (void)u3;
+    if (!h || !u1 || !u2 || !shells || !totals) return fail(BASTR_EINVAL, "bastr_pp_spectra: NULL argument");
+    if (h->nd != 2) return fail(BASTR_EINVAL, "bastr_pp_spectra: spectra2d is 2-D only");
+    int rc = pp_arena(h);
+    if (rc) return rc;
+    const double* src[3] = {u1, u2, nullptr};
+    return h->prec == BASTR_PREC_F32 ? pp_spectra_impl<float>(h, src, shells, totals)
+                                     : pp_spectra_impl<double>(h, src, shells, totals);
+}
+
+int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* out) {
+    if (!h || !u1 || !u2 || !out || (h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_pp_transfer: NULL argument");
+    int rc = pp_arena(h);
+    if (rc) return rc;
+    const double* src[3] = {u1, u2, u3};
+    return h->prec == BASTR_PREC_F32 ? pp_transfer_impl<float>(h, src, out) : pp_transfer_impl<double>(h, src, out);
+}
 
 }  // extern "C"

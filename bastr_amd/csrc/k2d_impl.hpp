@@ -425,9 +425,10 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
                 if (kk > 0.5) {
                     const double ik = 1.0 / kk;
                     const double c1 = kx * ik, c2 = ky * ik;
-                    if (2 * r == n1 && kx > 0) {
+                    if (2 * r == n1 && kx > 0 && 2 * kx < n0) {
                         // y-Nyquist row: this entry stands for the reference's modes
                         // (kx, +N/2) and (-kx, +N/2), whose kx*ky cross terms cancel
+                        // (the corner (N/2, N/2) stands alone and keeps them)
                         const double a1 = (double)u1[q].x * (double)u1[q].x + (double)u1[q].y * (double)u1[q].y;
                         const double a2 = (double)u2[q].x * (double)u2[q].x + (double)u2[q].y * (double)u2[q].y;
                         es = (a1 * (c2 * c2) + a2 * (c1 * c1)) * 0.5;

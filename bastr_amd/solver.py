@@ -109,6 +109,26 @@ class Solver:
         rc = self._check(self._fn("step")(self._h, nsteps, diags, _dptr(spec) if spec is not None else None))
         return rc, diags, spec
 
+    def pp_spectra(self, *u):
+        """Bastrpp spectra2d of the given fields: (Es[1..K], Ed[1..K]),
+        (Esspe, Edspe, k2Edspe, kM1Edspe)."""
+        K = self.allkmax
+        fs = [self._field(x) for x in u[: self.nd]]
+        ptrs = [_dptr(f) for f in fs] + [None] * (3 - self.nd)
+        shells = np.zeros(2 * K)
+        totals = np.zeros(4)
+        self._check(self._fn("pp_spectra")(self._h, *ptrs, _dptr(shells), _dptr(totals)))
+        return shells.reshape(2, K), totals
+
+    def pp_transfer(self, *u):
+        """Bastrpp transfer2d/3d of the given fields: array (3, K) of T, T1, T2."""
+        K = self.allkmax
+        fs = [self._field(x) for x in u[: self.nd]]
+        ptrs = [_dptr(f) for f in fs] + [None] * (3 - self.nd)
+        out = np.zeros(3 * K)
+        self._check(self._fn("pp_transfer")(self._h, *ptrs, _dptr(out)))
+        return out.reshape(3, K)
+
     def shell_counts(self):
         c = np.zeros(self.allkmax, dtype=np.int64)
         self._check(self._fn("shell_counts")(self._h, c.ctypes.data_as(C.POINTER(C.c_int64))))

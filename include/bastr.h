@@ -146,6 +146,24 @@ int bastr_get_time(const bastr_handle* h, int64_t* nstep, double* time);
  */
 int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra);
 
+/*
+ * Post-processing of a field, e.g. a dump (Bastrpp's commands; the solver state,
+ * nstep and time of the handle are left as they were).  Fields are Fortran-order
+ * real arrays as for bastr_set_fields, used as given (no entry dealias).
+ *
+ * bastr_pp_spectra: spectra2d (Bastrpp.F90:299-400), 2-D only: over the full
+ *   spectrum, without the 1/2 of the solver's diagnostics,
+ *   shells[0..K-1] = Es(1..K), shells[K..2K-1] = Ed(1..K) (K = allkmax);
+ *   totals = Esspe, Edspe, k2Edspe, kM1Edspe.
+ * bastr_pp_transfer: transfer2d / transfer3d (Bastrpp.F90:174-297, 402-599):
+ *   out[0..K-1] = T(1..K), out[K..2K-1] = T1, out[2K..3K-1] = T2, shell sums of
+ *   Re(conj(u_hat_i) fft(-u_j d_j u_i)), Im(k_j fft(u_i u_j) conj(u_hat_i)) and
+ *   Re(conj(u_hat_i) fft(u_i d_j u_j)).
+ */
+int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* shells,
+                     double* totals);
+int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* out);
+
 /* Number of modes per shell 1..allkmax (Ecount, solution.F90:976). */
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts);
 

@@ -891,3 +891,151 @@ int bo_init_random_phase(bo_state* s, double k0, uint64_t seed, double energy) {
     s->time = 0.0;
     return BASTR_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* post-processing of a field: Bastrpp's spectra2d and transfer2d/3d          */
+/* ------------------------------------------------------------------------- */
+
+static cplx* xalloc_c(size_t n) { return (cplx*)calloc(n, sizeof(cplx)); }
+
+/* spectra2d (Bastrpp.F90:299-400): shell sums of |u_s|^2 and |u_d|^2 over the
+ * full c2c spectrum of a field as read from a dump (no entry dealias, and no
+ * 1/2, unlike spectra_compute2D).  shells[0..K-1] = Es(1..K), shells[K..2K-1] =
+ * Ed(1..K); totals = Esspe, Edspe, k2Edspe, kM1Edspe (the module's Esspe and
+ * Edspe start from zero in a Bastrpp run). */
+int bo_pp_spectra(bo_state* s, const double* u1, const double* u2, const double* u3, double* shells,
+                  double* totals) {
+    (void)u3;
+    if (s->nd != 2) return fail(BASTR_EINVAL, "bo_pp_spectra: spectra2d is 2-D only");
+    if (!u1 || !u2 || !shells || !totals) return fail(BASTR_EINVAL, "bo_pp_spectra: NULL argument");
+    const size_t n = s->n;
+    const int K = s->allkmax;
+    cplx* a = xalloc_c(n);
+    cplx* b = xalloc_c(n);
+    double* Es = (double*)calloc((size_t)K + 1, sizeof(double));
+    double* Ed = (double*)calloc((size_t)K + 1, sizeof(double));
+    if (!a || !b || !Es || !Ed) { free(a); free(b); free(Es); free(Ed); return fail(BASTR_ENOMEM, "bo_pp_spectra"); }
+    for (size_t m = 0; m < n; ++m) { a[m] = u1[m]; b[m] = u2[m]; }
+    fftf(s, a);
+    fftf(s, b);
+    const double dk = 1.0;
+    double Esspe = 0, Edspe = 0, k2Edspe = 0, kM1Edspe = 0;
+    for (size_t m = 0; m < n; ++m) {
+        const double k1 = s->kw[0][m], k2 = s->kw[1][m];
+        const double kk = sqrt(k1 * k1 + k2 * k2);
+        cplx usspe = 0, udspe = 0;
+        if (kk > dk / 2) {
+            usspe = a[m] * k2 / kk - b[m] * k1 / kk;
+            udspe = a[m] * k1 / kk + b[m] * k2 / kk;
+            kM1Edspe = kM1Edspe + creal(udspe * conj(udspe)) / kk;
+        }
+        const int ko = kint2(kk, dk);
+        if (ko <= K) {
+            Es[ko] = Es[ko] + creal(usspe * conj(usspe));
+            Ed[ko] = Ed[ko] + creal(udspe * conj(udspe));
+        }
+        Esspe = Esspe + creal(usspe * conj(usspe));
+        Edspe = Edspe + creal(udspe * conj(udspe));
+        k2Edspe = k2Edspe + creal(udspe * conj(udspe)) * (kk * kk);
+    }
+    for (int i = 1; i <= K; ++i) { shells[i - 1] = Es[i]; shells[K + i - 1] = Ed[i]; }
+    totals[0] = Esspe; totals[1] = Edspe; totals[2] = k2Edspe; totals[3] = kM1Edspe;
+    free(a); free(b); free(Es); free(Ed);
+    return BASTR_OK;
+}
+
+/* transfer2d / transfer3d (Bastrpp.F90:174-297, 402-599): shell sums over the
+ * full c2c spectrum of
+ *   T  = Re(conj(u_hat_i) fft(-u_j d_j u_i)),
+ *   T1 = Im(k_j fft(u_i u_j) conj(u_hat_i)),
+ *   T2 = Re(conj(u_hat_i) fft(u_i theta)),  theta = d_j u_j,
+ * with the derivatives spectral and the products in physical space, from a
+ * field as read from a dump.  out[0..K-1] = T(1..K), out[K..] = T1, out[2K..] = T2. */
+int bo_pp_transfer(bo_state* s, const double* u1, const double* u2, const double* u3, double* out) {
+    const int nd = s->nd;
+    const size_t n = s->n;
+    const int K = s->allkmax;
+    const double* src[3] = {u1, u2, u3};
+    for (int c = 0; c < nd; ++c)
+        if (!src[c]) return fail(BASTR_EINVAL, "bo_pp_transfer: missing component");
+    if (!out) return fail(BASTR_EINVAL, "bo_pp_transfer: NULL argument");
+    cplx* sp[3] = {0, 0, 0};
+    cplx* ux[3][3] = {{0}};
+    cplx* tA[3] = {0, 0, 0};
+    cplx* tC[3] = {0, 0, 0};
+    int ok = 1;
+    for (int i = 0; i < nd; ++i) {
+        if (!(sp[i] = xalloc_c(n)) || !(tA[i] = xalloc_c(n)) || !(tC[i] = xalloc_c(n))) ok = 0;
+        for (int j = 0; j < nd; ++j)
+            if (!(ux[i][j] = xalloc_c(n))) ok = 0;
+    }
+    double* T = (double*)calloc((size_t)K + 1, sizeof(double));
+    double* T1 = (double*)calloc((size_t)K + 1, sizeof(double));
+    double* T2 = (double*)calloc((size_t)K + 1, sizeof(double));
+    if (!T || !T1 || !T2) ok = 0;
+    if (ok) {
+        for (int i = 0; i < nd; ++i) {
+            for (size_t m = 0; m < n; ++m) sp[i][m] = src[i][m];
+            fftf(s, sp[i]);
+        }
+        /* uixj = imag * uispe * kj; ifft of all, then of uispe */
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                for (size_t m = 0; m < n; ++m) ux[i][j][m] = I * sp[i][m] * s->kw[j][m];
+                ffti(s, ux[i][j]);
+            }
+        for (int i = 0; i < nd; ++i) ffti(s, sp[i]);
+        for (size_t m = 0; m < n; ++m) {
+            /* theta = dreal(u1x1 + u2x2 [+ u3x3]) */
+            cplx dsum = 0;
+            for (int j = 0; j < nd; ++j) dsum += ux[j][j][m];
+            const double th = creal(dsum);
+            for (int i = 0; i < nd; ++i) {
+                cplx t = 0;
+                for (int j = 0; j < nd; ++j) t = t - sp[j][m] * creal(ux[i][j][m]);
+                tA[i][m] = t;
+                tC[i][m] = sp[i][m] * th;
+            }
+            /* the derivative arrays are reused for the products u_i u_j */
+            cplx uu[3][3];
+            for (int i = 0; i < nd; ++i)
+                for (int j = 0; j < nd; ++j) uu[i][j] = sp[i][m] * sp[j][m];
+            for (int i = 0; i < nd; ++i)
+                for (int j = 0; j < nd; ++j) ux[i][j][m] = uu[i][j];
+        }
+        for (int i = 0; i < nd; ++i) { fftf(s, tA[i]); }
+        for (int i = 0; i < nd; ++i) { fftf(s, sp[i]); }
+        for (int i = 0; i < nd; ++i) { fftf(s, tC[i]); }
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) fftf(s, ux[i][j]);
+        for (size_t m = 0; m < n; ++m) {
+            double kk2 = 0;
+            for (int j = 0; j < nd; ++j) kk2 += s->kw[j][m] * s->kw[j][m];
+            const double kk = sqrt(kk2);
+            double trans = 0, trans2 = 0;
+            cplx z1 = 0;
+            for (int i = 0; i < nd; ++i) {
+                trans += creal(conj(sp[i][m]) * tA[i][m]);
+                trans2 += creal(conj(sp[i][m]) * tC[i][m]);
+                for (int j = 0; j < nd; ++j) z1 += s->kw[j][m] * ux[i][j][m] * conj(sp[i][m]);
+            }
+            const int ko = kint2(kk, 1.0);
+            if (ko <= K) {
+                T[ko] = T[ko] + trans;
+                T1[ko] = T1[ko] + cimag(z1);
+                T2[ko] = T2[ko] + trans2;
+            }
+        }
+        for (int i = 1; i <= K; ++i) {
+            out[i - 1] = T[i];
+            out[K + i - 1] = T1[i];
+            out[2 * K + i - 1] = T2[i];
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        free(sp[i]); free(tA[i]); free(tC[i]);
+        for (int j = 0; j < 3; ++j) free(ux[i][j]);
+    }
+    free(T); free(T1); free(T2);
+    return ok ? BASTR_OK : fail(BASTR_ENOMEM, "bo_pp_transfer: allocation failed");
+}

@@ -35,6 +35,12 @@ int bo_shell_counts(const bo_state* s, int64_t* counts);
 int bo_fft_forward(bo_state* s, double* inout);
 int bo_fft_backward(bo_state* s, double* inout);
 
+/* Post-processing of a field (Bastrpp's spectra2d, transfer2d/3d); the
+ * solver state is left as it was. */
+int bo_pp_spectra(bo_state* s, const double* u1, const double* u2, const double* u3, double* shells,
+                  double* totals);
+int bo_pp_transfer(bo_state* s, const double* u1, const double* u2, const double* u3, double* out);
+
 /* Stand-alone c2c transform of a Fortran-order (n0, n1[, n2]) complex128 array
  * with fft2d/ifft2d conventions: dir=-1 forward and normalised, dir=+1
  * backward unnormalised.  n2 == 0 means 2-D. */

@@ -427,3 +427,60 @@ def test_oracle_random_phase_field(orc, shape, k0, seed, energy):
     o2 = orc.solver(p)
     o2.set_fields(*u)
     assert rel_l2(o.get_fields(), o2.get_fields()) < 1e-13
+
+
+def _mixed(shape, seed):
+    d = hitgen(shape, k0=6 if len(shape) == 2 else 4, seed=seed)
+    r = hitgen(shape, k0=3, seed=seed + 1)
+    if len(shape) == 2:
+        u = [d[0] - 0.7 * r[1], d[1] + 0.7 * r[0]]
+    else:
+        u = [d[0] - 0.5 * r[1], d[1] + 0.5 * r[0], d[2] + 0.3 * r[2]]
+    return [np.asfortranarray(x) for x in u]
+
+
+def test_oracle_pp_spectra2d_is_twice_the_solver_spectra(orc):
+    """spectra2d (Bastrpp.F90:299-400) is spectra_compute2D without the 1/2,
+    over the same full spectrum (the solver's spectra are pinned above)."""
+    shape = (64, 32)
+    p = make_params(*shape, kmax=128)
+    o = orc.solver(p)
+    o.set_fields(*_mixed(shape, 4))
+    u = o.get_fields()  # entry-dealiased, the field the solver's diagnostics see
+    rc, dg, sp = o.step(1)
+    sh, tot = o.pp_spectra(*u)
+    assert np.abs(sh - 2 * sp[0, 1:, :]).max() <= 1e-14 * np.abs(sh).max()
+    ref = 2 * np.array([dg[0].Esspe, dg[0].Edspe, dg[0].k2Edspe, dg[0].kMEdspe])
+    assert np.abs(tot - ref).max() <= 1e-14 * np.abs(ref).max()
+    assert o.get_time() == (1, 1e-4)  # the solver state is untouched
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (32, 64), (32, 32, 32)])
+def test_oracle_pp_transfer_identities(orc, shape):
+    """transfer2d/3d (Bastrpp.F90:174-297, 402-599), pinned by what the
+    equations fix, independently of the formulas:
+      - T(k) is the nonlinear term's share of dE(k)/dt: an inviscid, unforced
+        solver step (compute_ut, another code path) gives dE/dt, extrapolated
+        to dt -> 0 from two step sizes;
+      - -u_j d_j u_i = -d_j(u_i u_j) + u_i theta, so T = T1 + T2 on shells whose
+        modes lie below the 2/3 cutoff (no aliasing there; with ia <= ja the
+        cutoff floor(ia/3) dealiases both directions)."""
+    nd = len(shape)
+    p = make_params(shape[0], shape[1], shape[2] if nd == 3 else 0, kmax=128)
+    o = orc.solver(p)
+    o.set_fields(*_mixed(shape, 8))
+    u = o.get_fields()
+    T = o.pp_transfer(*u)
+    low = slice(0, min(shape) // 3 - 2)
+    scale = np.abs(T[0, low]).max()
+    assert np.abs(T[0, low] - T[1, low] - T[2, low]).max() <= 1e-13 * scale
+    rates = []
+    for dt in (2e-5, 1e-5):
+        a = orc.solver(make_params(shape[0], shape[1], shape[2] if nd == 3 else 0, reynolds=0.0, deltat=dt, kmax=128))
+        a.set_fields(*u)
+        _, _, s2 = a.step(2)
+        rates.append((s2[1, 1:, :].sum(0) - s2[0, 1:, :].sum(0)) / dt)  # E(k) carries the 1/2
+    dEdt = 2 * rates[1] - rates[0]  # Richardson: first-order error removed
+    assert np.abs(dEdt[low] - T[0, low]).max() <= 1e-8 * scale
+    # each unextrapolated rate is off by O(dt): the check above is not vacuous
+    assert np.abs(rates[0][low] - T[0, low]).max() > 1e-7 * scale
