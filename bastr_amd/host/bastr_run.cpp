@@ -12,10 +12,14 @@
 // The time loop (mainloop, solution.F90:20-170) runs on the GPU through the
 // C-ABI of include/bastr.h; this program only reads, writes and schedules.
 //
-// Extra command (the role of Bastrpp's hitgen2d/3d, Bastrpp.F90:601-808, with
-// a reproducible seed):  bastr_run hitgen [seed [k0 [energy]]]  writes
-// datin/flowini{2,3}d.h5 for the grid of datin/input.dat; k0 and energy
-// default to input.dat's forcek and targetenergy, as in Bastrpp.
+// Post-processing commands of the reference's Bastrpp program (Bastrpp.F90):
+//   bastr_run hitgen [seed [k0 [energy]]]   hitgen2d/3d (:601-808) with a
+//       reproducible seed: datin/flowini{2,3}d.h5 for the grid of
+//       datin/input.dat; k0 and energy default to forcek and targetenergy
+//   bastr_run spectra2d NNNN    spectra2d (:307-416): pp/SpectraNNNN.dat and
+//       pp/Spectra_auxNNNN.dat from outdat/flowfieldNNNN.h5
+//   bastr_run transfer2d NNNN   transfer2d (:174-305): pp/TransferNNNN.dat
+//   bastr_run transfer3d NNNN   transfer3d (:418-599): pp/TransferNNNN.dat
 #include <dlfcn.h>
 #include <hdf5.h>  // types and prototypes only: the library is opened with dlopen below
 #include <sys/stat.h>
@@ -422,6 +426,73 @@ int hitgen(const Input& in, double k0, uint64_t seed, double target) {
     return 0;
 }
 
+// spectra2d / transfer2d / transfer3d of one dump (Bastrpp.F90:174-599): read
+// outdat/flowfieldNNNN.h5, evaluate on the device, write pp/*.dat with the
+// dump's nstep and time in the leading columns, as listwrite does.
+int postprocess(const Input& in, const std::string& cmd, int filenumb) {
+    const int nd = in.ka == 0 ? 2 : 3;
+    if ((cmd == "transfer3d") != (nd == 3))
+        stop_msg(cmd + " needs a " + (nd == 3 ? "2-D" : "3-D") + " case in datin/input.dat", 1);
+    const int nz = nd == 3 ? in.ka : 1;
+    const size_t n = (size_t)in.ia * in.ja * nz;
+    std::vector<double> u[3];
+    const char* names[3] = {"u1", "u2", "u3"};
+    const std::string fn = stepname(filenumb);
+    hid_t f = H5.fopen(fn.c_str(), kAccRdonly, H5P_DEFAULT);
+    if (f < 0) stop_msg(" !! error in h5io_init call h5fopen_f: " + fn, 2);
+    for (int c = 0; c < nd; ++c) {
+        u[c].assign(n, 0.0);
+        h5read_field(f, names[c], in.ia, in.ja, nz, u[c].data());
+    }
+    int ns = 0;
+    double time = 0;
+    h5read_scalar(f, "time", H5.native_double, &time);
+    h5read_scalar(f, "nstep", H5.native_int, &ns);
+    H5.fclose(f);
+    std::printf(" >> %s ... done\n", fn.c_str());
+    bastr_params p{};
+    p.ia = in.ia; p.ja = in.ja; p.ka = in.ka;
+    p.reynolds = in.reynolds;
+    p.deltat = in.deltat;
+    p.timemethod = 1;
+    p.kmax = in.kmax;
+    const char* prec = std::getenv("BASTR_PRECISION");
+    p.precision = (prec && std::string(prec) == "fp32") ? BASTR_PREC_F32 : BASTR_PREC_F64;
+    bastr_handle* h = nullptr;
+    check(bastr_create(&p, &h), "bastr_create");
+    const int K = bastr_allkmax(h);
+    mkdir("pp", 0755);
+    char num[8];
+    std::snprintf(num, sizeof num, "%04d", filenumb);
+    const double* u3 = nd == 3 ? u[2].data() : nullptr;
+    if (cmd == "spectra2d") {
+        std::vector<double> shells(2 * (size_t)K), tot(4);
+        check(bastr_pp_spectra(h, u[0].data(), u[1].data(), nullptr, shells.data(), tot.data()), "bastr_pp_spectra");
+        std::printf("Calculation finish!\n");
+        const std::string out = std::string("pp/Spectra") + num + ".dat";
+        Log lg = listinit(out, "ns ti k Es Ed", false, 0);
+        for (int b = 0; b < K; ++b) listwrite(lg, ns, time, {(double)(b + 1), shells[b], shells[K + b]});
+        std::fclose(lg.f);
+        std::printf(" << %s ... done\n", out.c_str());
+        const std::string aux = std::string("pp/Spectra_aux") + num + ".dat";
+        Log la = listinit(aux, "ns ti Esspe Edspe k2Edspe kM1Edspe", false, 0);
+        listwrite(la, ns, time, {tot[0], tot[1], tot[2], tot[3]});
+        std::fclose(la.f);
+        std::printf(" << %s ... done\n", aux.c_str());
+    } else {
+        std::vector<double> t(3 * (size_t)K);
+        check(bastr_pp_transfer(h, u[0].data(), u[1].data(), u3, t.data()), "bastr_pp_transfer");
+        std::printf("Calculation finish!\n");
+        const std::string out = std::string("pp/Transfer") + num + ".dat";
+        Log lg = listinit(out, "ns ti k T T1 T2", false, 0);
+        for (int b = 0; b < K; ++b) listwrite(lg, ns, time, {(double)(b + 1), t[b], t[K + b], t[2 * K + b]});
+        std::fclose(lg.f);
+        std::printf(" << %s ... done\n", out.c_str());
+    }
+    bastr_destroy(h);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -445,6 +516,12 @@ int main(int argc, char** argv) {
         const double energy = argc >= 5 ? std::atof(argv[4]) : in.target_energy;
         if (!(k0 > 0) || !(energy > 0)) stop_msg("hitgen: k0 and energy must be > 0", 2);
         return hitgen(in, k0, seed, energy);
+    }
+    if (argc >= 2 && (std::string(argv[1]) == "spectra2d" || std::string(argv[1]) == "transfer2d" ||
+                      std::string(argv[1]) == "transfer3d")) {
+        std::printf(" ** pp command: %s\n", argv[1]);
+        if (argc < 3) stop_msg(std::string(argv[1]) + ": file number missing", 2);
+        return postprocess(in, argv[1], std::atoi(argv[2]));
     }
     if (in.ka == 0 && in.ja == 0) stop_msg("1D case is not implemented yet!", 1);
     if (in.timemethod != 1 && in.timemethod != 2) stop_msg("Undefined time method", 0);

@@ -180,3 +180,44 @@ def test_driver_3d_and_nan_stop(tmp_path):
     lines = open(tmp_path / "log" / "stat3d_specall.dat").read().splitlines()
     assert len(lines) == 1 + 2 and "NaN" in lines[-1]
     assert (tmp_path / "outdat" / "flowfield0001.h5").exists()  # NaN dump (solution.F90:141-153)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(64, 32), (16, 16, 16)])
+def test_driver_postprocessing_commands(tmp_path, orc, shape):
+    """spectra2d / transfer2d / transfer3d NNNN (Bastrpp.F90:174-599): read
+    outdat/flowfieldNNNN.h5, write pp/*.dat with the dump's nstep and time."""
+    print(f"case driver pp {shape}", flush=True)
+    nd = len(shape)
+    ka = shape[2] if nd == 3 else 0
+    write_input(tmp_path, shape[0], shape[1], ka, 4, "1.d-4", True, 4, False, 10, 128, 1, 0, 5, "0.075d0",
+                False, 1)
+    assert run(tmp_path, "hitgen", "5", "3", "1.0").returncode == 0
+    assert run(tmp_path).returncode == 0
+    dump = tmp_path / "outdat" / "flowfield0001.h5"
+    u = [h5read(dump, v, shape) for v in ("u1", "u2", "u3")[:nd]]
+    ns, ti = int(h5scalar(dump, "nstep", "<i4")), float(h5scalar(dump, "time", "<f8"))
+    assert ns == 4
+    o = orc.solver(make_params(shape[0], shape[1], ka, kmax=128))
+    K = min(shape[0] // 3, 128)
+    cmds = ["transfer3d"] if nd == 3 else ["spectra2d", "transfer2d"]
+    for cmd in cmds:
+        r = run(tmp_path, cmd, "1")
+        assert r.returncode == 0, r.stdout
+    _, tr = read_log(tmp_path / "pp" / "Transfer0001.dat")
+    assert len(tr) == K and all(x[0] == ns and x[1] == ti for x in tr)
+    assert [x[2] for x in tr] == list(range(1, K + 1))
+    ref = o.pp_transfer(*u)
+    got = np.array([x[3:] for x in tr]).T
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+    if nd == 2:
+        h, sp = read_log(tmp_path / "pp" / "Spectra0001.dat")
+        assert h == "".join(f" {x:>20s}" for x in "ns ti k Es Ed".split())
+        sh, tot = o.pp_spectra(*u)
+        assert np.abs(np.array([x[3:] for x in sp]).T - sh).max() <= 1e-11 * np.abs(sh).max()
+        _, aux = read_log(tmp_path / "pp" / "Spectra_aux0001.dat")
+        assert len(aux) == 1 and aux[0][0] == ns
+        assert np.allclose(aux[0][2:], tot, rtol=1e-11)
+    # a 2-D command on a 3-D case (and the reverse) stops with a message
+    r = run(tmp_path, "transfer2d" if nd == 3 else "transfer3d", "1")
+    assert r.returncode != 0 and "needs a" in r.stdout
