@@ -15,7 +15,7 @@ program bastr_demo
   type(bastr_params) :: p
   type(c_ptr) :: h
   type(bastr_diag), allocatable :: diags(:)
-  real(c_double), allocatable :: spec(:), u1(:), u2(:), u3(:)
+  real(c_double), allocatable :: spec(:), u1(:), u2(:), u3(:), tr(:)
   integer(c_int) :: rc
   integer :: ia, ja, ka, n, s, allkmax
   integer(c_int64_t) :: seed, nstep
@@ -56,5 +56,17 @@ program bastr_demo
     if (bastr_get_fields(h, u1, u2, u3) /= BASTR_OK) error stop bastr_error()
     write(*, '(A,1X,I20,2(1X,E20.13E2))') ' final  ', nstep, time, sum(u1**2 + u2**2 + u3**2) / (ia * ja * ka)
   end if
+  ! transfer2d/3d of the final field, one line per shell as in pp/TransferNNNN.dat
+  allocate(tr(3 * allkmax))
+  if (ka == 0) then
+    rc = bastr_pp_transfer(h, u1, u2, out=tr)
+  else
+    rc = bastr_pp_transfer(h, u1, u2, u3, out=tr)
+  end if
+  if (rc /= BASTR_OK) error stop bastr_error()
+  do s = 1, allkmax
+    write(*, '(A,1X,I20,4(1X,E20.13E2))') ' transf ', nstep, real(s, c_double), tr(s), tr(allkmax + s), &
+        tr(2 * allkmax + s)
+  end do
   if (bastr_destroy(h) /= BASTR_OK) error stop bastr_error()
 end program bastr_demo

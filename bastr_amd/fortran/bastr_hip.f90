@@ -14,6 +14,7 @@ module bastr_hip
   public :: bastr_create, bastr_destroy, bastr_allkmax, bastr_set_fields, bastr_get_fields
   public :: bastr_init_random_phase, bastr_set_time, bastr_get_time, bastr_step
   public :: bastr_shell_counts, bastr_fft_forward, bastr_fft_backward
+  public :: bastr_pp_spectra, bastr_pp_transfer
 
   integer(c_int), parameter, public :: BASTR_OK = 0, BASTR_STOP_NAN = 1
   integer(c_int), parameter, public :: BASTR_EINVAL = -1, BASTR_EDEVICE = -2, BASTR_ENOMEM = -3
@@ -107,6 +108,23 @@ module bastr_hip
       import
       type(c_ptr), value :: h
       complex(c_double_complex), intent(inout) :: a(*)
+    end function
+    ! Bastrpp spectra2d (2-D): shells = Es(1:allkmax), Ed(1:allkmax);
+    ! totals = Esspe, Edspe, k2Edspe, kM1Edspe.  Pass shells/totals by keyword.
+    integer(c_int) function bastr_pp_spectra(h, u1, u2, u3, shells, totals) bind(C, name='bastr_pp_spectra')
+      import
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u1(*), u2(*)
+      real(c_double), intent(in), optional :: u3(*)
+      real(c_double), intent(out) :: shells(*), totals(4)
+    end function
+    ! Bastrpp transfer2d/3d: out = T(1:allkmax), T1(1:allkmax), T2(1:allkmax)
+    integer(c_int) function bastr_pp_transfer(h, u1, u2, u3, out) bind(C, name='bastr_pp_transfer')
+      import
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u1(*), u2(*)
+      real(c_double), intent(in), optional :: u3(*)
+      real(c_double), intent(out) :: out(*)
     end function
   end interface
 

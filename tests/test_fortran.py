@@ -27,7 +27,7 @@ def test_fortran_binding_refuses_bad_grid():
 
 
 def parse(out):
-    rows = {"specall": [], "factor": [], "final": []}
+    rows = {"specall": [], "factor": [], "final": [], "transf": []}
     for line in out.splitlines():
         parts = line.split()
         if parts and parts[0] in rows:
@@ -60,3 +60,8 @@ def test_fortran_demo_matches_library(hip, shape, n, seed):
     fin = rows["final"][0]
     assert fin[0] == nstep and abs(fin[1] - time) <= 1e-12 * time
     assert abs(fin[2] - sum(float(np.mean(x * x)) for x in u)) <= 1e-12 * fin[2]
+    # bastr_pp_transfer through the module's optional-argument interface
+    T = s.pp_transfer(*u)
+    tr = np.array([x[2:] for x in rows["transf"]]).T
+    assert [x[1] for x in rows["transf"]] == list(range(1, s.allkmax + 1))
+    assert np.abs(tr - T).max() <= 1e-12 * np.abs(T).max()
