@@ -48,6 +48,14 @@ __device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T
     return {a.x + b.y, b.x - a.y};
 }
 
+// pack_at from the half-row values a = A[h], b = B[h], h = min(m, N - m)
+template <typename T, int N>
+__device__ __forceinline__ cx<T> pack_vals(cx<T> a, cx<T> b, int m) {
+    if (m == 0 || m == N / 2) return {a.x, b.x};
+    if (m < N / 2) return {a.x - b.y, a.y + b.x};
+    return {a.x + b.y, b.x - a.y};
+}
+
 // Store the half spectra of a forward-transformed packed row: thread j holds
 // X[j + k*P]; writes (X[h] + conj X[N-h])/2 and (X[h] - conj X[N-h])/(2i).
 template <typename T, int N, int E>
@@ -255,10 +263,30 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
 }
 
+// Block reduction of NQ moment partial sums (sums; the last one of the
+// velgrad set, eta, is a minimum) into part[(q0 + q) * gridDim.x + block].
+template <int THREADS, int NQ>
+__device__ __forceinline__ void vg_reduce(double* vals, double (*red)[8], int q0, double* part) {
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) vals[q] = (q0 + q == 7) ? wave_min(vals[q]) : wave_sum(vals[q]);
+    __syncthreads();
+    if (lane == 0)
+        for (int q = 0; q < NQ; ++q) red[wave][q] = vals[q];
+    __syncthreads();
+    if (tid < NQ) {
+        const bool mn = q0 + tid == 7;
+        double acc = red[0][tid];
+        for (int wv = 1; wv < THREADS / 64; ++wv) acc = mn ? fmin(acc, red[wv][tid]) : acc + red[wv][tid];
+        part[(size_t)(q0 + tid) * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
 // Stage A of RK3 fused with velgrad_calculate2D (solution.F90:1110-1186):
 // the derivative fields of the nonlinear term are the velgrad ones, so one
-// pass over A1 A2 B1 B2 C1 C2 D produces both the x-forward nonlinear term
-// and the 8 moments' partial sums (part[q * gridDim.x + block]).
+// pass over A1 A2 B1 B2 produces both the x-forward nonlinear term and the
+// first-derivative moments' partial sums (part[q * gridDim.x + block], all q
+// but the dissipation term 3, which vg_dis_kernel adds from C1 C2 D).
 template <typename T, int N>
 __global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
 void row_stage_vg_kernel(VgArgs<T> a) {
@@ -278,9 +306,6 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     const cx<T>* A2 = a.w[1] + (size_t)row * hp;
     const cx<T>* B1 = a.w[2] + (size_t)row * hp;
     const cx<T>* B2 = a.w[3] + (size_t)row * hp;
-    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
-    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
-    const cx<T>* D = a.w[6] + (size_t)row * hp;
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
 
@@ -300,8 +325,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < E; ++k) y[k] = pack_at<T, N>(B1, B2, j + k * P);
     line_fft<T, N, E, true, false>(y, j, Ls, a.tw);  // d2u1, d2u2
-    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
-    T dv[E];
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         n[k].x = n[k].x - z[k].y * y[k].x;
@@ -310,7 +334,6 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         const double b11 = w[k].x, b21 = w[k].y, b12 = y[k].x, b22 = y[k].y;
         const double div = b11 + b22;
         const double uu = u1 * u1 + u2 * u2;
-        dv[k] = (T)div;
         s_um2 += div * div * uu;
         s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
         s_u2t += div * uu;
@@ -319,11 +342,63 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         s_urms += u1 * u1 + u2 * u2;
         s_dudx += b11 * b11 + b22 * b22;
     }
-    // nonlinear term: x-forward and split into the two half rows (done here, so
-    // its registers are free for the Laplacian transforms below)
+    // nonlinear term: x-forward and split into the two half rows
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+
+    if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+    double v0[3] = {s_um2, s_ijji, s_u2t};
+    double v1[4] = {s_urms, s_dudx, s_eps, eta};
+    vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
+    vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
+}
+
+// The dissipation moment of velgrad_calculate2D (solution.F90:1110-1186),
+//   sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta),
+// from A1 A2 (u), B2 (d2 u2), C1 C2 (y part of Lap u), D (y part of d2 Lap u2):
+// theta and Lap(theta) are real and share one transform.
+template <typename T, int N>
+__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void vg_dis_kernel(VgArgs<T> a) {
+    using C = VgCfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ double red[C::THREADS / 64][8];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < a.g.n1;
+    const int row = valid ? row0 : a.g.n1 - 1;
+    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const size_t hp = (size_t)a.g.hp;
+    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
+    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
+    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
+    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
+    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
+    const cx<T>* D = a.w[6] + (size_t)row * hp;
+
+    cx<T> z[E], t[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(A1, A2, j + k * P);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    // (theta, Lap(theta)): theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1 = (T)h;
+        const T k1o = (h == N / 2) ? (T)0 : k1;
+        const cx<T> a1 = A1[h], b2 = B2[h];
+        const cx<T> th = cadd(cscale(cmuli(a1), k1o), b2);
+        const cx<T> lt = cadd(cadd(cscale(cmuli(a1), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
+                              cadd(cscale(b2, -(k1 * k1)), D[h]));
+        t[k] = pack_vals<T, N>(th, lt, m);
+    }
+    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);
     // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
+    cx<T> w[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
@@ -332,45 +407,19 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         w[k] = cadd(cscale(pack_at<T, N>(A1, A2, m), -k1sq), pack_at<T, N>(C1, C2, m));
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+    double s_dis = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * (double)dv[k] * (double)w[k].x +
-                 2.0 * (double)z[k].y * (double)dv[k] * (double)w[k].y;
-    // Lap(theta) = x-inverse of (-i k1^3 A1 + i k1 C1 - k1^2 B2 + D), a real field
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T k1 = (T)h;
-        const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> th = cadd(cadd(cscale(cmuli(A1[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
-                              cadd(cscale(B2[h], -(k1 * k1)), D[h]));
-        if (m == 0 || m == N / 2) w[k] = {th.x, (T)0};
-        else if (m < N / 2) w[k] = th;
-        else w[k] = cconj(th);
-    }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+        s_dis += 2.0 * (double)z[k].x * (double)t[k].x * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)t[k].x * (double)w[k].y;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
-        s_dis += uu * (double)w[k].x;
+        s_dis += uu * (double)t[k].y;
     }
-
-    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = eta0; }
-    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
-    const int wave = tid / 64, lane = tid % 64;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
-    vals[7] = wave_min(vals[7]);
-    __syncthreads();
-    if (lane == 0)
-        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
-    __syncthreads();
-    if (tid < 8) {
-        double acc = tid == 7 ? eta0 : 0.0;
-        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
-        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;
-    }
+    if (!valid) s_dis = 0;
+    double v[1] = {s_dis};
+    vg_reduce<C::THREADS, 1>(v, red, 3, a.part);
 }
 
 // shell spectra (spectra_compute2D, solution.F90:930-1016); deterministic:
@@ -625,6 +674,7 @@ static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
     if (!a.hout[0] || !a.hout[1]) return -1;
     hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
