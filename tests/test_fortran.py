@@ -65,3 +65,43 @@ def test_fortran_demo_matches_library(hip, shape, n, seed):
     tr = np.array([x[2:] for x in rows["transf"]]).T
     assert [x[1] for x in rows["transf"]] == list(range(1, s.allkmax + 1))
     assert np.abs(tr - T).max() <= 1e-12 * np.abs(T).max()
+
+
+CHECK = os.path.join(ROOT, "bastr_amd", "bin", "fftwlink_check")
+
+
+def run_check(tmp_path, *args):
+    """fftwlink_check drives the drop-in fftwlink module (bastr_amd/fortran/fftwlink.f90)
+    through the calls Bastr.F90 makes: fftw_mpi_init, mpisizedis_fftw, allocation,
+    prepareplan_fftw_plan, fft2d/3d, ifft2d/3d, deallocation (fftwlink.F90:34-562)."""
+    if not os.path.exists(CHECK):
+        pytest.fail("fftwlink check not built; run __graft_entry__.build()")
+    return subprocess.run([CHECK, *map(str, args)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+
+
+def test_fftwlink_dropin_refuses_bad_grid(tmp_path):
+    r = run_check(tmp_path, 100, 64, 0)
+    # a Fortran `stop 'message'`, as the reference stops on an unsupported setup
+    assert "ia, ja must be powers of two" in r.stdout and "grid not supported" in r.stdout + r.stderr
+    assert "sizes" not in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(64, 32, 0), (16, 16, 16)])
+def test_fftwlink_dropin_matches_numpy(tmp_path, shape):
+    print(f"case fftwlink drop-in {shape}", flush=True)
+    ia, ja, ka = shape
+    r = run_check(tmp_path, ia, ja, ka)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = {l.split()[0]: l.split()[1:] for l in r.stdout.splitlines() if l.strip()}
+    nz = max(ka, 1)
+    assert [int(x) for x in lines["sizes"]] == [ia, ja, nz, ia * ja * nz]
+    assert float(lines["roundtrip"][0]) < 1e-13
+    assert [float(x) for x in lines["reductions"]] == [1.5, 2.5]
+    assert "** The job is done!" in r.stdout  # deallocation ends through mpistop
+    i, j, k = np.meshgrid(np.arange(1, ia + 1), np.arange(1, ja + 1), np.arange(1, nz + 1), indexing="ij")
+    u = (np.cos(0.3 * i + 0.7 * j + 1.1 * k) + 0.01 * i) + 1j * (np.sin(0.05 * i * j) - 0.02 * k)
+    axes = (0, 1) if ka == 0 else (0, 1, 2)
+    ref = np.fft.fftn(u, axes=axes) / (ia * ja * (ka if ka else 1))  # fft2d/3d: forward / N
+    got = np.fromfile(tmp_path / "fftwlink_fwd.bin", dtype=np.complex128).reshape((ia, ja, nz), order="F")
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
