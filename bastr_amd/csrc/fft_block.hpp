@@ -106,8 +106,24 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 
 // Per-line LDS view.  SPLIT: real and imaginary parts exchanged in two rounds
 // through one T buffer of the line's length; otherwise one cx<T> buffer.
-template <typename T, bool SPLIT>
+// WAVE: every lane of the line is in one wavefront (lines of at most 64
+// threads, not straddling a wave), so exchanges need no workgroup barrier.
+template <typename T, bool SPLIT, bool WAVE = false>
 struct LdsLine;
+
+// Ordering of the LDS accesses of one exchange: a workgroup barrier, or for
+// wave-private lines only a compiler barrier with wavefront-scope fences (a
+// wavefront's LDS operations complete in issue order).
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
 
 // Element e of a line lives at padded slot e + e/16: one spare slot every 16
 // breaks the power-of-two strides of the Stockham write pattern, which
@@ -116,16 +132,16 @@ __device__ __forceinline__ int lds_slot(int e) { return e + (e >> 4); }
 // LDS elements needed per line of length n
 constexpr int lds_line(int n) { return n + n / 16; }
 
-template <typename T>
-struct LdsLine<T, false> {
+template <typename T, bool WAVE>
+struct LdsLine<T, false, WAVE> {
     cx<T>* base;
     int stride;  // element e at base[lds_slot(e)*stride]
     __device__ __forceinline__ void st(int e, cx<T> v) const { base[lds_slot(e) * stride] = v; }
     __device__ __forceinline__ cx<T> ld(int e) const { return base[lds_slot(e) * stride]; }
 };
 
-template <typename T>
-struct LdsLine<T, true> {
+template <typename T, bool WAVE>
+struct LdsLine<T, true, WAVE> {
     T* base;
     int stride;
     __device__ __forceinline__ T& at(int e) const { return base[lds_slot(e) * stride]; }
@@ -133,28 +149,28 @@ struct LdsLine<T, true> {
 
 // Exchange: values v[r] written at positions widx(r), then read back so that
 // out[q] = line[ridx(q)].  All threads of the workgroup must call it.
-template <typename T, int NV, class W, class Rd>
-__device__ __forceinline__ void exchange(const LdsLine<T, false>& L, cx<T>* v, W widx, Rd ridx) {
-    __syncthreads();
+template <typename T, int NV, bool WAVE, class W, class Rd>
+__device__ __forceinline__ void exchange(const LdsLine<T, false, WAVE>& L, cx<T>* v, W widx, Rd ridx) {
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) L.st(widx(r), v[r]);
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) v[r] = L.ld(ridx(r));
 }
 
-template <typename T, int NV, class W, class Rd>
-__device__ __forceinline__ void exchange(const LdsLine<T, true>& L, cx<T>* v, W widx, Rd ridx) {
-    __syncthreads();
+template <typename T, int NV, bool WAVE, class W, class Rd>
+__device__ __forceinline__ void exchange(const LdsLine<T, true, WAVE>& L, cx<T>* v, W widx, Rd ridx) {
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].x;
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) v[r].x = L.at(ridx(r));
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].y;
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int r = 0; r < NV; ++r) v[r].y = L.at(ridx(r));
 }
@@ -195,8 +211,8 @@ struct FftShape {
 
 // Full line transform.  On entry v[k] = x[j + k*P]; on exit v[k] = X[j + k*P].
 // Unnormalised in both directions.
-template <typename T, int N, int E, bool INV, bool SPLIT>
-__device__ __forceinline__ void line_fft(cx<T>* v, int j, const LdsLine<T, SPLIT>& L,
+template <typename T, int N, int E, bool INV, bool SPLIT, bool WAVE>
+__device__ __forceinline__ void line_fft(cx<T>* v, int j, const LdsLine<T, SPLIT, WAVE>& L,
                                          const cx<T>* __restrict__ tw) {
     using S = FftShape<N, E>;
     constexpr int P = S::P;

@@ -58,14 +58,14 @@ __device__ __forceinline__ cx<T> pack_vals(cx<T> a, cx<T> b, int m) {
 
 // Store the half spectra of a forward-transformed packed row: thread j holds
 // X[j + k*P]; writes (X[h] + conj X[N-h])/2 and (X[h] - conj X[N-h])/(2i).
-template <typename T, int N, int E>
-__device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, false>& L, cx<T>* __restrict__ o1,
+template <typename T, int N, int E, bool WAVE>
+__device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, false, WAVE>& L, cx<T>* __restrict__ o1,
                                              cx<T>* __restrict__ o2, bool valid) {
     constexpr int P = N / E;
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int k = 0; k < E; ++k) L.st(j + k * P, v[k]);
-    __syncthreads();
+    lds_sync<WAVE>();
     const T half = (T)0.5;
 #pragma unroll
     for (int k = 0; k < E / 2; ++k) {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
     const size_t hp = (size_t)a.g.hp;
 
     if (MODE == R_C2C_FWD || MODE == R_C2C_INV) {

@@ -256,7 +256,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
     const size_t hp = (size_t)a.g.hp;
     const size_t ro = (size_t)row * hp;
 

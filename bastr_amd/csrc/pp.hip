@@ -28,7 +28,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row2_kernel(RowArgs<T
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
     const size_t ro = (size_t)row * a.g.hp;
     const cx<T>* A1 = a.hin[0] + ro;
     const cx<T>* A2 = a.hin[1] + ro;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(RowArgs<T
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
     const size_t ro = (size_t)row * a.g.hp;
     const cx<T>* Y[9];
 #pragma unroll
