@@ -117,7 +117,7 @@ static int lvg_of(int n) {
     return -1;
 }
 int vg_block_rows(int n, int prec, int nd) {
-    if (nd == 3) return prec == BASTR_PREC_F32 ? lvg_of<float, 4>(n) : lvg_of<double, 4>(n);
+    if (nd == 3) return prec == BASTR_PREC_F32 ? lvg_of<float, 8>(n) : lvg_of<double, 8>(n);
     return prec == BASTR_PREC_F32 ? lvg_of<float, 8>(n) : lvg_of<double, 8>(n);
 }
 

@@ -22,6 +22,7 @@ namespace k3d {
 
 using k2d::ksigned;
 using k2d::pack_at;
+using k2d::pack_vals;
 using k2d::unpack_store;
 using k2d::xcd_block;
 
@@ -365,10 +366,15 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
 }
 
 template <typename T, int N>
-using Vg3Cfg = k2d::VgCfg<T, N, 4>;
+using Vg3Cfg = k2d::VgCfg<T, N, 8>;
 
-// velocity-gradient moments, 3-D (velgrad_calculate3D, solution.F90:1188-1286)
-
+// velocity-gradient moments, 3-D (velgrad_calculate3D, solution.F90:1188-1286).
+// The nine gradient components are transformed in pairs chosen so that each
+// pair's products enter the moments at once: (b12, b21), (b13, b31),
+// (b23, b32) for the symmetric terms, then (b22, b33) completes theta with
+// b11 from the (u3, b11) transform.  Only u, b11 and later theta are held
+// across transforms, so 8 points per thread fit in registers and a 512-point
+// line is one wavefront.
 template <typename T, int N>
 __global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
 void vg3_kernel(Vg3Args<T> a) {
@@ -383,83 +389,122 @@ void vg3_kernel(Vg3Args<T> a) {
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
-    const size_t ro = (size_t)row * a.g.hp;
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    // a line of at least 64 threads is wave-uniform: its row offset lives in SGPRs
+    const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
     const cx<T>* W[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
     // W: 0..2 Y_c, 3..5 Y2_c (d2), 6..8 Y3_c (d3), 9..11 L_c, 12 D
-    // The gradient products enter the moments as soon as both factors are
-    // known (|u|^2 is known after the first two transforms), so at most three
-    // gradient components are held per point at any time.
+    // half-row value of d1 X at h (odd power of k1: zero at the Nyquist index)
+    auto d1 = [](const cx<T>* X, int h) {
+        const T ko = (h == N / 2) ? (T)0 : (T)h;
+        return cscale(cmuli(X[h]), ko);
+    };
     cx<T> z[E], w[E];
-    T u3[E], b31[E], b11[E], b21[E], b22[E], b23[E];
-    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_dis = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
+    T u3[E], b11[E];
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
     for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(W[2], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u3
-#pragma unroll
-    for (int k = 0; k < E; ++k) { u3[k] = w[k].x; b31[k] = w[k].y; }
-    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
-        const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
-        w[k] = cscale(cmuli(pack_at<T, N>(W[0], W[1], m)), kxs);
+        const int h = m <= N / 2 ? m : N - m;
+        w[k] = pack_vals<T, N>(W[2][h], d1(W[0], h), m);
     }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d1 u1, d1 u2
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u1
 #pragma unroll
-    for (int k = 0; k < E; ++k) { b11[k] = w[k].x; b21[k] = w[k].y; }
-    __builtin_amdgcn_sched_barrier(0);
+    for (int k = 0; k < E; ++k) { u3[k] = w[k].x; b11[k] = w[k].y; }
+    // the symmetric pairs b_ij, b_ji: i j = 1 2, 1 3, 2 3
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[3], W[4], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u1, d2 u2
+    for (int pr = 0; pr < 3; ++pr) {
+        __builtin_amdgcn_sched_barrier(0);
+        const cx<T>* X = pr == 0 ? W[3] : (pr == 1 ? W[6] : W[7]);  // d2 u1, d3 u1, d3 u2
+        const cx<T>* Y = pr == 0 ? W[1] : (pr == 1 ? W[2] : W[5]);  // (d1 of) u2, u3; d2 u3
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y + (double)u3[k] * u3[k];
-        const double b12 = w[k].x;
-        b22[k] = w[k].y;
-        s_ijji += 2.0 * b12 * (double)b21[k] * uu;
-        s_eps += nu * ((b12 - (double)b21[k]) * (b12 - (double)b21[k]));
-    }
-    __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < E; ++k) {
+            const int m = j + k * P;
+            const int h = m <= N / 2 ? m : N - m;
+            w[k] = pack_vals<T, N>(X[h], pr < 2 ? d1(Y, h) : Y[h], m);
+        }
+        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[6], W[7], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d3 u1, d3 u2
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y + (double)u3[k] * u3[k];
-        const double b13 = w[k].x;
-        b23[k] = w[k].y;
-        s_ijji += 2.0 * b13 * (double)b31[k] * uu;
-        s_eps += nu * ((b13 - (double)b31[k]) * (b13 - (double)b31[k]));
+        for (int k = 0; k < E; ++k) {
+            const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y + (double)u3[k] * u3[k];
+            const double bij = w[k].x, bji = w[k].y;
+            s_ijji += 2.0 * bij * bji * uu;
+            s_eps += nu * ((bij - bji) * (bij - bji));
+        }
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[5], W[8], j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u3, d3 u3
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[4], W[8], j + k * P);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u2, d3 u3
     double dv[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
-        const double b11k = b11[k], b22k = b22[k], b33 = w[k].y, b23k = b23[k], b32 = w[k].x;
+        const double b11k = b11[k], b22 = w[k].x, b33 = w[k].y;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-        const double div = b11k + b22k + b33;
+        const double div = b11k + b22 + b33;
         dv[k] = div;
         s_um2 += div * div * uu;
-        s_ijji += (b11k * b11k + b22k * b22k + b33 * b33 + 2.0 * b23k * b32) * uu;
+        s_ijji += (b11k * b11k + b22 * b22 + b33 * b33) * uu;
         s_u2t += div * uu;
-        s_eps += nu * (b23k - b32) * (b23k - b32) + nu * div * div;
+        s_eps += nu * div * div;
         eta = fmin(eta, sqrt(nu / fabs(div)));
         s_urms += u1 * u1 + u2 * u2 + uu3 * uu3;
-        s_dudx += b11k * b11k + b22k * b22k + b33 * b33;
+        s_dudx += b11k * b11k + b22 * b22 + b33 * b33;
     }
+    if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+    double v0[3] = {s_um2, s_ijji, s_u2t};
+    double v1[4] = {s_urms, s_dudx, s_eps, eta};
+    k2d::vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
+    k2d::vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
+}
+
+// The dissipation moment of velgrad_calculate3D (solution.F90:1188-1286),
+//   sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta),
+// in its own pass (part[3 * gridDim.x + block]): u, (u3, theta),
+// (Lap u1, Lap u2), (Lap u3, Lap theta).
+template <typename T, int N>
+__global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void vg3_dis_kernel(Vg3Args<T> a) {
+    using C = Vg3Cfg<T, N>;
+    constexpr int E = C::E, P = C::P, LR = C::L;
+    if (a.gate.closed()) return;
+    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ double red[C::THREADS / 64][8];
+    const int tid = threadIdx.x;
+    const int l = tid / P, j = tid % P;
+    const int nrows = a.g.n1 * a.g.n2;
+    const int row0 = blockIdx.x * LR + l;
+    const bool valid = row0 < nrows;
+    const int row = valid ? row0 : nrows - 1;
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
+    const cx<T>* W[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
+    cx<T> z[E], t[E], w[E];
+    double s_dis = 0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    __builtin_amdgcn_sched_barrier(0);
+    // (u3, theta), theta = i k1 Y_1 + Y2_2 + Y3_3
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1o = (h == N / 2) ? (T)0 : (T)h;
+        const cx<T> th = cadd(cscale(cmuli(W[0][h]), k1o), cadd(W[4][h], W[8][h]));
+        t[k] = pack_vals<T, N>(W[2][h], th, m);
+    }
+    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);
     // Lap(u1), Lap(u2) = x-inverse of (-k1^2 Y_c + L_c)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -472,7 +517,8 @@ void vg3_kernel(Vg3Args<T> a) {
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * dv[k] * (double)w[k].x + 2.0 * (double)z[k].y * dv[k] * (double)w[k].y;
+        s_dis += 2.0 * (double)z[k].x * (double)t[k].y * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)t[k].y * (double)w[k].y;
     // (Lap(u3), Lap(theta)): Lap(u3) from (-k1^2 Y_3 + L_3); Lap(theta) from
     // -k1^2 (i k1 Y_1 + Y2_2 + Y3_3) + i k1 L_1 + D
     __builtin_amdgcn_sched_barrier(0);
@@ -487,32 +533,18 @@ void vg3_kernel(Vg3Args<T> a) {
         const cx<T> lu3 = cadd(cscale(W[2][h], -(k1 * k1)), W[11][h]);
         const cx<T> th = cadd(cscale(cmuli(W[0][h]), k1o), cadd(W[4][h], W[8][h]));
         const cx<T> lth = cadd(cadd(cscale(th, -(k1 * k1)), cscale(cmuli(W[9][h]), k1o)), W[12][h]);
-        if (m == 0 || m == N / 2) w[k] = {lu3.x, lth.x};
-        else if (m < N / 2) w[k] = {lu3.x - lth.y, lu3.y + lth.x};
-        else w[k] = {lu3.x + lth.y, lth.x - lu3.y};
+        w[k] = pack_vals<T, N>(lu3, lth, m);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
+        const double u1 = z[k].x, u2 = z[k].y, uu3 = t[k].x;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-        s_dis += 2.0 * uu3 * dv[k] * (double)w[k].x + uu * (double)w[k].y;
+        s_dis += 2.0 * uu3 * (double)t[k].y * (double)w[k].x + uu * (double)w[k].y;
     }
-    if (!valid) { s_um2 = s_ijji = s_u2t = s_dis = s_urms = s_dudx = s_eps = 0; eta = eta0; }
-    double vals[8] = {s_um2, s_ijji, s_u2t, s_dis, s_urms, s_dudx, s_eps, eta};
-    const int wave = tid / 64, lane = tid % 64;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) vals[q] = wave_sum(vals[q]);
-    vals[7] = wave_min(vals[7]);
-    __syncthreads();
-    if (lane == 0)
-        for (int q = 0; q < 8; ++q) red[wave][q] = vals[q];
-    __syncthreads();
-    if (tid < 8) {
-        double acc = tid == 7 ? eta0 : 0.0;
-        for (int wv = 0; wv < C::THREADS / 64; ++wv) acc = tid == 7 ? fmin(acc, red[wv][7]) : acc + red[wv][tid];
-        a.part[(size_t)tid * gridDim.x + blockIdx.x] = acc;
-    }
+    if (!valid) s_dis = 0;
+    double v[1] = {s_dis};
+    k2d::vg_reduce<C::THREADS, 1>(v, red, 3, a.part);
 }
 
 // shell spectra, 3-D (spectra_compute3D :1018-1108): u_d = (u.k)/|k|,
@@ -765,6 +797,7 @@ static int vg3_n(const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
     const int g = (nrows + C::L - 1) / C::L;
     if (g != nblk) return -1;
     hipLaunchKernelGGL((vg3_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    hipLaunchKernelGGL((vg3_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
