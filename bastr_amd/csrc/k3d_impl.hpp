@@ -259,7 +259,8 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     const int row = valid ? row0 : nrows - 1;
     const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
     const size_t hp = (size_t)a.g.hp;
-    const size_t ro = (size_t)row * hp;
+    // a line of at least 64 threads is wave-uniform: its row offset lives in SGPRs
+    const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * hp;
 
     if (MODE == R3_FWD_PACK) {
         cx<T> v[E];
@@ -304,23 +305,45 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         }
         return;
     }
-    // R3_STAGE: u, grad u from nine half rows, nonlinear term, x-forward
+    // R3_STAGE: u, grad u from nine half rows, nonlinear term, x-forward.
+    // n3 first (its gradients (b31, b32, b33) come from the (u3, d1 u3) and
+    // (d2 u3, d3 u3) transforms), transformed and stored before n1, n2 are
+    // accumulated, so u, one partial term and one transform are live at a time.
     const cx<T>* Y1 = a.hin[0] + ro;
     const cx<T>* Y2 = a.hin[1] + ro;
     const cx<T>* Y3 = a.hin[2] + ro;
-    cx<T> z[E], w[E], n12[E];
-    T n3[E], u3[E];
+    cx<T> z[E], w[E];
+    T u3[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(Y1, Y2, j + k * P);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
+    {
+        T n3[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
+        for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
+        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        u3[k] = w[k].x;
-        n3[k] = -(z[k].x * w[k].y);
+        for (int k = 0; k < E; ++k) {
+            u3[k] = w[k].x;
+            n3[k] = -(z[k].x * w[k].y);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P);
+        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u3, d3 u3)
+#pragma unroll
+        for (int k = 0; k < E; ++k) w[k] = {n3[k] - z[k].y * w[k].x - u3[k] * w[k].y, (T)0};
     }
+    line_fft<T, N, E, false, false>(w, j, Ls, a.tw);
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int h = j + k * P;
+            if (h <= N / 2) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{w[k].x, (T)0} : w[k];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    cx<T> n12[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
@@ -330,6 +353,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[3] + ro, a.hin[4] + ro, j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u1, d2 u2)
@@ -338,6 +362,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         n12[k].x = n12[k].x - z[k].y * w[k].x;
         n12[k].y = n12[k].y - z[k].y * w[k].y;
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[6] + ro, a.hin[7] + ro, j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d3 u1, d3 u2)
@@ -346,23 +371,8 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         n12[k].x = n12[k].x - u3[k] * w[k].x;
         n12[k].y = n12[k].y - u3[k] * w[k].y;
     }
-#pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u3, d3 u3)
-#pragma unroll
-    for (int k = 0; k < E; ++k) n3[k] = n3[k] - z[k].y * w[k].x - u3[k] * w[k].y;
     line_fft<T, N, E, false, false>(n12, j, Ls, a.tw);
     unpack_store<T, N, E>(n12, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid);
-#pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = {n3[k], (T)0};
-    line_fft<T, N, E, false, false>(w, j, Ls, a.tw);
-    if (valid) {
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int h = j + k * P;
-            if (h <= N / 2) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{w[k].x, (T)0} : w[k];
-        }
-    }
 }
 
 template <typename T, int N>
