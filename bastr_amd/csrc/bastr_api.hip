@@ -68,6 +68,12 @@ __global__ void convert_kernel(const A* __restrict__ in, B* __restrict__ out, si
         out[i] = (B)in[i];
 }
 
+// 16-byte copy for bastr_prof_copy_gbps
+__global__ void copy16_kernel(const double2* __restrict__ in, double2* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
 void* dalloc(size_t bytes, bool* ok) {
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) { *ok = false; return nullptr; }
@@ -1328,6 +1334,39 @@ int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* tot
     if (total_ms) *total_ms = h->prof_ms[c];
     if (bytes) *bytes = units[c] * half;
     return BASTR_OK;
+}
+
+int bastr_prof_copy_gbps(bastr_handle* h, uint64_t bytes, int reps, double* gbps) {
+    if (!h || !gbps || reps < 1 || bytes < 16 || bytes % 16) return fail(BASTR_EINVAL, "bastr_prof_copy_gbps: bad argument");
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        return fail(BASTR_ENOMEM, "bastr_prof_copy_gbps: cannot allocate 2 x %llu bytes", (unsigned long long)bytes);
+    }
+    const size_t n = bytes / 16;
+    int rc = BASTR_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMemsetAsync(a, 0, bytes, h->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(BASTR_EDEVICE, "bastr_prof_copy_gbps: setup failed");
+    } else {
+        for (int r = 0; r < reps + 2; ++r) {
+            if (r == 2) (void)hipEventRecord(e0, h->stream);
+            hipLaunchKernelGGL(copy16_kernel, dim3(4096), dim3(256), 0, h->stream, (const double2*)a, (double2*)b, n);
+        }
+        (void)hipEventRecord(e1, h->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0)
+            rc = fail(BASTR_EDEVICE, "bastr_prof_copy_gbps: timing failed");
+        else
+            *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
 }
 
 const char* bastr_prof_name(int c) {

@@ -125,19 +125,22 @@ __device__ __forceinline__ void lds_sync() {
     }
 }
 
-// Element e of a line lives at padded slot e + e/16: one spare slot every 16
-// breaks the power-of-two strides of the Stockham write pattern, which
+// Element e of a line lives at padded slot e + e/16 (split lines, 4- or
+// 8-byte elements) or e + e/8 (complex 8- or 16-byte elements): the spare
+// slots break the power-of-two strides of the Stockham write pattern, which
 // otherwise land up to 8 lanes of a pass on the same LDS banks.
 __device__ __forceinline__ int lds_slot(int e) { return e + (e >> 4); }
+__device__ __forceinline__ int lds_slot_cx(int e) { return e + (e >> 3); }
 // LDS elements needed per line of length n
 constexpr int lds_line(int n) { return n + n / 16; }
+constexpr int lds_line_cx(int n) { return n + n / 8; }
 
 template <typename T, bool WAVE>
 struct LdsLine<T, false, WAVE> {
     cx<T>* base;
-    int stride;  // element e at base[lds_slot(e)*stride]
-    __device__ __forceinline__ void st(int e, cx<T> v) const { base[lds_slot(e) * stride] = v; }
-    __device__ __forceinline__ cx<T> ld(int e) const { return base[lds_slot(e) * stride]; }
+    int stride;  // element e at base[lds_slot_cx(e)*stride]
+    __device__ __forceinline__ void st(int e, cx<T> v) const { base[lds_slot_cx(e) * stride] = v; }
+    __device__ __forceinline__ cx<T> ld(int e) const { return base[lds_slot_cx(e) * stride]; }
 };
 
 template <typename T, bool WAVE>

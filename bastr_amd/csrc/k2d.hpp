@@ -47,7 +47,7 @@ struct RowCfg {
     static constexpr int E = 8;
     static constexpr int P = N / E;
     static constexpr int L0 = P >= 256 ? 1 : 256 / P;
-    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/16 padding)
+    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
 };
@@ -60,7 +60,7 @@ struct VgCfg {
     static constexpr int E = EE;
     static constexpr int P = N / E;
     static constexpr int L0 = P >= 256 ? 1 : 256 / P;
-    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/16 padding)
+    static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
 };

@@ -187,13 +187,13 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
 
     if (MODE == R_C2C_FWD || MODE == R_C2C_INV) {
@@ -293,14 +293,14 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
     const cx<T>* A1 = a.w[0] + (size_t)row * hp;
     const cx<T>* A2 = a.w[1] + (size_t)row * hp;
@@ -363,14 +363,14 @@ void vg_dis_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
     const cx<T>* A1 = a.w[0] + (size_t)row * hp;
     const cx<T>* A2 = a.w[1] + (size_t)row * hp;

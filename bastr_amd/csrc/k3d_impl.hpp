@@ -250,14 +250,14 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int nrows = a.g.n1 * a.g.n2;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
     // a line of at least 64 threads is wave-uniform: its row offset lives in SGPRs
     const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * hp;
@@ -391,7 +391,7 @@ void vg3_kernel(Vg3Args<T> a) {
     using C = Vg3Cfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
@@ -399,7 +399,7 @@ void vg3_kernel(Vg3Args<T> a) {
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     // a line of at least 64 threads is wave-uniform: its row offset lives in SGPRs
     const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
     const cx<T>* W[13];
@@ -486,7 +486,7 @@ void vg3_dis_kernel(Vg3Args<T> a) {
     using C = Vg3Cfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
@@ -494,7 +494,7 @@ void vg3_dis_kernel(Vg3Args<T> a) {
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
     const cx<T>* W[13];
 #pragma unroll

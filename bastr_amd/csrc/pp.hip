@@ -21,14 +21,14 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row2_kernel(RowArgs<T
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int nrows = a.g.n1;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t ro = (size_t)row * a.g.hp;
     const cx<T>* A1 = a.hin[0] + ro;
     const cx<T>* A2 = a.hin[1] + ro;
@@ -82,14 +82,14 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(RowArgs<T
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line(N)];
+    __shared__ cx<T> lds[LR * lds_line_cx(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int nrows = a.g.n1 * a.n2;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < nrows;
     const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line(N), 1};
+    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t ro = (size_t)row * a.g.hp;
     const cx<T>* Y[9];
 #pragma unroll

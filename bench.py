@@ -57,6 +57,21 @@ def prof_api(lib):
     lib.bastr_prof_read.restype = C.c_int
     lib.bastr_prof_name.argtypes = [C.c_int]
     lib.bastr_prof_name.restype = C.c_char_p
+    lib.bastr_prof_copy_gbps.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+    lib.bastr_prof_copy_gbps.restype = C.c_int
+
+
+def cpu_model():
+    """Host CPU model name and logical CPU count, as /proc/cpuinfo states them."""
+    name, count = None, 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                count += 1
+                name = name or line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name, count or os.cpu_count()
 
 
 def read_prof(lib, h, ncls=10):
@@ -208,6 +223,11 @@ def main():
     _, n_l, ms_l, by_l = prof[dom_name]
     mean_s = ms_l / n_l * 1e-3
     achieved = by_l / mean_s / 1e9
+
+    # measured device copy bandwidth (1 GiB, read + write), beside the nominal peak
+    copy_gbps = C.c_double()
+    if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(copy_gbps)) != 0:
+        copy_gbps.value = float("nan")
     s.close()
 
     npts = ia * ja * max(ka, 1)
@@ -218,10 +238,12 @@ def main():
     b_step = T * 2 * (4 if prec == 1 else 8) * npts
     step_rd = {"reference_transforms": T, "bytes_per_step": b_step,
                "achieved": round(b_step * steps / wall / 1e9, 1), "unit": "GB/s",
-               "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4)}
+               "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+               "frac_of_copy": round(b_step * steps / wall / 1e9 / copy_gbps.value, 4)}
     rd = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, grid),
-          "kernel": dom_name, "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)}
+          "kernel": dom_name, "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
+          "copy_measured": round(copy_gbps.value, 1), "frac_of_copy": round(achieved / copy_gbps.value, 4)}
     # accuracy of E(k): an FP32 run against the FP64 device path over the same
     # iterations (the FP64 path itself is checked against the oracle below)
     accuracy = {}
@@ -249,7 +271,9 @@ def main():
         dev = (sp0[-1], g0.get_fields())
         g0.close()
         v, thr, dt, osp, ofields = cpu_baseline(params(0), fields, cpu_steps)
+        model, ncpu = cpu_model()
         cpu = {"value": v, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
+               "cpu_model": model, "host_logical_cpus": ncpu,
                "sample": f"{cpu_steps} loop iteration(s) of the same {dims} FP64 workload ({dt:.1f} s), "
                          f"FP64 C restatement with its own radix-2 FFT, OpenMP"}
         e_abs, e_shell = ek_err(dev[0], osp)

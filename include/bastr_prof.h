@@ -37,6 +37,11 @@ int bastr_prof_enable(bastr_handle* h, uint32_t mask);
 int bastr_prof_read(const bastr_handle* h, int kclass, int64_t* launches, double* total_ms,
                     double* bytes_per_launch);
 const char* bastr_prof_name(int kclass);
+/* Device-to-device copy bandwidth on the handle's stream: `reps` copies of
+ * `bytes` between two scratch buffers of that size (allocated and freed by
+ * the call), timed with HIP events; *gbps = 2 * bytes * reps / time / 1e9
+ * (each byte read once and written once). */
+int bastr_prof_copy_gbps(bastr_handle* h, uint64_t bytes, int reps, double* gbps);
 
 #ifdef __cplusplus
 }
