@@ -124,6 +124,43 @@ def test_step_parity_large_grids(hip, orc, n, nsteps):
     assert np.abs(sg - so).max() / np.abs(so[:, 1:, :]).max() < 1e-10
 
 
+@pytest.mark.parametrize("shape,prec,force,nsteps", [
+    ((4096, 64), 0, 2, 3),    # 4096-point rows (one row per workgroup, largest LDS line)
+    ((64, 4096), 0, 0, 3),    # 4096-point columns
+    ((4096, 32), 1, 0, 3),    # FP32 at the largest row length
+])
+def test_step_parity_longest_lines(hip, orc, shape, prec, force, nsteps):
+    """ia or ja at the 4096 limit of bastr_create: the longest row and
+    column transforms, with the y-Nyquist row kept by the dealias (ja < 2 ia / 3)."""
+    print(f"case step_long {shape} prec={prec} n={nsteps}", flush=True)
+    p = make_params(*shape, forcemethod=force, lproject=True, kmax=128)
+    u = mixed_field(shape, seed=5)
+    g = Solver(hip, "bastr", make_params(*shape, forcemethod=force, lproject=True, kmax=128, precision=prec))
+    o = orc.solver(p)
+    g.set_fields(*u)
+    o.set_fields(*u)
+    (rg, dg, sg), (ro, do, so) = g.step(nsteps), o.step(nsteps)
+    assert rg == 0 and ro == 0
+    err = rel_l2(g.get_fields(), o.get_fields())
+    print("rel-L2", err)
+    assert err < (1e-11 if prec == 0 else 2e-5)
+    if prec == 0:
+        check_diags(dg, do)
+        assert np.abs(sg - so).max() / np.abs(so[:, 1:, :]).max() < 1e-10
+
+
+@pytest.mark.parametrize("shape", [(4096, 32), (32, 4096)])
+def test_fft_compat_longest_lines(hip, shape):
+    print(f"case fft_compat {shape}", flush=True)
+    rng = np.random.default_rng(3)
+    a = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    s = Solver(hip, "bastr", make_params(*shape))
+    f = s.fft_forward(a)
+    assert np.abs(f - np.fft.fft2(a) / a.size).max() <= 1e-13 * np.abs(f).max()
+    b = s.fft_backward(f)
+    assert np.abs(b - a).max() <= 1e-12 * np.abs(a).max()
+
+
 @pytest.mark.parametrize("shape,timemethod,force,proj,nsteps", [
     ((128, 128), 2, 0, False, 6),   # CN2D, free decay
     ((64, 128), 2, 2, True, 5),     # CN2D + band forcing + projection
