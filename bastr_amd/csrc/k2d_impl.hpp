@@ -379,34 +379,30 @@ void vg_dis_kernel(VgArgs<T> a) {
     const cx<T>* C2 = a.w[5] + (size_t)row * hp;
     const cx<T>* D = a.w[6] + (size_t)row * hp;
 
-    cx<T> z[E], t[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(A1, A2, j + k * P);
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    // (theta, Lap(theta)): theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
+    // one load phase: each input half row is read once and feeds all three
+    // packed transforms (u1, u2), (theta, Lap theta), (Lap u1, Lap u2)
+    cx<T> z[E], t[E], w[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
+        __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> a1 = A1[h], b2 = B2[h];
+        const cx<T> a1 = A1[h], a2 = A2[h], b2 = B2[h], c1 = C1[h], c2 = C2[h], d = D[h];
+        z[k] = pack_vals<T, N>(a1, a2, m);
+        // theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
         const cx<T> th = cadd(cscale(cmuli(a1), k1o), b2);
-        const cx<T> lt = cadd(cadd(cscale(cmuli(a1), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
-                              cadd(cscale(b2, -(k1 * k1)), D[h]));
+        const cx<T> lt = cadd(cadd(cscale(cmuli(a1), -(k1o * k1o * k1o)), cscale(cmuli(c1), k1o)),
+                              cadd(cscale(b2, -(k1 * k1)), d));
         t[k] = pack_vals<T, N>(th, lt, m);
+        // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
+        w[k] = pack_vals<T, N>(cadd(cscale(a1, -(k1 * k1)), c1), cadd(cscale(a2, -(k1 * k1)), c2), m);
     }
-    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);
-    // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
-    cx<T> w[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int kx = m <= N / 2 ? m : N - m;
-        const T k1sq = (T)kx * (T)kx;
-        w[k] = cadd(cscale(pack_at<T, N>(A1, A2, m), -k1sq), pack_at<T, N>(C1, C2, m));
-    }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+    __builtin_amdgcn_sched_barrier(0);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);  // theta, Lap(theta)
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
     double s_dis = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k)

@@ -499,62 +499,47 @@ void vg3_dis_kernel(Vg3Args<T> a) {
     const cx<T>* W[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
-    cx<T> z[E], t[E], w[E];
-    double s_dis = 0;
-#pragma unroll
-    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    __builtin_amdgcn_sched_barrier(0);
-    // (u3, theta), theta = i k1 Y_1 + Y2_2 + Y3_3
+    // one load phase: each of the nine input half rows is read once and feeds
+    // the packed transforms (u1, u2), (u3, theta), (Lap u1, Lap u2),
+    // (Lap u3, Lap theta); theta = i k1 Y_1 + Y2_2 + Y3_3,
+    // Lap(theta) = -k1^2 theta + i k1 L_1 + D, Lap(u_c) = -k1^2 Y_c + L_c
+    cx<T> z[E], t[E], w[E], v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T k1o = (h == N / 2) ? (T)0 : (T)h;
-        const cx<T> th = cadd(cscale(cmuli(W[0][h]), k1o), cadd(W[4][h], W[8][h]));
-        t[k] = pack_vals<T, N>(W[2][h], th, m);
-    }
-    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);
-    // Lap(u1), Lap(u2) = x-inverse of (-k1^2 Y_c + L_c)
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T k1sq = (T)h * (T)h;
-        w[k] = cadd(cscale(pack_at<T, N>(W[0], W[1], m), -k1sq), pack_at<T, N>(W[9], W[10], m));
-    }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
-#pragma unroll
-    for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * (double)t[k].y * (double)w[k].x +
-                 2.0 * (double)z[k].y * (double)t[k].y * (double)w[k].y;
-    // (Lap(u3), Lap(theta)): Lap(u3) from (-k1^2 Y_3 + L_3); Lap(theta) from
-    // -k1^2 (i k1 Y_1 + Y2_2 + Y3_3) + i k1 L_1 + D
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        // seven inputs per mode: at most two modes' loads in flight
-        if (k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> lu3 = cadd(cscale(W[2][h], -(k1 * k1)), W[11][h]);
-        const cx<T> th = cadd(cscale(cmuli(W[0][h]), k1o), cadd(W[4][h], W[8][h]));
-        const cx<T> lth = cadd(cadd(cscale(th, -(k1 * k1)), cscale(cmuli(W[9][h]), k1o)), W[12][h]);
-        w[k] = pack_vals<T, N>(lu3, lth, m);
+        const T q = -(k1 * k1);
+        const cx<T> y1 = W[0][h], y2 = W[1][h], y3 = W[2][h];
+        const cx<T> th = cadd(cscale(cmuli(y1), k1o), cadd(W[4][h], W[8][h]));
+        const cx<T> l1 = W[9][h];
+        z[k] = pack_vals<T, N>(y1, y2, m);
+        t[k] = pack_vals<T, N>(y3, th, m);
+        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), W[10][h]), m);
+        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), W[12][h]);
+        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), W[11][h]), lth, m);
     }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
+    __builtin_amdgcn_sched_barrier(0);
+    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
+    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);  // u3, theta
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
+    line_fft<T, N, E, true, false>(v, j, Ls, a.tw);  // Lap(u3), Lap(theta)
+    double s_dis = 0;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        s_dis += 2.0 * (double)z[k].x * (double)t[k].y * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)t[k].y * (double)w[k].y;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double u1 = z[k].x, u2 = z[k].y, uu3 = t[k].x;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-        s_dis += 2.0 * uu3 * (double)t[k].y * (double)w[k].x + uu * (double)w[k].y;
+        s_dis += 2.0 * uu3 * (double)t[k].y * (double)v[k].x + uu * (double)v[k].y;
     }
     if (!valid) s_dis = 0;
-    double v[1] = {s_dis};
-    k2d::vg_reduce<C::THREADS, 1>(v, red, 3, a.part);
+    double vals[1] = {s_dis};
+    k2d::vg_reduce<C::THREADS, 1>(vals, red, 3, a.part);
 }
 
 // shell spectra, 3-D (spectra_compute3D :1018-1108): u_d = (u.k)/|k|,
