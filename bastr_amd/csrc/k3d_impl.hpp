@@ -251,6 +251,11 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
     __shared__ cx<T> lds[LR * lds_line_cx(N)];
+    // In FP64, R3_STAGE keeps the packed (u1, u2) half-row spectrum here for
+    // its second use (the x-derivatives) instead of reading it again; each
+    // thread reads back only what it wrote.  (In FP32 the second read is cheaper.)
+    constexpr bool KEEP = MODE == R3_STAGE && sizeof(T) == 8;
+    __shared__ cx<T> keep[KEEP ? LR * N : 1];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int nrows = a.g.n1 * a.g.n2;
@@ -314,8 +319,12 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     const cx<T>* Y3 = a.hin[2] + ro;
     cx<T> z[E], w[E];
     T u3[E];
+    cx<T>* kp = keep + (KEEP ? l * N : 0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(Y1, Y2, j + k * P);
+    for (int k = 0; k < E; ++k) {
+        z[k] = pack_at<T, N>(Y1, Y2, j + k * P);
+        if constexpr (KEEP) kp[j + k * P] = z[k];
+    }
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
     {
         T n3[E];
@@ -348,7 +357,8 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
-        w[k] = cscale(cmuli(pack_at<T, N>(Y1, Y2, m)), kxs);
+        const cx<T> zs = KEEP ? kp[m] : pack_at<T, N>(Y1, Y2, m);
+        w[k] = cscale(cmuli(zs), kxs);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
 #pragma unroll
