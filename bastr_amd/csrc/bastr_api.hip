@@ -510,7 +510,9 @@ k3d::Geo3<T> geo3(const bastr_handle* h) {
     return g;
 }
 
-// z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: Z33_c = -k3^2, Z333 = -i k3^3 of u3)
+// z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: ZL_c = -(k2^2 + k3^2) u_c and
+// ZD = -i (k2^2 + k3^2)(k2 u2 + k3 u3), the y and z parts of Lap(u_c) and of
+// d2 Lap(u2) + d3 Lap(u3); k2 is the column's own wavenumber)
 template <typename T>
 int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/) {
     k3d::ColXArgs<T> c{};
@@ -522,8 +524,15 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
     if (mode >= 1)
         for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_IK, 3 + q);
     if (mode == 2) {
-        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_MK2, 6 + q);
-        c.job[nj++] = job1(2, k3d::M_MIK3, 9);
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(q, k3d::M_LAP, 6 + q);
+        k3d::ColJob d{};
+        d.nterm = 2;
+        d.in[0] = 1; d.mult[0] = k3d::M_LAP_IKC;
+        d.in[1] = 2; d.mult[1] = k3d::M_LAP_IK;
+        d.out = 9;
+        c.job[nj++] = d;
+        c.kc_div = h->hp;
+        c.kc_n = h->n1;
     }
     c.njob = nj;
     c.pitch = h->hp * h->n1;
@@ -537,7 +546,8 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
 }
 
 // y-inverse of the z-inverse outputs into W: Y_c, Y2_c = i k2 Y, Y3_c (and with
-// vg: L_c = y-part of -(k2^2 + k3^2) u_c, D = y-part of -(k2^2+k3^2)(i k2 u2 + i k3 u3))
+// vg: L_c = y-part of -(k2^2 + k3^2) u_c, D = y-part of -(k2^2+k3^2)(i k2 u2 + i k3 u3),
+// the y-inverses of ZL_c and ZD)
 template <typename T>
 int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/, int out0 = 0) {
     k3d::ColXArgs<T> c{};
@@ -551,22 +561,8 @@ int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/,
         for (int q = 0; q < 3; ++q) c.job[nj++] = job1(3 + q, k3d::M_ONE, 6 + q);
     }
     if (mode == 2) {
-        for (int q = 0; q < 3; ++q) {
-            k3d::ColJob j{};
-            j.nterm = 2;
-            j.in[0] = q;      j.mult[0] = k3d::M_MK2;
-            j.in[1] = 6 + q;  j.mult[1] = k3d::M_ONE;
-            j.out = 9 + q;
-            c.job[nj++] = j;
-        }
-        k3d::ColJob d{};
-        d.nterm = 4;
-        d.in[0] = 1; d.mult[0] = k3d::M_MIK3;  // -i k2^3 Z_u2
-        d.in[1] = 7; d.mult[1] = k3d::M_IK;    //  i k2   Z33_u2
-        d.in[2] = 5; d.mult[2] = k3d::M_MK2;   // -k2^2   Z3_u3
-        d.in[3] = 9; d.mult[3] = k3d::M_ONE;   //         Z333_u3
-        d.out = 12;
-        c.job[nj++] = d;
+        for (int q = 0; q < 3; ++q) c.job[nj++] = job1(6 + q, k3d::M_ONE, 9 + q);
+        c.job[nj++] = job1(9, k3d::M_ONE, 12);
     }
     c.njob = nj;
     c.pitch = h->hp;

@@ -20,7 +20,11 @@ using k2d::RowCfg;
 // Multiplier of a column-pass term, a function of the signed wavenumber k of
 // the position along the pass (ko = k with the Nyquist entry zeroed for odd
 // powers, the Hermitian part of a c2c real-part inverse).
-enum Mult { M_ONE = 0, M_IK = 1, M_MK2 = 2, M_MIK3 = 3 };
+// Multipliers of a column-pass term, k the pass wavenumber; the LAP forms also
+// use the column's own wavenumber kc (a z pass, whose columns sit at fixed ky):
+//   M_LAP = -(kc^2 + k^2), M_LAP_IKC = -i (kc^2 + k^2) kc, M_LAP_IK = -i (kc^2 + k^2) k.
+// Odd powers of a wavenumber vanish at that axis's Nyquist index.
+enum Mult { M_ONE = 0, M_IK = 1, M_MK2 = 2, M_MIK3 = 3, M_LAP = 4, M_LAP_IKC = 5, M_LAP_IK = 6 };
 
 struct ColJob {
     int nterm;
@@ -41,6 +45,7 @@ struct ColXArgs {
     long plane_stride;   // elements between planes
     int nplanes;
     double scale;        // applied to every output (1: none)
+    int kc_div, kc_n;    // LAP multipliers: kc = ksigned(column / kc_div, kc_n) (kc_div 0: unused)
     const cx<T>* tw;
 };
 

@@ -35,11 +35,14 @@ using k2d::xcd_block;
 // ---------------------------------------------------------------------------
 // multiplier of `mult` at signed wavenumber k as a complex factor (re, im)
 template <typename T>
-__device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko) {
+__device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko, T kc, T kco) {
     switch (mult) {
         case M_IK: return {(T)0, ko};
         case M_MK2: return {-(k * k), (T)0};
         case M_MIK3: return {(T)0, -(ko * ko * ko)};
+        case M_LAP: return {-(kc * kc + k * k), (T)0};
+        case M_LAP_IKC: return {(T)0, -((kc * kc + k * k) * kco)};
+        case M_LAP_IK: return {(T)0, -((kc * kc + k * k) * ko)};
         default: return {(T)1, (T)0};
     }
 }
@@ -66,6 +69,13 @@ void colx_kernel(ColXArgs<T> a) {
     const size_t pitch = (size_t)a.pitch;
     const LdsLine<T, true> L{lds + l, CB};
     const ColJob& jb = a.job[q];
+    // the column's own wavenumber (LAP multipliers only)
+    T kc = 0, kco = 0;
+    if (a.kc_div > 0) {
+        const int ic = (cb * CB + l) / a.kc_div;
+        kc = (T)ksigned(ic, a.kc_n);
+        kco = (2 * ic == a.kc_n) ? (T)0 : kc;
+    }
     cx<T> v[E];
     if (MT == 1) {
         const cx<T>* __restrict__ src = a.in[jb.in[0]];
@@ -77,7 +87,7 @@ void colx_kernel(ColXArgs<T> a) {
             for (int k = 0; k < E; ++k) {
                 const int m = j + k * P;
                 const T kk = (T)ksigned(m, N);
-                v[k] = cmul(v[k], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk));
+                v[k] = cmul(v[k], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco));
             }
         }
     } else {
@@ -93,7 +103,7 @@ void colx_kernel(ColXArgs<T> a) {
             for (int k = 0; k < E; ++k) {
                 const int m = j + k * P;
                 const T kk = (T)ksigned(m, N);
-                v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk)));
+                v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco)));
             }
         }
         __builtin_amdgcn_sched_barrier(0);
