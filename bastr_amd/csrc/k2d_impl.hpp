@@ -383,7 +383,11 @@ void vg_dis_kernel(VgArgs<T> a) {
     // packed transforms (u1, u2), (theta, Lap theta), (Lap u1, Lap u2)
     cx<T> z[E], t[E], w[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
+    for (int kk = 0; kk < E; ++kk) {
+        // points k and E-1-k in turn: the half-row entries of point E-1-k are
+        // the mirrors (h = N - m) of point k's on the partner lane, so the
+        // second read of each entry follows the first and hits the cache
+        const int k = (kk & 1) ? E - 1 - (kk >> 1) : (kk >> 1);
         __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
