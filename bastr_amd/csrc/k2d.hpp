@@ -52,9 +52,8 @@ struct RowCfg {
     static constexpr int THREADS = L * P;
 };
 
-// Row-pass shape of the velocity-gradient kernels, which hold several
-// gradient components per point across their transforms: 8 points per thread
-// in 2-D, 4 in the 3-D kernel (three components) to stay within 256 registers.
+// Row-pass shape of the velocity-gradient kernels (8 points per thread; the
+// kernels order their transforms so that at most three lines are live).
 template <typename T, int N, int EE = 8>
 struct VgCfg {
     static constexpr int E = EE;
