@@ -20,10 +20,11 @@ namespace k2d {
 // Column-pass launch shape for (T, N): CB adjacent columns per workgroup,
 // E points per thread (EMAX from N = 256 up, else 8), P = N/E threads per
 // column, about 256 threads per workgroup.  CB*sizeof(complex) >= 64 B in
-// FP64 (32 B in FP32, and for 4096-point FP64 columns) keeps each row access
-// to whole sectors (the neighbouring column blocks, run on the same XCD, read
-// the rest of each line from its L2; narrower blocks cost extra write-backs
-// of partly written lines); CB <= 16 so
+// FP64 up to 2048-point columns, 128 B in FP32 up to 1024 (measured: FP32
+// columns of 64-byte rows fetched 1.5x their bytes), 32 B otherwise, keeps
+// each row access to whole sectors (the neighbouring column blocks, run on
+// the same XCD, read the rest of each line from its L2; narrower blocks cost
+// extra write-backs of partly written lines); CB <= 16 so
 // CB divides every supported row length; LDS is CB*N*sizeof(T) (+1/16
 // padding) for the split real/imaginary exchange.  Small workgroups let
 // several of them share a CU, so one block's loads overlap another's
@@ -32,7 +33,8 @@ template <typename T, int N, int EMAX = 16>
 struct ColCfg {
     static constexpr int E = N >= 256 ? EMAX : 8;
     static constexpr int P = N / E;
-    static constexpr int CBMIN = (sizeof(T) == 8 && N <= 2048 ? 64 : 32) / (int)sizeof(cx<T>);
+    static constexpr int CBMIN =
+        (sizeof(T) == 8 && N <= 2048 ? 64 : (sizeof(T) == 4 && N <= 1024 ? 128 : 32)) / (int)sizeof(cx<T>);
     static constexpr int CBW = (256 / P) > CBMIN ? (256 / P) : CBMIN;
     static constexpr int CB = CBW > 16 ? 16 : CBW;
     static constexpr int THREADS = CB * P;
