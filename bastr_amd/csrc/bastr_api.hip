@@ -6,6 +6,7 @@
 // iteration (solution.F90:54-97) is the launch sequence in run_iteration().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -131,6 +132,11 @@ struct bastr_handle {
     std::vector<std::pair<int, size_t>> ev_pending;
     int64_t prof_launch[BASTR_K_NCLASS] = {};
     double prof_ms[BASTR_K_NCLASS] = {};
+    // the last unprofiled step call's iteration sequence, captured as a HIP
+    // graph and replayed while the step count stays the same
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+    bool use_graph = true;
     // post-processing arena (bastr_pp_*), allocated on first use
     void* pp[30] = {};
     int* pp_status = nullptr;  // always INT_MAX: the post-processing gate stays open
@@ -802,8 +808,18 @@ int run_iteration3(bastr_handle* h, int it) {
     return 0;
 }
 
+void drop_graph(bastr_handle* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->gsteps = 0;
+}
+
+// longest step call replayed as one graph (about 25 nodes per iteration)
+constexpr int kMaxGraphSteps = 256;
+
 int ensure_capacity(bastr_handle* h, int n) {
     if (n <= h->cap) return 0;
+    drop_graph(h);  // the captured launches point at the old buffers
     if (h->ddiag) (void)hipFree(h->ddiag);
     if (h->dspec) (void)hipFree(h->dspec);
     h->ddiag = nullptr;
@@ -827,8 +843,36 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(fill_diag_kernel, dim3((nsteps + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nsteps);
     HIPCHK(hipGetLastError());
-    for (int it = 0; it < nsteps; ++it)
-        if ((rc = (h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it)))) return rc;
+    // The ~25 launches of each iteration are replayed as one graph; the capture
+    // is kept for the next call with the same step count (every argument is
+    // fixed per handle: buffers, the iteration index within the call, the
+    // device-side NaN gate).  Profiled calls launch directly: their timing
+    // events stay out of the capture.
+    if (h->use_graph && h->prof_mask == 0 && nsteps <= kMaxGraphSteps) {
+        if (!h->gexec || h->gsteps != nsteps) {
+            drop_graph(h);
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int it = 0; it < nsteps && !rc; ++it)
+                rc = h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it);
+            const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+            if (rc || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : fail(BASTR_EDEVICE, "bastr_step: graph capture failed: %s", hipGetErrorString(ec));
+            }
+            const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                h->gexec = nullptr;
+                return fail(BASTR_EDEVICE, "bastr_step: graph instantiation failed: %s", hipGetErrorString(ei));
+            }
+            h->gsteps = nsteps;
+        }
+        HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    } else {
+        for (int it = 0; it < nsteps; ++it)
+            if ((rc = (h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it)))) return rc;
+    }
     int status = INT_MAX;
     HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -891,6 +935,10 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     h->n2 = nd == 3 ? p->ka : 1;
     h->prec = p->precision;
     h->esz = p->precision == BASTR_PREC_F32 ? 4 : 8;
+    {
+        const char* gv = std::getenv("BASTR_GRAPH");  // "0": launch every iteration directly
+        h->use_graph = !(gv && std::strcmp(gv, "0") == 0);
+    }
     h->allkmax = p->ia / 3 < p->kmax ? p->ia / 3 : p->kmax;
     h->nu = p->reynolds == 0.0 ? 0.0 : 1.0 / p->reynolds;
     const int cutoff = (int)std::floor((double)p->ia / 3.0);
@@ -1001,6 +1049,7 @@ int bastr_destroy(bastr_handle* h) {
         if (h->pp[q]) (void)hipFree(h->pp[q]);
     if (h->pp_status) (void)hipFree(h->pp_status);
     if (h->pp_part) (void)hipFree(h->pp_part);
+    drop_graph(h);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
