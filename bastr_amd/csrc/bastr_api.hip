@@ -91,6 +91,15 @@ struct bastr_handle {
     int nd = 2, n0 = 0, n1 = 0, prec = 0, allkmax = 0;
     double nu = 0;
     int nh = 0, hp = 0, cut2 = 0;
+    // Dealias-limited passes.  The step's spectra are zero from |k| = cut on
+    // along every axis (the entry dealias zeroes them and every tendency is
+    // dealiased), so the step's passes neither read nor write those modes:
+    // x passes stop at kx = cut, column passes run cols columns (cut rounded
+    // up to 16, or every column when that exceeds hp) and skip |k| >= cut
+    // along the pass, and the ky rows |ky| < cut are rows_n rows, row r being
+    // ky index r, or r + rows_skip from r = rows_at on.
+    bool prune = true;
+    int cut = 0, cols = 0, rows_n = 0, rows_at = 0, rows_skip = 0;
     size_t npts = 0, nhalf = 0;
     size_t esz = 8;  // sizeof(T)
     hipStream_t stream = nullptr;
@@ -218,11 +227,15 @@ Geo<T> geo(const bastr_handle* h) {
     g.hp = h->hp;
     g.cut2 = h->cut2;
     g.lproject = h->p.lproject;
+    g.kxlim = h->hp;
     g.nu = h->nu;
     g.dt = h->p.deltat;
     g.inv_n = 1.0 / ((double)h->n0 * (double)h->n1);
     return g;
 }
+
+// x-pass limit of the step's passes (kx < cut when dealias-limited)
+int step_kxlim(const bastr_handle* h) { return h->prune ? h->cut : h->hp; }
 
 template <typename T>
 cx<T>* C(void* p) { return (cx<T>*)p; }
@@ -241,7 +254,7 @@ k3d::ColJob job1(int in, int mult, int out) {
 //   mode 1: + W2 = i k2 u1, W3 = i k2 u2                 (compute_ut2D :825-830)
 //   mode 2: + W4 = -k2^2 u1, W5 = -k2^2 u2, W6 = -i k2^3 u2   (velgrad_calculate2D)
 template <typename T>
-int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode) {
+int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode, bool prune) {
     k3d::ColXArgs<T> c{};
     c.gate = g;
     c.in[0] = C<T>((void*)src[0]);
@@ -261,7 +274,8 @@ int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode) 
     }
     c.njob = nj;
     c.pitch = h->hp;
-    c.ncols = h->hp;
+    c.ncols = prune && h->prune ? h->cols : h->hp;
+    c.kcut = prune && h->prune ? h->cut : 0;
     c.plane_stride = 0;
     c.nplanes = 1;
     c.scale = 1.0;
@@ -271,9 +285,10 @@ int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode) 
 }
 
 // 2-D y-forward of nf half-spectrum fields src -> dst, times `scale`
+// (prune: only the columns and rows the dealiased update reads)
 template <typename T>
 int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst, int nf, double scale, int pitch,
-              const void* tw) {
+              const void* tw, bool prune = false) {
     k3d::ColXArgs<T> c{};
     c.gate = g;
     for (int q = 0; q < nf; ++q) {
@@ -283,7 +298,8 @@ int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
     }
     c.njob = nf;
     c.pitch = pitch;
-    c.ncols = pitch;
+    c.ncols = prune && h->prune ? h->cols : pitch;
+    c.kcut = prune && h->prune ? h->cut : 0;
     c.plane_stride = 0;
     c.nplanes = 1;
     c.scale = scale;
@@ -294,7 +310,7 @@ int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
 
 // spectral update (dealias/stage combination) from the y-forward outputs in W[0..1]
 template <typename T>
-int run_update2(bastr_handle* h, const Gate& g, int mode, bool first) {
+int run_update2(bastr_handle* h, const Gate& g, int mode, bool first, bool prune) {
     ColFwdArgs<T> c{};
     c.g = geo<T>(h);
     c.gate = g;
@@ -307,6 +323,17 @@ int run_update2(bastr_handle* h, const Gate& g, int mode, bool first) {
     }
     c.pitch = h->hp;
     c.cn_first = first ? 1 : 0;
+    if (prune && h->prune) {
+        c.nkx = h->cut;
+        c.nky = h->rows_n;
+        c.ky_skip_at = h->rows_at;
+        c.ky_skip = h->rows_skip;
+    } else {
+        c.nkx = h->nh;
+        c.nky = h->n1;
+        c.ky_skip_at = h->n1;
+        c.ky_skip = 0;
+    }
     LAUNCHCHK(launch_update<T>(mode, c, h->stream));
     return 0;
 }
@@ -330,7 +357,7 @@ int run_prologue(bastr_handle* h) {
     void* w[2] = {h->W[0], h->W[1]};
     int rc = run_yfwd2<T>(h, open, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1);
     if (rc) return rc;
-    return run_update2<T>(h, open, UP_PROLOGUE, false);
+    return run_update2<T>(h, open, UP_PROLOGUE, false, false);
 }
 
 // u = ifft2d(u_hat) into ureal
@@ -340,7 +367,7 @@ int run_to_physical(bastr_handle* h) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
     HIPCHK(hipGetLastError());
     const void* Us[2] = {h->U[0], h->U[1]};
-    int rc = run_yinv2<T>(h, open, Us, 0);
+    int rc = run_yinv2<T>(h, open, Us, 0, false);
     if (rc) return rc;
     RowArgs<T> r{};
     r.g = geo<T>(h);
@@ -362,12 +389,13 @@ template <typename T>
 int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* Uin, bool first = false) {
     if (!first) {
         ProfScope ps(h, BASTR_K_COL_INV);
-        const int rc = run_yinv2<T>(h, gstep, Uin, 1);
+        const int rc = run_yinv2<T>(h, gstep, Uin, 1, true);
         if (rc) return rc;
     }
     // (stage A's y-inverse ran in the diagnostic phase with the velgrad outputs)
     RowArgs<T> r{};
     r.g = geo<T>(h);
+    r.g.kxlim = step_kxlim(h);
     r.gate = gstep;
     for (int q = 0; q < 4; ++q) r.hin[q] = C<T>(h->W[q]);
     r.hout[0] = C<T>(h->NX[0]);
@@ -382,8 +410,8 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
         ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
         void* nx[2] = {h->NX[0], h->NX[1]};
         void* w[2] = {h->W[0], h->W[1]};
-        int rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1);
-        if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first);
+        int rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
+        if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true);
         if (rc) return rc;
     }
     return 0;
@@ -411,11 +439,12 @@ int run_iteration(bastr_handle* h, int it) {
     {
         ProfScope ps(h, BASTR_K_COL_INV_VG);
         const void* Us[2] = {h->U[0], h->U[1]};
-        const int rc = run_yinv2<T>(h, gdiag, Us, 2);
+        const int rc = run_yinv2<T>(h, gdiag, Us, 2, true);
         if (rc) return rc;
     }
     VgArgs<T> va{};
     va.g = geo<T>(h);
+    va.g.kxlim = step_kxlim(h);
     va.gate = gdiag;
     for (int q = 0; q < 7; ++q) va.w[q] = C<T>(h->W[q]);
     va.part = h->part_vg;
@@ -510,6 +539,7 @@ k3d::Geo3<T> geo3(const bastr_handle* h) {
     g.hp = h->hp;
     g.cut2 = h->cut2;
     g.lproject = h->p.lproject;
+    g.kxlim = h->hp;
     g.nu = h->nu;
     g.dt = h->p.deltat;
     g.inv_n = 1.0 / ((double)h->n0 * (double)h->n1 * (double)h->n2);
@@ -520,7 +550,7 @@ k3d::Geo3<T> geo3(const bastr_handle* h) {
 // ZD = -i (k2^2 + k3^2)(k2 u2 + k3 u3), the y and z parts of Lap(u_c) and of
 // d2 Lap(u2) + d3 Lap(u3); k2 is the column's own wavenumber)
 template <typename T>
-int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/) {
+int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/, bool prune) {
     k3d::ColXArgs<T> c{};
     c.gate = g;
     for (int q = 0; q < 3; ++q) c.in[q] = C<T>(src[q]);
@@ -542,9 +572,19 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
     }
     c.njob = nj;
     c.pitch = h->hp * h->n1;
-    c.ncols = h->hp * h->n1;
-    c.plane_stride = 0;
-    c.nplanes = 1;
+    if (prune && h->prune && h->cols < h->hp) {
+        // one plane per kept ky row, its first cols kx columns
+        c.ncols = h->cols;
+        c.plane_stride = h->hp;
+        c.nplanes = h->rows_n;
+        c.pskip_at = h->rows_at;
+        c.pskip = h->rows_skip;
+    } else {
+        c.ncols = h->hp * h->n1;
+        c.plane_stride = 0;
+        c.nplanes = 1;
+    }
+    c.kcut = prune && h->prune ? h->cut : 0;
     c.scale = 1.0;
     c.tw = C<T>(h->tw1);
     LAUNCHCHK(k3d::launch_colx<T>(h->n2, true, c, h->stream));
@@ -555,7 +595,7 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
 // vg: L_c = y-part of -(k2^2 + k3^2) u_c, D = y-part of -(k2^2+k3^2)(i k2 u2 + i k3 u3),
 // the y-inverses of ZL_c and ZD)
 template <typename T>
-int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/, int out0 = 0) {
+int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/, bool prune, int out0 = 0) {
     k3d::ColXArgs<T> c{};
     c.gate = g;
     for (int q = 0; q < 10; ++q) c.in[q] = C<T>(h->Z[q]);
@@ -572,7 +612,8 @@ int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/,
     }
     c.njob = nj;
     c.pitch = h->hp;
-    c.ncols = h->hp;
+    c.ncols = prune && h->prune ? h->cols : h->hp;
+    c.kcut = prune && h->prune ? h->cut : 0;
     c.plane_stride = (long)h->hp * h->n1;
     c.nplanes = h->n2;
     c.scale = 1.0;
@@ -583,7 +624,7 @@ int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/,
 
 // y-forward of the x-pass outputs NX -> NY
 template <typename T>
-int run_yfwd3(bastr_handle* h, const Gate& g) {
+int run_yfwd3(bastr_handle* h, const Gate& g, bool prune) {
     k3d::ColXArgs<T> c{};
     c.gate = g;
     for (int q = 0; q < 3; ++q) {
@@ -593,7 +634,8 @@ int run_yfwd3(bastr_handle* h, const Gate& g) {
     }
     c.njob = 3;
     c.pitch = h->hp;
-    c.ncols = h->hp;
+    c.ncols = prune && h->prune ? h->cols : h->hp;
+    c.kcut = prune && h->prune ? h->cut : 0;
     c.plane_stride = (long)h->hp * h->n1;
     c.nplanes = h->n2;
     c.scale = 1.0;
@@ -625,7 +667,7 @@ int run_zfwd3(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
 
 // z-forward of NY fused with the spectral update of the state
 template <typename T>
-int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first) {
+int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first, bool prune) {
     k3d::Cf3Args<T> c{};
     c.g = geo3<T>(h);
     c.gate = g;
@@ -636,6 +678,18 @@ int run_cf3(bastr_handle* h, const Gate& g, int mode, bool first) {
         c.KA[q] = C<T>(h->KA[q]);
     }
     c.cn_first = first ? 1 : 0;
+    if (prune && h->prune && h->cols < h->hp) {
+        c.nkx = h->cols;
+        c.nky = h->rows_n;
+        c.ky_skip_at = h->rows_at;
+        c.ky_skip = h->rows_skip;
+    } else {  // every column, as one flat row
+        c.nkx = h->hp * h->n1;
+        c.nky = 1;
+        c.ky_skip_at = 1;
+        c.ky_skip = 0;
+    }
+    c.kzcut = prune && h->prune ? h->cut : 0;
     c.tw = C<T>(h->tw1);
     LAUNCHCHK(k3d::launch_cf3<T>(h->n2, mode, c, h->stream));
     return 0;
@@ -645,6 +699,7 @@ template <typename T>
 int run_x3(bastr_handle* h, const Gate& g) {
     k3d::Row3Args<T> r{};
     r.g = geo3<T>(h);
+    r.g.kxlim = step_kxlim(h);
     r.gate = g;
     for (int q = 0; q < 9; ++q) r.hin[q] = C<T>(h->W[q]);
     for (int q = 0; q < 3; ++q) r.hout[q] = C<T>(h->NX[q]);
@@ -668,8 +723,8 @@ int run_prologue3(bastr_handle* h) {
     r.tw = C<T>(h->tw0);
     LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
     int rc;
-    if ((rc = run_yfwd3<T>(h, open))) return rc;
-    return run_cf3<T>(h, open, k3d::C3_PROLOGUE, false);
+    if ((rc = run_yfwd3<T>(h, open, false))) return rc;
+    return run_cf3<T>(h, open, k3d::C3_PROLOGUE, false, false);
 }
 
 template <typename T>
@@ -678,8 +733,8 @@ int run_to_physical3(bastr_handle* h) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
     HIPCHK(hipGetLastError());
     int rc;
-    if ((rc = run_zinv3<T>(h, open, h->U, 0))) return rc;
-    if ((rc = run_yinv3<T>(h, open, 0, 3))) return rc;  // into W[3..5]; ureal aliases W[0..2]
+    if ((rc = run_zinv3<T>(h, open, h->U, 0, false))) return rc;
+    if ((rc = run_yinv3<T>(h, open, 0, false, 3))) return rc;  // into W[3..5]; ureal aliases W[0..2]
     k3d::Row3Args<T> r{};
     r.g = geo3<T>(h);
     r.gate = open;
@@ -699,8 +754,8 @@ int run_stage3(bastr_handle* h, int stage, const Gate& g, bool first) {
     if (!first) {
         {
             ProfScope ps(h, BASTR_K_COL_INV);
-            if ((rc = run_zinv3<T>(h, g, h->UB, 1))) return rc;
-            if ((rc = run_yinv3<T>(h, g, 1))) return rc;
+            if ((rc = run_zinv3<T>(h, g, h->UB, 1, true))) return rc;
+            if ((rc = run_yinv3<T>(h, g, 1, true))) return rc;
         }
     }
     {
@@ -709,8 +764,8 @@ int run_stage3(bastr_handle* h, int stage, const Gate& g, bool first) {
     }
     static const int modes[5] = {k3d::C3_A, k3d::C3_B, k3d::C3_C, k3d::C3_CN_MID, k3d::C3_CN_LAST};
     ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
-    if ((rc = run_yfwd3<T>(h, g))) return rc;
-    return run_cf3<T>(h, g, modes[stage], first);
+    if ((rc = run_yfwd3<T>(h, g, true))) return rc;
+    return run_cf3<T>(h, g, modes[stage], first, true);
 }
 
 template <typename T>
@@ -731,13 +786,14 @@ int run_iteration3(bastr_handle* h, int it) {
     }
     {
         ProfScope ps(h, BASTR_K_COL_INV_VG);
-        if ((rc = run_zinv3<T>(h, gdiag, h->U, 2))) return rc;
-        if ((rc = run_yinv3<T>(h, gdiag, 2))) return rc;
+        if ((rc = run_zinv3<T>(h, gdiag, h->U, 2, true))) return rc;
+        if ((rc = run_yinv3<T>(h, gdiag, 2, true))) return rc;
     }
     {
         ProfScope ps(h, BASTR_K_VELGRAD);
         k3d::Vg3Args<T> va{};
         va.g = geo3<T>(h);
+        va.g.kxlim = step_kxlim(h);
         va.gate = gdiag;
         for (int q = 0; q < 13; ++q) va.w[q] = C<T>(h->W[q]);
         va.part = h->part_vg;
@@ -946,6 +1002,22 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     h->nh = h->n0 / 2 + 1;
     const int cb = col_block_width(h->n1, h->prec);
     h->hp = (h->nh + cb - 1) / cb * cb;
+    {
+        const char* pv = std::getenv("BASTR_PRUNE");  // "0": the step's passes cover every mode
+        h->prune = !(pv && std::strcmp(pv, "0") == 0);
+        h->cut = cutoff;
+        const int c16 = (cutoff + 15) / 16 * 16;  // a multiple of every column block width
+        h->cols = c16 <= h->hp ? c16 : h->hp;
+        if (2 * cutoff - 1 < h->n1) {
+            h->rows_n = 2 * cutoff - 1;
+            h->rows_at = cutoff;
+            h->rows_skip = h->n1 - (2 * cutoff - 1);
+        } else {
+            h->rows_n = h->n1;
+            h->rows_at = h->n1;
+            h->rows_skip = 0;
+        }
+    }
     h->npts = (size_t)h->n0 * h->n1 * h->n2;
     h->nhalf = (size_t)h->hp * h->n1 * h->n2;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1143,7 +1215,7 @@ static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, dou
         }
         r.tw = C<T>(h->tw0);
         LAUNCHCHK(k3d::launch_row3<T>(h->n0, k3d::R3_FWD_PACK, r, h->n1 * h->n2, h->stream));
-        int rc = run_yfwd3<T>(h, open);
+        int rc = run_yfwd3<T>(h, open, false);
         if (rc) return rc;
         void* src[1] = {h->NY[0]};
         void* dst[1] = {h->U[0]};

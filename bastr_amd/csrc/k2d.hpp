@@ -73,6 +73,7 @@ struct Geo {
     int hp;          // half-spectrum pitch
     int cut2;        // floor(ia/3)^2: dealias when kx^2+ky^2 >= cut2 (tool.F90:185-191)
     int lproject;
+    int kxlim;       // x passes: half-row entries kx >= kxlim are zero (not read, not written)
     double nu, dt, inv_n;
 };
 
@@ -103,6 +104,9 @@ struct ColFwdArgs {
     cx<T>* KA[2];        // k_A (UP_A writes), k_A + 4 k_B (UP_B writes, UP_C reads)
     int pitch;
     int cn_first;        // UP_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
+    int nkx;             // modes kx < nkx of
+    int nky;             // nky rows: row r is ky index r, or r + ky_skip from r = ky_skip_at on
+    int ky_skip_at, ky_skip;
 };
 
 template <typename T>

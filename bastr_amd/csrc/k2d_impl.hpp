@@ -36,9 +36,12 @@ __device__ __forceinline__ int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 
 // z = a + i b for packed position m of a length-N row, from half rows A, B
 // (kx = 0..N/2).  DC and Nyquist take real parts (the real part of a c2c
-// inverse equals the c2r inverse of the Hermitian part).
+// inverse equals the c2r inverse of the Hermitian part).  Entries kx >= lim
+// are zero and not read.
 template <typename T, int N>
-__device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, int m) {
+__device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, int m,
+                                         int lim = N) {
+    if ((m <= N / 2 ? m : N - m) >= lim) return {(T)0, (T)0};
     if (m == 0 || m == N / 2) return {A[m].x, B[m].x};
     if (m < N / 2) {
         const cx<T> a = A[m], b = B[m];
@@ -57,10 +60,11 @@ __device__ __forceinline__ cx<T> pack_vals(cx<T> a, cx<T> b, int m) {
 }
 
 // Store the half spectra of a forward-transformed packed row: thread j holds
-// X[j + k*P]; writes (X[h] + conj X[N-h])/2 and (X[h] - conj X[N-h])/(2i).
+// X[j + k*P]; writes (X[h] + conj X[N-h])/2 and (X[h] - conj X[N-h])/(2i)
+// for h < lim.
 template <typename T, int N, int E, bool WAVE>
 __device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, false, WAVE>& L, cx<T>* __restrict__ o1,
-                                             cx<T>* __restrict__ o2, bool valid) {
+                                             cx<T>* __restrict__ o2, bool valid, int lim = N) {
     constexpr int P = N / E;
     lds_sync<WAVE>();
 #pragma unroll
@@ -74,12 +78,12 @@ __device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, f
         const cx<T> bm = L.ld((N - h) & (N - 1));
         const cx<T> b = cconj(bm);
         const cx<T> d = csub(a, b);
-        if (valid) {
+        if (valid && h < lim) {
             o1[h] = {(a.x + b.x) * half, (a.y + b.y) * half};
             o2[h] = {d.y * half, -d.x * half};
         }
     }
-    if (j == 0 && valid) {
+    if (j == 0 && valid && N / 2 < lim) {
         const cx<T> a = L.ld(N / 2);
         o1[N / 2] = {a.x, (T)0};
         o2[N / 2] = {a.y, (T)0};
@@ -99,8 +103,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void update_kernel(ColFwdArgs<T> a) {
     if (a.gate.closed()) return;
     const int kx = blockIdx.x * 256 + threadIdx.x;
-    const int m = blockIdx.y;
-    if (kx >= a.g.nh) return;
+    const int m = (int)blockIdx.y >= a.ky_skip_at ? (int)blockIdx.y + a.ky_skip : (int)blockIdx.y;
+    if (kx >= a.nkx) return;
     const int N = a.g.n1;
     const int ky = ksigned(m, N);
     const int kk2 = kx * kx + ky * ky;
@@ -223,7 +227,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     if (MODE == R_INV_PACK) {
         cx<T> v[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(A1, A2, j + k * P);
+        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(A1, A2, j + k * P, a.g.kxlim);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -236,11 +240,12 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
         return;
     }
     // R_STAGE: u, du/dx, du/dy from half rows, nonlinear term, forward x pass
+    const int lim = a.g.kxlim;
     cx<T> z[E], zx[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
-        z[k] = pack_at<T, N>(A1, A2, m);
+        z[k] = pack_at<T, N>(A1, A2, m, lim);
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         zx[k] = cscale(cmuli(z[k]), kxs);
     }
@@ -252,7 +257,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     const cx<T>* B1 = a.hin[2] + (size_t)row * hp;
     const cx<T>* B2 = a.hin[3] + (size_t)row * hp;
 #pragma unroll
-    for (int k = 0; k < E; ++k) zx[k] = pack_at<T, N>(B1, B2, j + k * P);
+    for (int k = 0; k < E; ++k) zx[k] = pack_at<T, N>(B1, B2, j + k * P, lim);
     line_fft<T, N, E, true, false>(zx, j, Ls, a.tw);  // (d2 u1, d2 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -260,7 +265,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
         n[k].y = n[k].y - z[k].y * zx[k].y;
     }
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
-    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid, lim);
 }
 
 // Block reduction of NQ moment partial sums (sums; the last one of the
@@ -308,12 +313,13 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     const cx<T>* B2 = a.w[3] + (size_t)row * hp;
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
+    const int lim = a.g.kxlim;
 
     cx<T> z[E], w[E], n[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
-        z[k] = pack_at<T, N>(A1, A2, m);
+        z[k] = pack_at<T, N>(A1, A2, m, lim);
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         w[k] = cscale(cmuli(z[k]), kxs);
     }
@@ -323,7 +329,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
     cx<T> y[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) y[k] = pack_at<T, N>(B1, B2, j + k * P);
+    for (int k = 0; k < E; ++k) y[k] = pack_at<T, N>(B1, B2, j + k * P, lim);
     line_fft<T, N, E, true, false>(y, j, Ls, a.tw);  // d2u1, d2u2
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
@@ -344,7 +350,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     }
     // nonlinear term: x-forward and split into the two half rows
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
-    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid, lim);
 
     if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
     double v0[3] = {s_um2, s_ijji, s_u2t};
@@ -393,7 +399,10 @@ void vg_dis_kernel(VgArgs<T> a) {
         const int h = m <= N / 2 ? m : N - m;
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> a1 = A1[h], a2 = A2[h], b2 = B2[h], c1 = C1[h], c2 = C2[h], d = D[h];
+        const bool in = h < a.g.kxlim;  // entries from kxlim on are zero
+        const cx<T> zero{(T)0, (T)0};
+        const cx<T> a1 = in ? A1[h] : zero, a2 = in ? A2[h] : zero, b2 = in ? B2[h] : zero;
+        const cx<T> c1 = in ? C1[h] : zero, c2 = in ? C2[h] : zero, d = in ? D[h] : zero;
         z[k] = pack_vals<T, N>(a1, a2, m);
         // theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
         const cx<T> th = cadd(cscale(cmuli(a1), k1o), b2);
@@ -680,7 +689,9 @@ static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
 
 template <typename T>
 int launch_update(int mode, const ColFwdArgs<T>& a, hipStream_t s) {
-    dim3 grid((a.g.nh + 255) / 256, a.g.n1), block(256);
+    const int last_row = a.nky - 1 >= a.ky_skip_at ? a.nky - 1 + a.ky_skip : a.nky - 1;
+    if (a.nkx < 1 || a.nkx > a.g.nh || a.nky < 1 || a.ky_skip < 0 || last_row >= a.g.n1) return -1;
+    dim3 grid((a.nkx + 255) / 256, a.nky), block(256);
     switch (mode) {
 #define M(MM) case MM: hipLaunchKernelGGL((update_kernel<T, MM>), grid, block, 0, s, a); break;
         M(UP_PROLOGUE) M(UP_A) M(UP_B) M(UP_C) M(UP_CN_MID) M(UP_CN_LAST)

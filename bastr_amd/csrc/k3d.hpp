@@ -45,7 +45,10 @@ struct ColXArgs {
     long plane_stride;   // elements between planes
     int nplanes;
     double scale;        // applied to every output (1: none)
-    int kc_div, kc_n;    // LAP multipliers: kc = ksigned(column / kc_div, kc_n) (kc_div 0: unused)
+    int kc_div, kc_n;    // LAP multipliers: kc = ksigned(column offset / kc_div, kc_n) (kc_div 0: unused)
+    int kcut;            // > 0: entries with |k| >= kcut along the pass are zero: an inverse
+                         // pass does not read them, a forward pass does not write them
+    int pskip_at, pskip; // plane p >= pskip_at is plane p + pskip (0, 0: planes in order)
     const cx<T>* tw;
 };
 
@@ -55,6 +58,7 @@ struct Geo3 {
     int nh, hp;
     int cut2;
     int lproject;
+    int kxlim;       // x passes: half-row entries kx >= kxlim are zero (not read, not written)
     double nu, dt, inv_n;
 };
 
@@ -70,6 +74,12 @@ struct Cf3Args {
     cx<T>* UB[3];
     cx<T>* KA[3];
     int cn_first;         // C3_CN_MID: this sweep's midpoint is u_hat itself (first sweep)
+    // columns: nkx consecutive offsets (kx, or the flat ky*hp + kx when nky = 1)
+    // from the start of each of nky ky rows (row r is ky index r, or r + ky_skip
+    // from r = ky_skip_at on)
+    int nkx, nky, ky_skip_at, ky_skip;
+    int kzcut;            // > 0: modes with |kz| >= kzcut are left as they are (the
+                          // dealiased state's zeros; not read, not written)
     const cx<T>* tw;
 };
 

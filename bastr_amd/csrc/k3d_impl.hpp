@@ -62,7 +62,8 @@ void colx_kernel(ColXArgs<T> a) {
     const int vb = xcd_block(blockIdx.x, a.njob * nbpp * a.nplanes);
     if (vb < 0) return;
     const int q = vb % a.njob, blk = vb / a.njob;
-    const int plane = blk / nbpp, cb = blk % nbpp;
+    const int plane0 = blk / nbpp, cb = blk % nbpp;
+    const int plane = plane0 >= a.pskip_at ? plane0 + a.pskip : plane0;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
     const size_t base = (size_t)plane * a.plane_stride + (size_t)cb * CB + l;
@@ -72,16 +73,23 @@ void colx_kernel(ColXArgs<T> a) {
     // the column's own wavenumber (LAP multipliers only)
     T kc = 0, kco = 0;
     if (a.kc_div > 0) {
-        const int ic = (cb * CB + l) / a.kc_div;
+        const int ic = (int)(base / (size_t)a.kc_div);
         kc = (T)ksigned(ic, a.kc_n);
         kco = (2 * ic == a.kc_n) ? (T)0 : kc;
     }
+    // entries |k| >= kcut along the pass: zero in an inverse pass's inputs,
+    // not stored by a forward pass
+    const int kcut = a.kcut > 0 ? a.kcut : N;
+    auto live = [&](int m) { return (m <= N / 2 ? m : N - m) < kcut; };
     cx<T> v[E];
     if (MT == 1) {
         const cx<T>* __restrict__ src = a.in[jb.in[0]];
         const int mult = jb.mult[0];
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = src[base + (size_t)(j + k * P) * pitch];
+        for (int k = 0; k < E; ++k) {
+            const int m = j + k * P;
+            v[k] = (!INV || live(m)) ? src[base + (size_t)m * pitch] : cx<T>{(T)0, (T)0};
+        }
         if (mult != M_ONE) {
 #pragma unroll
             for (int k = 0; k < E; ++k) {
@@ -103,7 +111,8 @@ void colx_kernel(ColXArgs<T> a) {
             for (int k = 0; k < E; ++k) {
                 const int m = j + k * P;
                 const T kk = (T)ksigned(m, N);
-                v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco)));
+                if (!INV || live(m))
+                    v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco)));
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -112,7 +121,10 @@ void colx_kernel(ColXArgs<T> a) {
     cx<T>* __restrict__ dst = a.out[jb.out];
     const T sc = (T)a.scale;
 #pragma unroll
-    for (int k = 0; k < E; ++k) dst[base + (size_t)(j + k * P) * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
+    for (int k = 0; k < E; ++k) {
+        const int m = j + k * P;
+        if (INV || live(m)) dst[base + (size_t)m * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -136,11 +148,14 @@ void cf3_kernel(Cf3Args<T> a) {
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
     const int ncols = a.g.hp * a.g.n1;
-    const int blk = xcd_block(blockIdx.x, ncols / CB);
+    const int nbr = a.nkx / CB;
+    const int blk = xcd_block(blockIdx.x, a.nky * nbr);
     if (blk < 0) return;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
-    const int col = blk * CB + l;
+    const int r0 = blk / nbr;
+    const int r = r0 >= a.ky_skip_at ? r0 + a.ky_skip : r0;
+    const int col = r * a.g.hp + (blk % nbr) * CB + l;
     const LdsLine<T, true> L{lds + l, CB};
     const size_t pitch = (size_t)ncols;
     const T sc = (T)a.g.inv_n;
@@ -159,6 +174,9 @@ void cf3_kernel(Cf3Args<T> a) {
         if (k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int kz = ksigned(m, N);
+        // beyond the dealias radius along z the state holds zeros that stay
+        // zeros; the inverse passes do not read these entries
+        if (MODE != C3_PROLOGUE && a.kzcut > 0 && (kz < 0 ? -kz : kz) >= a.kzcut) continue;
         const int kk2 = kx * kx + ky * ky + kz * kz;
         const size_t idx = (size_t)m * pitch + col;
         cx<T> t[3];
@@ -234,7 +252,8 @@ void cf3_kernel(Cf3Args<T> a) {
 
 // one real field as a packed row (second field zero)
 template <typename T, int N>
-__device__ __forceinline__ cx<T> pack1(const cx<T>* __restrict__ A, int m) {
+__device__ __forceinline__ cx<T> pack1(const cx<T>* __restrict__ A, int m, int lim = N) {
+    if ((m <= N / 2 ? m : N - m) >= lim) return {(T)0, (T)0};
     if (m == 0 || m == N / 2) return {A[m].x, (T)0};
     if (m < N / 2) return A[m];
     return cconj(A[N - m]);
@@ -242,9 +261,10 @@ __device__ __forceinline__ cx<T> pack1(const cx<T>* __restrict__ A, int m) {
 
 // pack (A, i*k1*A) for one half row A: value at m of A + i*(i k1 A)
 template <typename T, int N>
-__device__ __forceinline__ cx<T> pack_a_ika(const cx<T>* __restrict__ A, int m) {
+__device__ __forceinline__ cx<T> pack_a_ika(const cx<T>* __restrict__ A, int m, int lim = N) {
     const int h = m <= N / 2 ? m : N - m;
     const T ko = (h == N / 2) ? (T)0 : (T)h;
+    if (h >= lim) return {(T)0, (T)0};
     const cx<T> a = A[h];
     const cx<T> b = cscale(cmuli(a), ko);  // i k1 A at h
     if (m == 0 || m == N / 2) return {a.x, b.x};
@@ -301,7 +321,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     if (MODE == R3_INV_PACK) {
         cx<T> v[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(a.hin[0] + ro, a.hin[1] + ro, j + k * P);
+        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(a.hin[0] + ro, a.hin[1] + ro, j + k * P, a.g.kxlim);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -312,7 +332,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
             }
         }
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack1<T, N>(a.hin[2] + ro, j + k * P);
+        for (int k = 0; k < E; ++k) v[k] = pack1<T, N>(a.hin[2] + ro, j + k * P, a.g.kxlim);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -327,19 +347,20 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     const cx<T>* Y1 = a.hin[0] + ro;
     const cx<T>* Y2 = a.hin[1] + ro;
     const cx<T>* Y3 = a.hin[2] + ro;
+    const int lim = a.g.kxlim;
     cx<T> z[E], w[E];
     T u3[E];
     cx<T>* kp = keep + (KEEP ? l * N : 0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        z[k] = pack_at<T, N>(Y1, Y2, j + k * P);
+        z[k] = pack_at<T, N>(Y1, Y2, j + k * P, lim);
         if constexpr (KEEP) kp[j + k * P] = z[k];
     }
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
     {
         T n3[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
+        for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P, lim);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -348,7 +369,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P);
+        for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P, lim);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u3, d3 u3)
 #pragma unroll
         for (int k = 0; k < E; ++k) w[k] = {n3[k] - z[k].y * w[k].x - u3[k] * w[k].y, (T)0};
@@ -358,7 +379,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int h = j + k * P;
-            if (h <= N / 2) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{w[k].x, (T)0} : w[k];
+            if (h <= N / 2 && h < lim) a.hout[2][ro + h] = (h == 0 || h == N / 2) ? cx<T>{w[k].x, (T)0} : w[k];
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -367,7 +388,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
-        const cx<T> zs = KEEP ? kp[m] : pack_at<T, N>(Y1, Y2, m);
+        const cx<T> zs = KEEP ? kp[m] : pack_at<T, N>(Y1, Y2, m, lim);
         w[k] = cscale(cmuli(zs), kxs);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
@@ -375,7 +396,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     for (int k = 0; k < E; ++k) n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[3] + ro, a.hin[4] + ro, j + k * P);
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[3] + ro, a.hin[4] + ro, j + k * P, lim);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u1, d2 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -384,7 +405,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[6] + ro, a.hin[7] + ro, j + k * P);
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[6] + ro, a.hin[7] + ro, j + k * P, lim);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d3 u1, d3 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -392,7 +413,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         n12[k].y = n12[k].y - u3[k] * w[k].y;
     }
     line_fft<T, N, E, false, false>(n12, j, Ls, a.tw);
-    unpack_store<T, N, E>(n12, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid);
+    unpack_store<T, N, E>(n12, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid, lim);
 }
 
 template <typename T, int N>
@@ -427,23 +448,26 @@ void vg3_kernel(Vg3Args<T> a) {
     for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
+    const int lim = a.g.kxlim;
     // W: 0..2 Y_c, 3..5 Y2_c (d2), 6..8 Y3_c (d3), 9..11 L_c, 12 D
-    // half-row value of d1 X at h (odd power of k1: zero at the Nyquist index)
-    auto d1 = [](const cx<T>* X, int h) {
+    // half-row value of X at h (zero from kxlim on) and of d1 X (odd power of
+    // k1: zero at the Nyquist index)
+    auto at = [lim](const cx<T>* X, int h) { return h < lim ? X[h] : cx<T>{(T)0, (T)0}; };
+    auto d1 = [at](const cx<T>* X, int h) {
         const T ko = (h == N / 2) ? (T)0 : (T)h;
-        return cscale(cmuli(X[h]), ko);
+        return cscale(cmuli(at(X, h)), ko);
     };
     cx<T> z[E], w[E];
     T u3[E], b11[E];
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
-    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P);
+    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P, lim);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
-        w[k] = pack_vals<T, N>(W[2][h], d1(W[0], h), m);
+        w[k] = pack_vals<T, N>(at(W[2], h), d1(W[0], h), m);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u1
 #pragma unroll
@@ -458,7 +482,7 @@ void vg3_kernel(Vg3Args<T> a) {
         for (int k = 0; k < E; ++k) {
             const int m = j + k * P;
             const int h = m <= N / 2 ? m : N - m;
-            w[k] = pack_vals<T, N>(X[h], pr < 2 ? d1(Y, h) : Y[h], m);
+            w[k] = pack_vals<T, N>(at(X, h), pr < 2 ? d1(Y, h) : at(Y, h), m);
         }
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
 #pragma unroll
@@ -471,7 +495,7 @@ void vg3_kernel(Vg3Args<T> a) {
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[4], W[8], j + k * P);
+    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[4], W[8], j + k * P, lim);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u2, d3 u3
     double dv[E];
 #pragma unroll
@@ -532,14 +556,17 @@ void vg3_dis_kernel(Vg3Args<T> a) {
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
         const T q = -(k1 * k1);
-        const cx<T> y1 = W[0][h], y2 = W[1][h], y3 = W[2][h];
-        const cx<T> th = cadd(cscale(cmuli(y1), k1o), cadd(W[4][h], W[8][h]));
-        const cx<T> l1 = W[9][h];
+        // entries from kxlim on are zero
+        const bool in = h < a.g.kxlim;
+        const cx<T> zero{(T)0, (T)0};
+        const cx<T> y1 = in ? W[0][h] : zero, y2 = in ? W[1][h] : zero, y3 = in ? W[2][h] : zero;
+        const cx<T> th = cadd(cscale(cmuli(y1), k1o), in ? cadd(W[4][h], W[8][h]) : zero);
+        const cx<T> l1 = in ? W[9][h] : zero;
         z[k] = pack_vals<T, N>(y1, y2, m);
         t[k] = pack_vals<T, N>(y3, th, m);
-        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), W[10][h]), m);
-        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), W[12][h]);
-        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), W[11][h]), lth, m);
+        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), in ? W[10][h] : zero), m);
+        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), in ? W[12][h] : zero);
+        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), in ? W[11][h] : zero), lth, m);
     }
     __builtin_amdgcn_sched_barrier(0);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
@@ -831,8 +858,11 @@ template <typename T, int N>
 static int cf3_n(int mode, const Cf3Args<T>& a, hipStream_t s) {
     using C = Fwd3Cfg<T, N>;
     const int ncols = a.g.hp * a.g.n1;
-    if (ncols % C::CB) return -1;
-    dim3 grid(k2d::xcd_grid_h(ncols / C::CB)), block(C::THREADS);
+    const int last_row = a.nky - 1 >= a.ky_skip_at ? a.nky - 1 + a.ky_skip : a.nky - 1;
+    if (a.nkx < C::CB || a.nkx % C::CB || a.nky < 1 || a.ky_skip < 0 ||
+        (long)last_row * a.g.hp + a.nkx > ncols || (a.nky > 1 && a.nkx > a.g.hp))
+        return -1;
+    dim3 grid(k2d::xcd_grid_h(a.nky * (a.nkx / C::CB))), block(C::THREADS);
     switch (mode) {
 #define M(MM) case MM: hipLaunchKernelGGL((cf3_kernel<T, N, MM>), grid, block, 0, s, a); break;
         M(C3_PROLOGUE) M(C3_A) M(C3_B) M(C3_C) M(C3_CN_MID) M(C3_CN_LAST)

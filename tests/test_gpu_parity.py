@@ -233,6 +233,43 @@ def test_graph_replay_equals_direct_launches(hip, monkeypatch, shape, prec):
     assert all(np.array_equal(x, y) for x, y in zip(a.get_fields(), b.get_fields()))
 
 
+@pytest.mark.parametrize("shape,prec,tm,force,proj", [
+    ((64, 64), abi.PREC_F64, 1, 2, True),
+    ((128, 64), abi.PREC_F64, 1, 0, False),   # y-Nyquist row inside the dealias radius
+    ((64, 128), abi.PREC_F64, 2, 5, True),    # CN, more ky rows than kept
+    ((16, 16), abi.PREC_F64, 1, 0, False),    # every column kept
+    ((256, 256), abi.PREC_F32, 1, 2, True),
+    ((16, 16, 16), abi.PREC_F64, 1, 0, True),
+    ((32, 32, 32), abi.PREC_F64, 2, 1, False),
+    ((64, 64, 64), abi.PREC_F64, 1, 2, True),
+    ((32, 32, 32), abi.PREC_F32, 1, 2, True),
+])
+def test_dealias_limited_passes_equal_full_passes(hip, monkeypatch, shape, prec, tm, force, proj):
+    """The step's passes skip the modes beyond the dealias radius (zero in
+    every spectrum they touch); the run equals one whose passes cover every
+    mode, bit for bit."""
+    print(f"case prune {shape} prec={prec} tm={tm} fm={force}", flush=True)
+    ka = shape[2] if len(shape) == 3 else 0
+    p = make_params(shape[0], shape[1], ka, timemethod=tm, forcemethod=force, lproject=proj, precision=prec,
+                    target_energy=0.6 if force in (1, 5) else 0.075)
+    u = hitgen(shape, k0=4, seed=23)
+    monkeypatch.setenv("BASTR_PRUNE", "0")
+    full = Solver(hip, "bastr", p)
+    monkeypatch.delenv("BASTR_PRUNE")
+    lim = Solver(hip, "bastr", p)
+    full.set_fields(*u)
+    lim.set_fields(*u)
+    for n in (3, 2):
+        ra, da, sa = full.step(n)
+        rb, db, sb = lim.step(n)
+        assert ra == rb == 0
+        assert np.array_equal(sa, sb)
+        for names in (abi.Diag.SPECALL, abi.Diag.VELGRAD):
+            assert np.array_equal(diag_table(da, names), diag_table(db, names))
+        assert np.array_equal([d.force_factor for d in da], [d.force_factor for d in db], equal_nan=True)
+    assert all(np.array_equal(x, y) for x, y in zip(full.get_fields(), lim.get_fields()))
+
+
 def test_fp32_against_fp64_oracle(hip, orc):
     print("case fp32 128x128", flush=True)
     shape = (128, 128)
