@@ -34,29 +34,97 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 __device__ __forceinline__ int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 
-// z = a + i b for packed position m of a length-N row, from half rows A, B
-// (kx = 0..N/2).  DC and Nyquist take real parts (the real part of a c2c
-// inverse equals the c2r inverse of the Hermitian part).  Entries kx >= lim
-// are zero and not read.
-template <typename T, int N>
-__device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, int m,
-                                         int lim = N) {
-    if ((m <= N / 2 ? m : N - m) >= lim) return {(T)0, (T)0};
-    if (m == 0 || m == N / 2) return {A[m].x, B[m].x};
-    if (m < N / 2) {
-        const cx<T> a = A[m], b = B[m];
-        return {a.x - b.y, a.y + b.x};
-    }
-    const cx<T> a = A[N - m], b = B[N - m];
-    return {a.x + b.y, b.x - a.y};
-}
-
 // pack_at from the half-row values a = A[h], b = B[h], h = min(m, N - m)
 template <typename T, int N>
 __device__ __forceinline__ cx<T> pack_vals(cx<T> a, cx<T> b, int m) {
     if (m == 0 || m == N / 2) return {a.x, b.x};
     if (m < N / 2) return {a.x - b.y, a.y + b.x};
     return {a.x + b.y, b.x - a.y};
+}
+
+// Half-row entry X[h], zero from h = lim on.  The load is issued for every
+// lane (a lane past lim reads the DC entry, which its row's other lanes read
+// too) and the value selected, so a row's loads stay one branch-free group.
+template <typename T>
+__device__ __forceinline__ cx<T> half_at(const cx<T>* __restrict__ X, int h, int lim) {
+    const bool in = h < lim;
+    const cx<T> v = X[in ? h : 0];
+    return in ? v : cx<T>{(T)0, (T)0};
+}
+
+// z = a + i b for packed position m of a length-N row, from half rows A, B
+// (kx = 0..N/2).  DC and Nyquist take real parts (the real part of a c2c
+// inverse equals the c2r inverse of the Hermitian part).  Entries kx >= lim
+// are zero.
+template <typename T, int N>
+__device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, int m,
+                                         int lim = N) {
+    const int h = m <= N / 2 ? m : N - m;
+    return pack_vals<T, N>(half_at(A, h, lim), half_at(B, h, lim), m);
+}
+
+// Loads through buffer resources: the hardware returns zero for an offset at
+// or past the resource's range, with no memory access.
+constexpr int kBufWord3 = 0x00020000;          // gfx9 raw buffer, no format conversion
+constexpr unsigned kBufRange = 0x80000000u;    // whole-array resources (every array < 2 GiB)
+
+template <typename T>
+__device__ __forceinline__ cx<T> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    cx<T> v;
+    if constexpr (sizeof(cx<T>) == 16) {
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+        __builtin_memcpy(&v, &w, 16);
+    } else {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        const u2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+        __builtin_memcpy(&v, &w, 8);
+    }
+    return v;
+}
+
+// Half row ro (element offset of its first entry) of the array at base, its
+// entries from lim on read as zero.  BUF (the row offset is wave-uniform):
+// a resource whose range ends at the row's entry lim, so no per-lane
+// predicate at all; otherwise a resource over the whole array and an offset
+// moved out of range for h >= lim (one select per point, shared by every
+// array read at that point).
+template <typename T, bool BUF>
+struct HalfRow;
+
+template <typename T>
+struct HalfRow<T, true> {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ HalfRow(const cx<T>* base, size_t ro, int lim)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)(base + ro), (short)0, lim * (int)sizeof(cx<T>), kBufWord3)) {}
+    __device__ __forceinline__ cx<T> operator[](int h) const { return buf_load<T>(r, (unsigned)h * sizeof(cx<T>)); }
+};
+
+template <typename T>
+struct HalfRow<T, false> {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned ro;
+    int lim;
+    __device__ __forceinline__ HalfRow(const cx<T>* base, size_t ro_, int lim_)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)kBufRange, kBufWord3)),
+          ro((unsigned)ro_), lim(lim_) {}
+    __device__ __forceinline__ cx<T> operator[](int h) const {
+        return buf_load<T>(r, h < lim ? (ro + (unsigned)h) * (unsigned)sizeof(cx<T>) : kBufRange);
+    }
+};
+
+// pack_at over two HalfRows
+template <typename T, int N, typename R>
+__device__ __forceinline__ cx<T> pack_rows(const R& A, const R& B, int m) {
+    const int h = m <= N / 2 ? m : N - m;
+    return pack_vals<T, N>(A[h], B[h], m);
+}
+
+// the row index of a line, made wave-uniform (SGPR) when the line spans whole wavefronts
+template <int P>
+__device__ __forceinline__ int line_row(int row) {
+    if constexpr (P >= 64) return __builtin_amdgcn_readfirstlane(row);
+    return row;
 }
 
 // Store the half spectra of a forward-transformed packed row: thread j holds
@@ -222,12 +290,14 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
         unpack_store<T, N, E>(v, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid);
         return;
     }
-    const cx<T>* A1 = a.hin[0] + (size_t)row * hp;
-    const cx<T>* A2 = a.hin[1] + (size_t)row * hp;
+    constexpr bool BUF = P >= 64;
+    const size_t ro = (size_t)line_row<P>(row) * hp;
+    const int lim = a.g.kxlim;
+    const HalfRow<T, BUF> A1(a.hin[0], ro, lim), A2(a.hin[1], ro, lim);
     if (MODE == R_INV_PACK) {
         cx<T> v[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(A1, A2, j + k * P, a.g.kxlim);
+        for (int k = 0; k < E; ++k) v[k] = pack_rows<T, N>(A1, A2, j + k * P);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -240,12 +310,11 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
         return;
     }
     // R_STAGE: u, du/dx, du/dy from half rows, nonlinear term, forward x pass
-    const int lim = a.g.kxlim;
     cx<T> z[E], zx[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
-        z[k] = pack_at<T, N>(A1, A2, m, lim);
+        z[k] = pack_rows<T, N>(A1, A2, m);
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         zx[k] = cscale(cmuli(z[k]), kxs);
     }
@@ -254,10 +323,9 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     cx<T> n[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * zx[k].x), -(z[k].x * zx[k].y)};
-    const cx<T>* B1 = a.hin[2] + (size_t)row * hp;
-    const cx<T>* B2 = a.hin[3] + (size_t)row * hp;
+    const HalfRow<T, BUF> B1(a.hin[2], ro, lim), B2(a.hin[3], ro, lim);
 #pragma unroll
-    for (int k = 0; k < E; ++k) zx[k] = pack_at<T, N>(B1, B2, j + k * P, lim);
+    for (int k = 0; k < E; ++k) zx[k] = pack_rows<T, N>(B1, B2, j + k * P);
     line_fft<T, N, E, true, false>(zx, j, Ls, a.tw);  // (d2 u1, d2 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -307,19 +375,19 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     const int row = valid ? row0 : a.g.n1 - 1;
     const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
-    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
-    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
-    const cx<T>* B1 = a.w[2] + (size_t)row * hp;
-    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
+    constexpr bool BUF = P >= 64;
+    const size_t ro = (size_t)line_row<P>(row) * hp;
+    const int lim = a.g.kxlim;
+    const HalfRow<T, BUF> A1(a.w[0], ro, lim), A2(a.w[1], ro, lim);
+    const HalfRow<T, BUF> B1(a.w[2], ro, lim), B2(a.w[3], ro, lim);
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
-    const int lim = a.g.kxlim;
 
     cx<T> z[E], w[E], n[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
-        z[k] = pack_at<T, N>(A1, A2, m, lim);
+        z[k] = pack_rows<T, N>(A1, A2, m);
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         w[k] = cscale(cmuli(z[k]), kxs);
     }
@@ -329,7 +397,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
     cx<T> y[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) y[k] = pack_at<T, N>(B1, B2, j + k * P, lim);
+    for (int k = 0; k < E; ++k) y[k] = pack_rows<T, N>(B1, B2, j + k * P);
     line_fft<T, N, E, true, false>(y, j, Ls, a.tw);  // d2u1, d2u2
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
@@ -350,7 +418,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     }
     // nonlinear term: x-forward and split into the two half rows
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
-    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + (size_t)row * hp, a.hout[1] + (size_t)row * hp, valid, lim);
+    unpack_store<T, N, E>(n, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid, lim);
 
     if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
     double v0[3] = {s_um2, s_ijji, s_u2t};
@@ -378,12 +446,11 @@ void vg_dis_kernel(VgArgs<T> a) {
     const int row = valid ? row0 : a.g.n1 - 1;
     const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
     const size_t hp = (size_t)a.g.hp;
-    const cx<T>* A1 = a.w[0] + (size_t)row * hp;
-    const cx<T>* A2 = a.w[1] + (size_t)row * hp;
-    const cx<T>* B2 = a.w[3] + (size_t)row * hp;
-    const cx<T>* C1 = a.w[4] + (size_t)row * hp;
-    const cx<T>* C2 = a.w[5] + (size_t)row * hp;
-    const cx<T>* D = a.w[6] + (size_t)row * hp;
+    constexpr bool BUF = P >= 64;
+    const size_t ro = (size_t)line_row<P>(row) * hp;
+    const int lim = a.g.kxlim;  // entries from kxlim on are zero
+    const HalfRow<T, BUF> A1(a.w[0], ro, lim), A2(a.w[1], ro, lim), B2(a.w[3], ro, lim);
+    const HalfRow<T, BUF> C1(a.w[4], ro, lim), C2(a.w[5], ro, lim), D(a.w[6], ro, lim);
 
     // one load phase: each input half row is read once and feeds all three
     // packed transforms (u1, u2), (theta, Lap theta), (Lap u1, Lap u2)
@@ -399,10 +466,7 @@ void vg_dis_kernel(VgArgs<T> a) {
         const int h = m <= N / 2 ? m : N - m;
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
-        const bool in = h < a.g.kxlim;  // entries from kxlim on are zero
-        const cx<T> zero{(T)0, (T)0};
-        const cx<T> a1 = in ? A1[h] : zero, a2 = in ? A2[h] : zero, b2 = in ? B2[h] : zero;
-        const cx<T> c1 = in ? C1[h] : zero, c2 = in ? C2[h] : zero, d = in ? D[h] : zero;
+        const cx<T> a1 = A1[h], a2 = A2[h], b2 = B2[h], c1 = C1[h], c2 = C2[h], d = D[h];
         z[k] = pack_vals<T, N>(a1, a2, m);
         // theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
         const cx<T> th = cadd(cscale(cmuli(a1), k1o), b2);

@@ -23,6 +23,8 @@ namespace k3d {
 using k2d::ksigned;
 using k2d::pack_at;
 using k2d::pack_vals;
+using k2d::HalfRow;
+using k2d::pack_rows;
 using k2d::unpack_store;
 using k2d::xcd_block;
 
@@ -251,20 +253,20 @@ void cf3_kernel(Cf3Args<T> a) {
 }
 
 // one real field as a packed row (second field zero)
-template <typename T, int N>
-__device__ __forceinline__ cx<T> pack1(const cx<T>* __restrict__ A, int m, int lim = N) {
-    if ((m <= N / 2 ? m : N - m) >= lim) return {(T)0, (T)0};
-    if (m == 0 || m == N / 2) return {A[m].x, (T)0};
-    if (m < N / 2) return A[m];
-    return cconj(A[N - m]);
+template <typename T, int N, typename R>
+__device__ __forceinline__ cx<T> pack1(const R& A, int m) {
+    const int h = m <= N / 2 ? m : N - m;
+    const cx<T> a = A[h];
+    if (m == 0 || m == N / 2) return {a.x, (T)0};
+    if (m < N / 2) return a;
+    return cconj(a);
 }
 
 // pack (A, i*k1*A) for one half row A: value at m of A + i*(i k1 A)
-template <typename T, int N>
-__device__ __forceinline__ cx<T> pack_a_ika(const cx<T>* __restrict__ A, int m, int lim = N) {
+template <typename T, int N, typename R>
+__device__ __forceinline__ cx<T> pack_a_ika(const R& A, int m) {
     const int h = m <= N / 2 ? m : N - m;
     const T ko = (h == N / 2) ? (T)0 : (T)h;
-    if (h >= lim) return {(T)0, (T)0};
     const cx<T> a = A[h];
     const cx<T> b = cscale(cmuli(a), ko);  // i k1 A at h
     if (m == 0 || m == N / 2) return {a.x, b.x};
@@ -318,10 +320,13 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         }
         return;
     }
+    constexpr bool BUF = P >= 64;
+    const int lim = a.g.kxlim;
     if (MODE == R3_INV_PACK) {
+        const HalfRow<T, BUF> H0(a.hin[0], ro, lim), H1(a.hin[1], ro, lim), H2(a.hin[2], ro, lim);
         cx<T> v[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack_at<T, N>(a.hin[0] + ro, a.hin[1] + ro, j + k * P, a.g.kxlim);
+        for (int k = 0; k < E; ++k) v[k] = pack_rows<T, N>(H0, H1, j + k * P);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -332,7 +337,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
             }
         }
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = pack1<T, N>(a.hin[2] + ro, j + k * P, a.g.kxlim);
+        for (int k = 0; k < E; ++k) v[k] = pack1<T, N>(H2, j + k * P);
         line_fft<T, N, E, true, false>(v, j, Ls, a.tw);
         if (valid) {
 #pragma unroll
@@ -344,23 +349,20 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     // n3 first (its gradients (b31, b32, b33) come from the (u3, d1 u3) and
     // (d2 u3, d3 u3) transforms), transformed and stored before n1, n2 are
     // accumulated, so u, one partial term and one transform are live at a time.
-    const cx<T>* Y1 = a.hin[0] + ro;
-    const cx<T>* Y2 = a.hin[1] + ro;
-    const cx<T>* Y3 = a.hin[2] + ro;
-    const int lim = a.g.kxlim;
+    const HalfRow<T, BUF> Y1(a.hin[0], ro, lim), Y2(a.hin[1], ro, lim), Y3(a.hin[2], ro, lim);
     cx<T> z[E], w[E];
     T u3[E];
     cx<T>* kp = keep + (KEEP ? l * N : 0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        z[k] = pack_at<T, N>(Y1, Y2, j + k * P, lim);
+        z[k] = pack_rows<T, N>(Y1, Y2, j + k * P);
         if constexpr (KEEP) kp[j + k * P] = z[k];
     }
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
     {
         T n3[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P, lim);
+        for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -369,7 +371,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[5] + ro, a.hin[8] + ro, j + k * P, lim);
+        for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(HalfRow<T, BUF>(a.hin[5], ro, lim), HalfRow<T, BUF>(a.hin[8], ro, lim), j + k * P);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u3, d3 u3)
 #pragma unroll
         for (int k = 0; k < E; ++k) w[k] = {n3[k] - z[k].y * w[k].x - u3[k] * w[k].y, (T)0};
@@ -388,7 +390,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
-        const cx<T> zs = KEEP ? kp[m] : pack_at<T, N>(Y1, Y2, m, lim);
+        const cx<T> zs = KEEP ? kp[m] : pack_rows<T, N>(Y1, Y2, m);
         w[k] = cscale(cmuli(zs), kxs);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
@@ -396,7 +398,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     for (int k = 0; k < E; ++k) n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[3] + ro, a.hin[4] + ro, j + k * P, lim);
+    for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(HalfRow<T, BUF>(a.hin[3], ro, lim), HalfRow<T, BUF>(a.hin[4], ro, lim), j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u1, d2 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -405,7 +407,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(a.hin[6] + ro, a.hin[7] + ro, j + k * P, lim);
+    for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(HalfRow<T, BUF>(a.hin[6], ro, lim), HalfRow<T, BUF>(a.hin[7], ro, lim), j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d3 u1, d3 u2)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -443,31 +445,29 @@ void vg3_kernel(Vg3Args<T> a) {
     const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     // a line of at least 64 threads is wave-uniform: its row offset lives in SGPRs
     const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
-    const cx<T>* W[13];
-#pragma unroll
-    for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
+    constexpr bool BUF = P >= 64;
+    const int lim = a.g.kxlim;  // entries from kxlim on are zero
+    using Row = HalfRow<T, BUF>;
+    auto W = [&](int q) { return Row(a.w[q], ro, lim); };
     const double nu = a.g.nu;
     const double eta0 = (double)(2.0f * 3.14f);
-    const int lim = a.g.kxlim;
     // W: 0..2 Y_c, 3..5 Y2_c (d2), 6..8 Y3_c (d3), 9..11 L_c, 12 D
-    // half-row value of X at h (zero from kxlim on) and of d1 X (odd power of
-    // k1: zero at the Nyquist index)
-    auto at = [lim](const cx<T>* X, int h) { return h < lim ? X[h] : cx<T>{(T)0, (T)0}; };
-    auto d1 = [at](const cx<T>* X, int h) {
+    // half-row value of d1 X (odd power of k1: zero at the Nyquist index)
+    auto d1 = [](const Row& X, int h) {
         const T ko = (h == N / 2) ? (T)0 : (T)h;
-        return cscale(cmuli(at(X, h)), ko);
+        return cscale(cmuli(X[h]), ko);
     };
     cx<T> z[E], w[E];
     T u3[E], b11[E];
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
 #pragma unroll
-    for (int k = 0; k < E; ++k) z[k] = pack_at<T, N>(W[0], W[1], j + k * P, lim);
+    for (int k = 0; k < E; ++k) z[k] = pack_rows<T, N>(W(0), W(1), j + k * P);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
-        w[k] = pack_vals<T, N>(at(W[2], h), d1(W[0], h), m);
+        w[k] = pack_vals<T, N>(W(2)[h], d1(W(0), h), m);
     }
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u1
 #pragma unroll
@@ -476,13 +476,13 @@ void vg3_kernel(Vg3Args<T> a) {
 #pragma unroll
     for (int pr = 0; pr < 3; ++pr) {
         __builtin_amdgcn_sched_barrier(0);
-        const cx<T>* X = pr == 0 ? W[3] : (pr == 1 ? W[6] : W[7]);  // d2 u1, d3 u1, d3 u2
-        const cx<T>* Y = pr == 0 ? W[1] : (pr == 1 ? W[2] : W[5]);  // (d1 of) u2, u3; d2 u3
+        const Row X = W(pr == 0 ? 3 : (pr == 1 ? 6 : 7));  // d2 u1, d3 u1, d3 u2
+        const Row Y = W(pr == 0 ? 1 : (pr == 1 ? 2 : 5));  // (d1 of) u2, u3; d2 u3
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int m = j + k * P;
             const int h = m <= N / 2 ? m : N - m;
-            w[k] = pack_vals<T, N>(at(X, h), pr < 2 ? d1(Y, h) : at(Y, h), m);
+            w[k] = pack_vals<T, N>(X[h], pr < 2 ? d1(Y, h) : Y[h], m);
         }
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);
 #pragma unroll
@@ -495,7 +495,7 @@ void vg3_kernel(Vg3Args<T> a) {
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_at<T, N>(W[4], W[8], j + k * P, lim);
+    for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(W(4), W(8), j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u2, d3 u3
     double dv[E];
 #pragma unroll
@@ -540,9 +540,10 @@ void vg3_dis_kernel(Vg3Args<T> a) {
     const int row = valid ? row0 : nrows - 1;
     const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
     const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
-    const cx<T>* W[13];
-#pragma unroll
-    for (int q = 0; q < 13; ++q) W[q] = a.w[q] + ro;
+    constexpr bool BUF = P >= 64;
+    const int lim = a.g.kxlim;  // entries from kxlim on are zero
+    using Row = HalfRow<T, BUF>;
+    auto W = [&](int q) { return Row(a.w[q], ro, lim); };
     // one load phase: each of the nine input half rows is read once and feeds
     // the packed transforms (u1, u2), (u3, theta), (Lap u1, Lap u2),
     // (Lap u3, Lap theta); theta = i k1 Y_1 + Y2_2 + Y3_3,
@@ -556,17 +557,14 @@ void vg3_dis_kernel(Vg3Args<T> a) {
         const T k1 = (T)h;
         const T k1o = (h == N / 2) ? (T)0 : k1;
         const T q = -(k1 * k1);
-        // entries from kxlim on are zero
-        const bool in = h < a.g.kxlim;
-        const cx<T> zero{(T)0, (T)0};
-        const cx<T> y1 = in ? W[0][h] : zero, y2 = in ? W[1][h] : zero, y3 = in ? W[2][h] : zero;
-        const cx<T> th = cadd(cscale(cmuli(y1), k1o), in ? cadd(W[4][h], W[8][h]) : zero);
-        const cx<T> l1 = in ? W[9][h] : zero;
+        const cx<T> y1 = W(0)[h], y2 = W(1)[h], y3 = W(2)[h];
+        const cx<T> th = cadd(cscale(cmuli(y1), k1o), cadd(W(4)[h], W(8)[h]));
+        const cx<T> l1 = W(9)[h];
         z[k] = pack_vals<T, N>(y1, y2, m);
         t[k] = pack_vals<T, N>(y3, th, m);
-        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), in ? W[10][h] : zero), m);
-        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), in ? W[12][h] : zero);
-        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), in ? W[11][h] : zero), lth, m);
+        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), W(10)[h]), m);
+        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), W(12)[h]);
+        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), W(11)[h]), lth, m);
     }
     __builtin_amdgcn_sched_barrier(0);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
