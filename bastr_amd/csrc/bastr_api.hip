@@ -431,6 +431,12 @@ int run_iteration(bastr_handle* h, int it) {
     sa.allkmax = h->allkmax;
     sa.part_bins = h->part_bins;
     sa.part_tot = h->part_tot;
+    if (h->prune) {  // the modes inside the dealias limits (the rest are zeros)
+        sa.nkx = h->cut;
+        sa.nky = h->rows_n;
+        sa.ky_at = h->rows_at;
+        sa.ky_skip = h->rows_skip;
+    }
     {
         ProfScope ps(h, BASTR_K_SPECTRA);
         LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
@@ -782,6 +788,12 @@ int run_iteration3(bastr_handle* h, int it) {
         sa.allkmax = h->allkmax;
         sa.part_bins = h->part_bins;
         sa.part_tot = h->part_tot;
+        if (h->prune) {  // the modes inside the dealias limits (the rest are zeros)
+            sa.nkx = h->cut;
+            sa.nky = sa.nkz = h->rows_n;
+            sa.ky_at = sa.kz_at = h->rows_at;
+            sa.ky_skip = sa.kz_skip = h->rows_skip;
+        }
         LAUNCHCHK(k3d::launch_spec3<T>(sa, h->nblk_spec, h->stream));
     }
     {
@@ -1443,11 +1455,37 @@ int bastr_prof_enable(bastr_handle* h, uint32_t mask) {
 
 int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* total_ms, double* bytes) {
     if (!h || c < 0 || c >= BASTR_K_NCLASS) return fail(BASTR_EINVAL, "bastr_prof_read: bad argument");
-    // compulsory traffic in units of one half-spectrum field (nh*n1*n2 complex);
-    // 3-D classes cover the z and y passes of one stage together
-    static const double units2[BASTR_K_NCLASS] = {2, 9, 9, 0, 6, 6, 8, 10, 10, 0};
-    static const double units3[BASTR_K_NCLASS] = {3, 36, 13, 0, 24, 12, 18, 21, 21, 0};
-    const double* units = h->nd == 3 ? units3 : units2;
+    // Compulsory traffic, in units of one half-spectrum field F = nh*n1*n2
+    // complex: every array a class must read or write, once, over the modes
+    // its passes touch.  With the dealias limits, an array indexed by kx
+    // counts kx < cut (the fraction fc of F), one still indexed by ky (not
+    // yet y-transformed, or y-transformed again) also |ky| < cut (fy), and
+    // by kz |kz| < cut (fz).  3-D classes cover a stage's z and y passes.
+    const bool lim = h->prune;
+    const double fc = lim ? (double)h->cut / h->nh : 1.0;
+    const double fy = lim ? (double)h->rows_n / h->n1 : 1.0;
+    const double fz = h->nd == 3 ? fy : 1.0;
+    double units[BASTR_K_NCLASS] = {};
+    if (h->nd == 2) {
+        units[BASTR_K_SPECTRA] = 2 * fc * fy;
+        units[BASTR_K_COL_INV_VG] = fc * (2 * fy + 7);   // y-inverse: u_hat -> 7 fields
+        units[BASTR_K_VELGRAD] = 9 * fc;                 // x passes: r7 w2
+        units[BASTR_K_COL_INV] = fc * (2 * fy + 4);      // y-inverse: 2 -> 4
+        units[BASTR_K_ROW_STAGE] = 6 * fc;               // r4 w2
+        units[BASTR_K_COL_FWD_A] = 2 * fc + 6 * fc * fy; // y-forward of 2, update r U w KA UB
+        units[BASTR_K_COL_FWD_B] = 2 * fc + 8 * fc * fy; // r U KA w UB KA
+        units[BASTR_K_COL_FWD_C] = 2 * fc + 8 * fc * fy; // r U KA UB w U
+    } else {
+        const double fyz = fy * fz;
+        units[BASTR_K_SPECTRA] = 3 * fc * fyz;
+        units[BASTR_K_COL_INV_VG] = 3 * fc * fyz + 20 * fc * fy + 13 * fc;  // z: 3 -> 10, y: 10 -> 13
+        units[BASTR_K_VELGRAD] = 13 * fc;
+        units[BASTR_K_COL_INV] = 3 * fc * fyz + 12 * fc * fy + 9 * fc;      // z: 3 -> 6, y: 6 -> 9
+        units[BASTR_K_ROW_STAGE] = 12 * fc;                                  // r9 w3
+        units[BASTR_K_COL_FWD_A] = 3 * fc + 6 * fc * fy + 9 * fc * fyz;    // y-fwd 3 -> 3, z-fwd + update
+        units[BASTR_K_COL_FWD_B] = 3 * fc + 6 * fc * fy + 12 * fc * fyz;
+        units[BASTR_K_COL_FWD_C] = 3 * fc + 6 * fc * fy + 12 * fc * fyz;
+    }
     const double half = (double)h->nh * h->n1 * h->n2 * 2.0 * h->esz;
     if (launches) *launches = h->prof_launch[c];
     if (total_ms) *total_ms = h->prof_ms[c];

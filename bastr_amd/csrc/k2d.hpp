@@ -140,6 +140,9 @@ struct SpecArgs {
     int allkmax;
     double* part_bins;  // [gridDim.x][allkmax+1][2]
     double* part_tot;   // [gridDim.x][4]
+    // modes summed: kx < nkx in nky ky rows (row r is ky index r, or r + ky_skip
+    // from r = ky_at on); 0, 0, 0, 0: every mode (the others are known zeros)
+    int nkx, nky, ky_at, ky_skip;
 };
 
 struct FinArgs {

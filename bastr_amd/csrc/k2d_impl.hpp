@@ -507,10 +507,12 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     double* tots = sh + 4 * nb * 2;   // [4][4]
     for (int i = tid; i < 4 * nb * 2; i += 256) bins[i] = 0;
     __syncthreads();
-    const int n0 = a.g.n0, n1 = a.g.n1, nh = a.g.nh;
+    const int n0 = a.g.n0, n1 = a.g.n1;
+    const int nh = a.nkx > 0 ? a.nkx : a.g.nh;  // kx limit of the modes summed
+    const int nrow = a.nky > 0 ? a.nky : n1;
     const size_t hp = (size_t)a.g.hp;
     const int nbx = (nh + 63) / 64;
-    const long ntask = (long)n1 * nbx;
+    const long ntask = (long)nrow * nbx;
     // tasks are dealt round-robin over blocks and waves, so the few tasks
     // with many shells <= allkmax (small |k|) are spread over the machine
     const long tstride = 4L * gridDim.x;
@@ -522,7 +524,8 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
             const long tq = t + tstride * q;
-            const int r = (int)(tq / nbx);
+            const int r0 = (int)(tq / nbx);
+            const int r = r0 >= a.ky_at ? r0 + a.ky_skip : r0;
             const int kx = (int)(tq % nbx) * 64 + lane;
             if (tq < t1 && kx < nh) {
                 u1[q] = a.U[0][(size_t)r * hp + kx];
@@ -536,7 +539,8 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
         for (int q = 0; q < TB; ++q) {
             const long tq = t + tstride * q;
             if (tq >= t1) break;  // wave-uniform
-            const int r = (int)(tq / nbx);
+            const int r0 = (int)(tq / nbx);
+            const int r = r0 >= a.ky_at ? r0 + a.ky_skip : r0;
             const int kx = (int)(tq % nbx) * 64 + lane;
             double es = 0, ed = 0, w = 0;
             int s = INT_MAX;

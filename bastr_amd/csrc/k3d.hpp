@@ -113,6 +113,9 @@ struct Spec3Args {
     int allkmax;
     double* part_bins;    // [(K+1)*2][gridDim.x]
     double* part_tot;     // [4][gridDim.x]
+    // modes summed: kx < nkx, nky ky and nkz kz indices (index r is r, or
+    // r + skip from r = at on); 0s: every mode (the others are known zeros)
+    int nkx, nky, ky_at, ky_skip, nkz, kz_at, kz_skip;
 };
 
 template <typename T>

@@ -599,10 +599,19 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
     double* tots = sh + 4 * nb * 2;
     for (int i = tid; i < 4 * nb * 2; i += 256) bins[i] = 0;
     __syncthreads();
-    const int n0 = a.g.n0, n1 = a.g.n1, n2 = a.g.n2, nh = a.g.nh;
+    const int n0 = a.g.n0, n1 = a.g.n1, n2 = a.g.n2;
+    const int nh = a.nkx > 0 ? a.nkx : a.g.nh;  // kx limit of the modes summed
+    const int ny = a.nky > 0 ? a.nky : n1, nz = a.nkz > 0 ? a.nkz : n2;
     const size_t hp = (size_t)a.g.hp;
     const int nbx = (nh + 63) / 64;
-    const long ntask = (long)n1 * n2 * nbx;
+    const long ntask = (long)ny * nz * nbx;
+    // (ky index, kz index) of task row r0
+    auto row_of = [&](long r0, int& iy, int& iz) {
+        iy = (int)(r0 % ny);
+        iz = (int)(r0 / ny);
+        if (iy >= a.ky_at) iy += a.ky_skip;
+        if (iz >= a.kz_at) iz += a.kz_skip;
+    };
     const long tstride = 4L * gridDim.x;
     double tot[4] = {0, 0, 0, 0};
     constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
@@ -611,10 +620,11 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
             const long tq = t + tstride * q;
-            const long r = tq / nbx;  // row = kz*n1 + ky
+            int iy, iz;
+            row_of(tq / nbx, iy, iz);
             const int kx = (int)(tq % nbx) * 64 + lane;
             const bool ok = tq < ntask && kx < nh;
-            const size_t o = ok ? (size_t)r * hp + kx : 0;
+            const size_t o = ok ? ((size_t)iz * n1 + iy) * hp + kx : 0;
 #pragma unroll
             for (int c = 0; c < 3; ++c) u[q][c] = ok ? a.U[c][o] : cx<T>{(T)0, (T)0};
         }
@@ -622,12 +632,13 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
         for (int q = 0; q < TB; ++q) {
             const long tq = t + tstride * q;
             if (tq >= ntask) break;  // wave-uniform
-            const long r = tq / nbx;
+            int iy, iz;
+            row_of(tq / nbx, iy, iz);
             const int kx = (int)(tq % nbx) * 64 + lane;
             double es = 0, ed = 0, w = 0;
             int s = INT_MAX;
             if (kx < nh) {
-                const int ky = ksigned((int)(r % n1), n1), kz = ksigned((int)(r / n1), n2);
+                const int ky = ksigned(iy, n1), kz = ksigned(iz, n2);
                 w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
                 const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
                 if (kk > 0.5) {
