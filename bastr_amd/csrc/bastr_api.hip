@@ -308,7 +308,8 @@ int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
     return 0;
 }
 
-// spectral update (dealias/stage combination) from the y-forward outputs in W[0..1]
+// spectral update (dealias/stage combination) from the y-forward outputs in
+// W[0..1] (prune: the modes inside the dealias limits)
 template <typename T>
 int run_update2(bastr_handle* h, const Gate& g, int mode, bool first, bool prune) {
     ColFwdArgs<T> c{};
@@ -335,6 +336,28 @@ int run_update2(bastr_handle* h, const Gate& g, int mode, bool first, bool prune
         c.ky_skip = 0;
     }
     LAUNCHCHK(launch_update<T>(mode, c, h->stream));
+    return 0;
+}
+
+// y-forward of the x-pass outputs NX fused with the spectral update of a stage
+template <typename T>
+int run_cf2(bastr_handle* h, const Gate& g, int mode, bool first) {
+    ColFwdArgs<T> c{};
+    c.g = geo<T>(h);
+    c.gate = g;
+    c.nx[0] = C<T>(h->NX[0]);
+    c.nx[1] = C<T>(h->NX[1]);
+    for (int q = 0; q < 2; ++q) {
+        c.U[q] = C<T>(h->U[q]);
+        c.UB[q] = C<T>(h->UB[q]);
+        c.KA[q] = C<T>(h->KA[q]);
+    }
+    c.pitch = h->hp;
+    c.cn_first = first ? 1 : 0;
+    c.nkx = h->prune ? h->cols : h->hp;
+    c.kcut = h->prune ? h->cut : 0;
+    c.tw = C<T>(h->tw1);
+    LAUNCHCHK(k3d::launch_cf2<T>(h->n1, mode, c, h->stream));
     return 0;
 }
 
@@ -408,10 +431,16 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     {
         static const int modes[5] = {UP_A, UP_B, UP_C, UP_CN_MID, UP_CN_LAST};
         ProfScope ps(h, BASTR_K_COL_FWD_A + (stage < 3 ? stage : 2));
-        void* nx[2] = {h->NX[0], h->NX[1]};
-        void* w[2] = {h->W[0], h->W[1]};
-        int rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
-        if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true);
+        int rc;
+        if (k3d::cf2_full_rows<T>(h->n1)) {
+            rc = run_cf2<T>(h, gstep, modes[stage], first);
+        } else {
+            // the fused kernel would need narrower column blocks here: y-forward, then the update
+            void* nx[2] = {h->NX[0], h->NX[1]};
+            void* w[2] = {h->W[0], h->W[1]};
+            rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
+            if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true);
+        }
         if (rc) return rc;
     }
     return 0;

@@ -107,6 +107,10 @@ struct ColFwdArgs {
     int nkx;             // modes kx < nkx of
     int nky;             // nky rows: row r is ky index r, or r + ky_skip from r = ky_skip_at on
     int ky_skip_at, ky_skip;
+    // k3d::launch_cf2 (y-forward fused with the update): nkx columns, rows
+    // |ky| >= kcut left as they are (kcut 0: every row), nx the x-forward outputs
+    int kcut;
+    const cx<T>* tw;
 };
 
 template <typename T>

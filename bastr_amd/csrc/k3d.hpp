@@ -150,6 +150,10 @@ struct Scale3Args {
 
 template <typename T> int launch_colx(int n, bool inverse, const ColXArgs<T>& a, hipStream_t s);
 template <typename T> int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s);
+template <typename T> int launch_cf2(int n, int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s);
+// whether cf2 keeps the column passes' block width at n rows (else the
+// y-forward and the update run as two kernels)
+template <typename T> bool cf2_full_rows(int n);
 template <typename T> int launch_row3(int n, int mode, const Row3Args<T>& a, int nrows, hipStream_t s);
 template <typename T> int launch_vg3(int n, const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk);
 template <typename T> int launch_spec3(const Spec3Args<T>& a, int nblk, hipStream_t s);

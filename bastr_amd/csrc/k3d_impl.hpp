@@ -253,6 +253,120 @@ void cf3_kernel(Cf3Args<T> a) {
 }
 
 // one real field as a packed row (second field zero)
+// ---------------------------------------------------------------------------
+// 2-D y-forward pass fused with the spectral update (the 2-D counterpart of
+// cf3_kernel): a workgroup transforms both components of CB kx columns and
+// updates the modes in registers, so the y-transformed tendencies never go
+// to HBM.  Stage semantics as k2d::update_kernel (RK32D :183-200, CN2D
+// :215-225, compute_ut2D :841-846, dealiasing2D, projection2D); rows with
+// |ky| >= kcut hold the dealiased state's zeros and are left as they are.
+// ---------------------------------------------------------------------------
+// 8 points per thread in FP64 (two components are live across the second
+// transform), the column rules' block width capped at 512 threads (so 256
+// registers per thread are available)
+template <typename T, int N>
+struct Fwd2Cfg {
+    using Base = ColCfg<T, N, sizeof(T) == 8 ? 8 : 16>;
+    static constexpr int E = Base::E, P = Base::P;
+    static constexpr int CB = Base::CB * P > 512 ? (P >= 512 ? 1 : 512 / P) : Base::CB;
+    static constexpr int THREADS = CB * P;
+};
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((Fwd2Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+void cf2_kernel(k2d::ColFwdArgs<T> a) {
+    using C = Fwd2Cfg<T, N>;
+    constexpr int E = C::E, P = C::P, CB = C::CB;
+    if (a.gate.closed()) return;
+    __shared__ T lds[CB * lds_line(N)];
+    const int blk = xcd_block(blockIdx.x, a.nkx / CB);
+    if (blk < 0) return;
+    const int tid = threadIdx.x;
+    const int l = tid % CB, j = tid / CB;
+    const int kx = blk * CB + l;
+    const LdsLine<T, true> L{lds + l, CB};
+    const size_t pitch = (size_t)a.pitch;
+    const T sc = (T)a.g.inv_n;
+    const int kcut = a.kcut > 0 ? a.kcut : N;
+    cx<T> v[2][E];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[c][k] = a.nx[c][(size_t)(j + k * P) * pitch + kx];
+        line_fft<T, N, E, false, true>(v[c], j, L, a.tw);
+    }
+    const T dt = (T)a.g.dt;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        if (k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+        const int m = j + k * P;
+        const int ky = ksigned(m, N);
+        if ((ky < 0 ? -ky : ky) >= kcut) continue;
+        const int kk2 = kx * kx + ky * ky;
+        const size_t idx = (size_t)m * pitch + kx;
+        cx<T> t0 = cscale(v[0][k], sc), t1 = cscale(v[1][k], sc);
+        const cx<T> U0 = a.U[0][idx], U1 = a.U[1][idx];
+        cx<T> s0, s1, ka0, ka1;
+        if (MODE == k2d::UP_A) {
+            s0 = U0; s1 = U1;
+        } else if (MODE == k2d::UP_CN_MID || MODE == k2d::UP_CN_LAST) {
+            if (a.cn_first) { s0 = U0; s1 = U1; }
+            else { s0 = a.UB[0][idx]; s1 = a.UB[1][idx]; }
+        } else if (MODE == k2d::UP_B) {
+            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+            s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
+            s1 = cadd(U1, cscale(cscale(ka1, dt), (T)0.5));
+        } else {
+            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+            s0 = a.UB[0][idx]; s1 = a.UB[1][idx];
+        }
+        const T vis = (T)(-a.g.nu * (double)kk2);
+        t0 = cadd(t0, cscale(s0, vis));
+        t1 = cadd(t1, cscale(s1, vis));
+        if (kk2 >= a.g.cut2) { t0 = {0, 0}; t1 = {0, 0}; }
+        if (a.g.lproject) {
+            if (kk2 > 0) {
+                const T kkr = sqrt((T)kk2);
+                const T c1 = (T)kx / kkr, c2 = (T)ky / kkr;
+                if (2 * m == N) {
+                    // y-Nyquist row: the partner of (kx, +N/2) in the reference's
+                    // c2c projection is (-kx, +N/2); the kx*ky cross terms cancel
+                    t0 = cscale(cscale(t0, c1), c1);
+                    t1 = cscale(cscale(t1, c2), c2);
+                } else {
+                    const cx<T> ud = cadd(cscale(t0, c1), cscale(t1, c2));
+                    t0 = cscale(ud, c1);
+                    t1 = cscale(ud, c2);
+                }
+            } else {
+                t0 = {0, 0}; t1 = {0, 0};
+            }
+        }
+        if (MODE == k2d::UP_CN_MID) {
+            const cx<T> n0 = cadd(U0, cscale(t0, dt)), n1 = cadd(U1, cscale(t1, dt));
+            a.UB[0][idx] = cscale(cadd(n0, U0), (T)0.5);
+            a.UB[1][idx] = cscale(cadd(n1, U1), (T)0.5);
+        } else if (MODE == k2d::UP_CN_LAST) {
+            a.U[0][idx] = cadd(U0, cscale(t0, dt));
+            a.U[1][idx] = cadd(U1, cscale(t1, dt));
+        } else if (MODE == k2d::UP_A) {
+            a.KA[0][idx] = t0; a.KA[1][idx] = t1;
+            a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
+            a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
+        } else if (MODE == k2d::UP_B) {
+            const T dt2 = (T)2 * dt;
+            a.UB[0][idx] = cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2));
+            a.UB[1][idx] = cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2));
+            a.KA[0][idx] = cadd(ka0, cscale(t0, (T)4));
+            a.KA[1][idx] = cadd(ka1, cscale(t1, (T)4));
+        } else {
+            const T r6 = (T)(1.0 / 6.0);
+            a.U[0][idx] = cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6));
+            a.U[1][idx] = cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6));
+        }
+    }
+}
+
 template <typename T, int N, typename R>
 __device__ __forceinline__ cx<T> pack1(const R& A, int m) {
     const int h = m <= N / 2 ? m : N - m;
@@ -880,6 +994,37 @@ static int cf3_n(int mode, const Cf3Args<T>& a, hipStream_t s) {
     }
     return 0;
 }
+template <typename T, int N>
+static int cf2_n(int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s) {
+    using C = Fwd2Cfg<T, N>;
+    if (a.nkx < C::CB || a.nkx % C::CB || a.nkx > a.pitch || a.g.n1 != N) return -1;
+    dim3 grid(k2d::xcd_grid_h(a.nkx / C::CB)), block(C::THREADS);
+    switch (mode) {
+#define M(MM) case MM: hipLaunchKernelGGL((cf2_kernel<T, N, MM>), grid, block, 0, s, a); break;
+        M(k2d::UP_A) M(k2d::UP_B) M(k2d::UP_C) M(k2d::UP_CN_MID) M(k2d::UP_CN_LAST)
+#undef M
+        default: return -1;
+    }
+    return 0;
+}
+template <typename T>
+bool cf2_full_rows(int n) {
+    switch (n) {
+#define X(NN) case NN: return Fwd2Cfg<T, NN>::CB == Fwd2Cfg<T, NN>::Base::CB;
+        COLX_SIZES(X)
+#undef X
+    }
+    return false;
+}
+template <typename T>
+int launch_cf2(int n, int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s) {
+    switch (n) {
+#define X(NN) case NN: return cf2_n<T, NN>(mode, a, s);
+        COLX_SIZES(X)
+#undef X
+    }
+    return -1;
+}
 template <typename T>
 int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s) {
     switch (n) {
@@ -931,6 +1076,8 @@ int launch_scale3(const Scale3Args<T>& a, hipStream_t s) {
 
 #define K3D_INSTANTIATE(T)                                                             \
     template int launch_colx<T>(int, bool, const ColXArgs<T>&, hipStream_t);           \
+    template int launch_cf2<T>(int, int, const k2d::ColFwdArgs<T>&, hipStream_t);      \
+    template bool cf2_full_rows<T>(int);                                               \
     template int launch_cf3<T>(int, int, const Cf3Args<T>&, hipStream_t);              \
     template int launch_row3<T>(int, int, const Row3Args<T>&, int, hipStream_t);       \
     template int launch_vg3<T>(int, const Vg3Args<T>&, int, hipStream_t, int);         \
