@@ -31,8 +31,19 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
         n = a.nblk_vg;
         is_min = v == 7;
     }
-    double acc = is_min ? (double)(2.0f * 3.14f) : 0.0;
-    for (int w = tid; w < n; w += 256) acc = is_min ? fmin(acc, src[w]) : acc + src[w];
+    // each thread folds src[tid], src[tid + 256], ... in order; the loads of
+    // eight of them are issued together (a 3-D velgrad moment has one partial
+    // per row block: 65536 at 512^3)
+    const double none = is_min ? (double)(2.0f * 3.14f) : 0.0;
+    double acc = none;
+    for (int w0 = tid; w0 < n; w0 += 256 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = w0 + u * 256 < n ? src[w0 + u * 256] : none;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (w0 + u * 256 < n) acc = is_min ? fmin(acc, v[u]) : acc + v[u];
+    }
     sh[tid] = acc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
