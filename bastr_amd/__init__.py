@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 from . import abi
-from .solver import BastrError, Solver, diag_table, make_params  # noqa: F401
+from .solver import BastrError, Ensemble, Solver, diag_table, make_params  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbastr_hip.so")
@@ -22,7 +22,7 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} missing (run __graft_entry__.build())")
         L = C.CDLL(LIB_PATH)
-        abi.bind(L, "bastr")
+        abi.bind(L, "bastr", ensemble=True)
         _LIB = L
     return _LIB
 
@@ -30,3 +30,9 @@ def load():
 def solver(params):
     """A HIP-backed solver handle."""
     return Solver(load(), "bastr", params)
+
+
+def ensemble(params, nmembers, forced=None):
+    """A HIP-backed ensemble of nmembers independent 2-D runs on one grid;
+    forced: None or nmembers flags (0 = free decay, 1 = forced as params say)."""
+    return Ensemble(load(), params, nmembers, forced)

@@ -73,9 +73,20 @@ SIGNATURES = [
 ]
 
 
-def bind(lib, prefix):
+# Ensemble entry points (HIP library only: the oracle is a one-field restatement).
+ENSEMBLE_SIGNATURES = [
+    ("create_ensemble", C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_int32), C.POINTER(_P)]),
+    ("nmembers", C.c_int, [_P]),
+    ("set_member_fields", C.c_int, [_P, C.c_int32, _D, _D, _D]),
+    ("get_member_fields", C.c_int, [_P, C.c_int32, _D, _D, _D]),
+    ("init_random_phase_member", C.c_int, [_P, C.c_int32, C.c_double, C.c_uint64, C.c_double]),
+    ("get_member_time", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+]
+
+
+def bind(lib, prefix, ensemble=False):
     """Attach argtypes/restype for every entry point; raise if one is missing."""
-    for name, res, args in SIGNATURES:
+    for name, res, args in SIGNATURES + (ENSEMBLE_SIGNATURES if ensemble else []):
         fn = getattr(lib, f"{prefix}_{name}")  # AttributeError = missing export
         fn.restype = res
         fn.argtypes = args

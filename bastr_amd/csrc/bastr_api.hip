@@ -52,7 +52,9 @@ int fail(int code, const char* fmt, ...) {
 
 bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
-__global__ void set_int_kernel(int* p, int v) { *p = v; }
+__global__ void set_ints_kernel(int* p, int n, int v) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
 
 __global__ void fill_diag_kernel(bastr_diag* d, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +127,17 @@ struct bastr_handle {
     double* part_en = nullptr;   // energy partials (forcing methods 1, 5)
     double* dfactor = nullptr;   // forcing method 1 factor
     int nblk_spec = 0, nblk_vg = 0, nblk_en = 0;
-    int* status = nullptr;
+    int* status = nullptr;       // [nmem]
+    // Ensemble: nmem independent 2-D members of this grid, stacked member-major
+    // in every field buffer, partial-sum buffer and diagnostic slot array;
+    // every launch of the step covers all of them (blockIdx.z = member).  lw is
+    // the current launch width: nmem, or 1 while a MemberView points the
+    // buffers at one member.
+    int nmem = 1, lw = 1;
+    int* mforce = nullptr;            // [nmem] device: member is band-forced (null: all are)
+    std::vector<char> mset;           // member's field has been set
+    std::vector<int64_t> mnstep;      // per-member nstep / time (ensembles)
+    std::vector<double> mtime;
     bastr_diag* ddiag = nullptr;
     double* dspec = nullptr;
     int cap = 0;
@@ -231,8 +243,58 @@ Geo<T> geo(const bastr_handle* h) {
     g.nu = h->nu;
     g.dt = h->p.deltat;
     g.inv_n = 1.0 / ((double)h->n0 * (double)h->n1);
+    g.hstride = (long)h->nhalf;
+    g.rstride = (long)h->npts;
+    g.nmem = h->lw;
     return g;
 }
+
+// every member's NaN gate open (status = INT_MAX)
+int reset_status(bastr_handle* h) {
+    hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(256), 0, h->stream, h->status, h->lw, INT_MAX);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Points a handle's per-member buffers at member m of an ensemble and
+// narrows its launches to that member, for the one-member host operations
+// (set/get fields, the random-phase generator); restores them on exit.
+struct MemberView {
+    bastr_handle* h;
+    void *U[2], *UB[2], *KA[2], *NX[2], *ureal[2], *W[7];
+    int* status;
+    double *part_en, *dfactor;
+    int lw;
+    MemberView(bastr_handle* h_, int m) : h(h_) {
+        const size_t co = (size_t)m * h->nhalf * 2 * h->esz, ro = (size_t)m * h->npts * h->esz;
+        auto mv = [](void*& p, size_t o) { p = (void*)((char*)p + o); };
+        for (int c = 0; c < 2; ++c) {
+            U[c] = h->U[c]; UB[c] = h->UB[c]; KA[c] = h->KA[c]; NX[c] = h->NX[c]; ureal[c] = h->ureal[c];
+            mv(h->U[c], co); mv(h->UB[c], co); mv(h->KA[c], co); mv(h->NX[c], co); mv(h->ureal[c], ro);
+        }
+        for (int q = 0; q < 7; ++q) {
+            W[q] = h->W[q];
+            mv(h->W[q], co);
+        }
+        status = h->status;
+        part_en = h->part_en;
+        dfactor = h->dfactor;
+        lw = h->lw;
+        h->status += m;
+        h->part_en += (size_t)m * h->nblk_en;
+        h->dfactor += m;
+        h->lw = 1;
+    }
+    ~MemberView() {
+        for (int c = 0; c < 2; ++c) {
+            h->U[c] = U[c]; h->UB[c] = UB[c]; h->KA[c] = KA[c]; h->NX[c] = NX[c]; h->ureal[c] = ureal[c];
+        }
+        for (int q = 0; q < 7; ++q) h->W[q] = W[q];
+        h->status = status;
+        h->part_en = part_en;
+        h->dfactor = dfactor;
+        h->lw = lw;
+    }
+};
 
 // x-pass limit of the step's passes (kx < cut when dealias-limited)
 int step_kxlim(const bastr_handle* h) { return h->prune ? h->cut : h->hp; }
@@ -280,6 +342,8 @@ int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode, 
     c.nplanes = 1;
     c.scale = 1.0;
     c.tw = C<T>(h->tw1);
+    c.mstride = (long)h->nhalf;
+    c.nmem = h->lw;
     LAUNCHCHK(k3d::launch_colx<T>(h->n1, true, c, h->stream));
     return 0;
 }
@@ -304,6 +368,8 @@ int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
     c.nplanes = 1;
     c.scale = scale;
     c.tw = (const cx<T>*)tw;
+    c.mstride = (long)h->nhalf;
+    c.nmem = h->lw;
     LAUNCHCHK(k3d::launch_colx<T>(h->n1, false, c, h->stream));
     return 0;
 }
@@ -373,8 +439,7 @@ int run_prologue(bastr_handle* h) {
     r.hout[0] = C<T>(h->NX[0]);
     r.hout[1] = C<T>(h->NX[1]);
     r.tw = C<T>(h->tw0);
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
     void* nx[2] = {h->NX[0], h->NX[1]};
     void* w[2] = {h->W[0], h->W[1]};
@@ -387,8 +452,7 @@ int run_prologue(bastr_handle* h) {
 template <typename T>
 int run_to_physical(bastr_handle* h) {
     const Gate open{h->status, 0, 0};
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     const void* Us[2] = {h->U[0], h->U[1]};
     int rc = run_yinv2<T>(h, open, Us, 0, false);
     if (rc) return rc;
@@ -505,6 +569,9 @@ int run_iteration(bastr_handle* h, int it) {
     fa.diag = h->ddiag + it;
     fa.spec = h->dspec + (size_t)it * 3 * h->allkmax;
     fa.status = h->status;
+    fa.nmem = h->lw;
+    fa.dstride = h->cap;
+    fa.sstride = (long)h->cap * 3 * h->allkmax;
     {
         ProfScope ps(h, BASTR_K_FINALIZE);
         LAUNCHCHK(launch_finalize(fa, h->stream));
@@ -547,6 +614,8 @@ int run_iteration(bastr_handle* h, int it) {
         f.nblk_en = h->nblk_en;
         f.factor_out = h->dfactor;
         f.diag = h->ddiag + it;
+        f.dstride = h->cap;
+        f.mforce = h->mforce;
         LAUNCHCHK(launch_force_band<T>(f, h->stream));
         if (fm == 1) {
             ScaleArgs<T> sc{};
@@ -555,6 +624,7 @@ int run_iteration(bastr_handle* h, int it) {
             sc.U[0] = C<T>(h->U[0]);
             sc.U[1] = C<T>(h->U[1]);
             sc.factor = h->dfactor;
+            sc.mforce = h->mforce;
             LAUNCHCHK(launch_scale<T>(sc, h->stream));
         }
     }
@@ -746,8 +816,7 @@ int run_x3(bastr_handle* h, const Gate& g) {
 template <typename T>
 int run_prologue3(bastr_handle* h) {
     const Gate open{h->status, 0, 0};
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     k3d::Row3Args<T> r{};
     r.g = geo3<T>(h);
     r.gate = open;
@@ -765,8 +834,7 @@ int run_prologue3(bastr_handle* h) {
 template <typename T>
 int run_to_physical3(bastr_handle* h) {
     const Gate open{h->status, 0, 0};
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     int rc;
     if ((rc = run_zinv3<T>(h, open, h->U, 0, false))) return rc;
     if ((rc = run_yinv3<T>(h, open, 0, false, 3))) return rc;  // into W[3..5]; ureal aliases W[0..2]
@@ -921,11 +989,50 @@ int ensure_capacity(bastr_handle* h, int n) {
     if (h->dspec) (void)hipFree(h->dspec);
     h->ddiag = nullptr;
     h->dspec = nullptr;
-    HIPCHK(hipMalloc(&h->ddiag, sizeof(bastr_diag) * n));
-    HIPCHK(hipMalloc(&h->dspec, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * n));
-    HIPCHK(hipMemset(h->dspec, 0, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * n));
+    const size_t slots = (size_t)n * h->nmem;  // member-major: member m's slots start at m*n
+    HIPCHK(hipMalloc(&h->ddiag, sizeof(bastr_diag) * slots));
+    HIPCHK(hipMalloc(&h->dspec, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * slots));
+    HIPCHK(hipMemset(h->dspec, 0, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * slots));
     h->cap = n;
     return 0;
+}
+
+// Ensemble results of a step call: member m's diagnostics go to
+// diags[m*nsteps + it] and its spectra to spectra[(m*nsteps + it)*3*allkmax];
+// a member whose energy went NaN at iteration s has s+1 iterations of
+// diagnostics (the rest keep the fill values) and advances s iterations.
+int collect_ensemble(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
+    const int M = h->nmem, K = h->allkmax;
+    std::vector<int> status(M, INT_MAX);
+    HIPCHK(hipMemcpyAsync(status.data(), h->status, sizeof(int) * M, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    prof_collect(h);
+    bool any_nan = false;
+    for (int m = 0; m < M; ++m) {
+        const int done = status[m] == INT_MAX ? nsteps : status[m] + 1;
+        any_nan = any_nan || status[m] != INT_MAX;
+        if (diags) {
+            HIPCHK(hipMemcpy(diags + (size_t)m * nsteps, h->ddiag + (size_t)m * h->cap, sizeof(bastr_diag) * nsteps,
+                             hipMemcpyDeviceToHost));
+        }
+        if (spectra && K > 0) {
+            double* sp = spectra + (size_t)m * nsteps * 3 * K;
+            HIPCHK(hipMemcpy(sp, h->dspec + (size_t)m * h->cap * 3 * K, sizeof(double) * 3 * K * (size_t)nsteps,
+                             hipMemcpyDeviceToHost));
+            for (int it = 0; it < nsteps; ++it)
+                for (int b = 1; b <= K; ++b) sp[(size_t)it * 3 * K + b - 1] = h->kn[b];
+        }
+        for (int it = 0; it < done; ++it) {
+            if (diags) {
+                diags[(size_t)m * nsteps + it].nstep = h->mnstep[m];
+                diags[(size_t)m * nsteps + it].time = h->mtime[m];
+            }
+            if (it == status[m]) break;
+            h->mnstep[m] = h->mnstep[m] + 1;
+            h->mtime[m] = h->mtime[m] + h->p.deltat;
+        }
+    }
+    return any_nan ? BASTR_STOP_NAN : BASTR_OK;
 }
 
 template <typename T>
@@ -936,10 +1043,12 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
         h->prologue_pending = false;
     }
     if ((rc = ensure_capacity(h, nsteps))) return rc;
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(fill_diag_kernel, dim3((nsteps + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nsteps);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
+    {
+        const int nfill = h->nmem > 1 ? h->cap * h->nmem : nsteps;
+        hipLaunchKernelGGL(fill_diag_kernel, dim3((nfill + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nfill);
+        HIPCHK(hipGetLastError());
+    }
     // The ~25 launches of each iteration are replayed as one graph; the capture
     // is kept for the next call with the same step count (every argument is
     // fixed per handle: buffers, the iteration index within the call, the
@@ -970,6 +1079,7 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
         for (int it = 0; it < nsteps; ++it)
             if ((rc = (h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it)))) return rc;
     }
+    if (h->nmem > 1) return collect_ensemble(h, nsteps, diags, spectra);
     int status = INT_MAX;
     HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1003,10 +1113,16 @@ extern "C" {
 
 const char* bastr_last_error(void) { return g_err; }
 
-int bastr_create(const bastr_params* p, bastr_handle** out) {
+static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr_handle** out) {
     if (!p || !out) return fail(BASTR_EINVAL, "bastr_create: NULL argument");
     *out = nullptr;
     const int nd = p->ka == 0 ? 2 : 3;
+    if (nmem < 1 || nmem > 65535) return fail(BASTR_EINVAL, "bastr_create_ensemble: nmembers %d not in [1, 65535]", nmem);
+    if (nmem > 1 && nd != 2) return fail(BASTR_EINVAL, "bastr_create_ensemble: ensembles are 2-D (ka = 0)");
+    if (forced)
+        for (int m = 0; m < nmem; ++m)
+            if (forced[m] != 0 && forced[m] != 1)
+                return fail(BASTR_EINVAL, "bastr_create_ensemble: forced[%d] = %d is not 0 or 1", m, forced[m]);
     if (nd == 2 && (!pow2_in(p->ia, 16, 4096) || !pow2_in(p->ja, 16, 4096)))
         return fail(BASTR_EINVAL, "bastr_create: ia, ja must be powers of two in [16, 4096] (got %d, %d)", p->ia, p->ja);
     if (nd == 3 && (!pow2_in(p->ia, 16, 512) || p->ja != p->ia || p->ka != p->ia))
@@ -1027,6 +1143,10 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     bastr_handle* h = new bastr_handle();
     h->p = *p;
     h->nd = nd;
+    h->nmem = h->lw = nmem;
+    h->mset.assign(nmem, 0);
+    h->mnstep.assign(nmem, 0);
+    h->mtime.assign(nmem, 0.0);
     h->n0 = p->ia;
     h->n1 = p->ja;
     h->n2 = nd == 3 ? p->ka : 1;
@@ -1068,14 +1188,15 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     bool ok = true;
     const size_t cbytes = h->nhalf * 2 * h->esz;
     if (nd == 2) {
+        const size_t M = (size_t)nmem;  // member-major stacks of every field
         for (int c = 0; c < 2; ++c) {
-            h->U[c] = dalloc(cbytes, &ok);
-            h->UB[c] = dalloc(cbytes, &ok);
-            h->KA[c] = dalloc(cbytes, &ok);
-            h->NX[c] = dalloc(cbytes, &ok);
-            h->ureal[c] = dalloc(h->npts * h->esz, &ok);
+            h->U[c] = dalloc(cbytes * M, &ok);
+            h->UB[c] = dalloc(cbytes * M, &ok);
+            h->KA[c] = dalloc(cbytes * M, &ok);
+            h->NX[c] = dalloc(cbytes * M, &ok);
+            h->ureal[c] = dalloc(h->npts * h->esz * M, &ok);
         }
-        for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes, &ok);
+        for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes * M, &ok);
     } else {
         for (int c = 0; c < 3; ++c) {
             h->U[c] = dalloc(cbytes, &ok);
@@ -1098,20 +1219,24 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
         const int L = vg_block_rows(h->n0, h->prec, nd);
         h->nblk_vg = (h->n1 * h->n2 + L - 1) / L;
     }
-    h->part_bins = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * (h->allkmax + 1) * 2, &ok);
-    h->part_tot = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * 4, &ok);
-    h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8, &ok);
-    h->status = (int*)dalloc(sizeof(int), &ok);
-    h->red = (double*)dalloc(sizeof(double) * (2 * (size_t)h->allkmax + 12), &ok);
+    const size_t M = (size_t)nmem;
+    h->part_bins = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * (h->allkmax + 1) * 2 * M, &ok);
+    h->part_tot = (double*)dalloc(sizeof(double) * (size_t)h->nblk_spec * 4 * M, &ok);
+    h->part_vg = (double*)dalloc(sizeof(double) * (size_t)h->nblk_vg * 8 * M, &ok);
+    h->status = (int*)dalloc(sizeof(int) * M, &ok);
+    h->red = (double*)dalloc(sizeof(double) * (2 * (size_t)h->allkmax + 12) * M, &ok);
     h->nblk_en = 512;
-    h->part_en = (double*)dalloc(sizeof(double) * h->nblk_en, &ok);
-    h->dfactor = (double*)dalloc(sizeof(double), &ok);
+    h->part_en = (double*)dalloc(sizeof(double) * h->nblk_en * M, &ok);
+    h->dfactor = (double*)dalloc(sizeof(double) * M, &ok);
+    if (forced && nmem > 1) h->mforce = (int*)dalloc(sizeof(int) * M, &ok);
     if (!ok) {
         bastr_destroy(h);
         return fail(BASTR_ENOMEM, "bastr_create: device allocation failed");
     }
     int rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n0, &h->tw0) : upload_twiddles<double>(h->n0, &h->tw0);
     if (!rc) rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n1, &h->tw1) : upload_twiddles<double>(h->n1, &h->tw1);
+    if (!rc && h->mforce && hipMemcpy(h->mforce, forced, sizeof(int) * nmem, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(BASTR_EDEVICE, "bastr_create_ensemble: copy of the forcing flags failed");
     if (rc) {
         bastr_destroy(h);
         return rc;
@@ -1138,11 +1263,19 @@ int bastr_create(const bastr_params* p, bastr_handle** out) {
     return BASTR_OK;
 }
 
+int bastr_create(const bastr_params* p, bastr_handle** out) { return create_impl(p, 1, nullptr, out); }
+
+int bastr_create_ensemble(const bastr_params* p, int32_t nmembers, const int32_t* forced, bastr_handle** out) {
+    return create_impl(p, nmembers, forced, out);
+}
+
+int bastr_nmembers(const bastr_handle* h) { return h ? h->nmem : -1; }
+
 int bastr_destroy(bastr_handle* h) {
     if (!h) return BASTR_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf, h->part_bins, h->part_tot,
-                    h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec};
+                    h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec, h->mforce};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int c = 0; c < 3; ++c) {
@@ -1173,6 +1306,7 @@ int bastr_allkmax(const bastr_handle* h) { return h ? h->allkmax : -1; }
 
 int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const double* u3) {
     if (!h || !u1 || !u2 || (h && h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_set_fields: NULL argument");
+    if (h->lw > 1) return fail(BASTR_EINVAL, "bastr_set_fields: an ensemble handle takes the bastr_*_member entry points");
     const double* src[3] = {u1, u2, u3};
     for (int c = 0; c < h->nd; ++c) {
         if (h->prec == BASTR_PREC_F64) {
@@ -1197,6 +1331,7 @@ static int prologue_any(bastr_handle* h) {
 
 int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3) {
     if (!h || !u1 || !u2 || (h && h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_get_fields: NULL argument");
+    if (h->lw > 1) return fail(BASTR_EINVAL, "bastr_get_fields: an ensemble handle takes the bastr_*_member entry points");
     if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_get_fields: fields not set");
     int rc;
     if (h->prologue_pending) {
@@ -1226,8 +1361,7 @@ int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3) {
 template <typename T>
 static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, double energy) {
     const Gate open{h->status, 0, 0};
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     // 1. noise in the first physical field, zeros in the others
     LAUNCHCHK(rp::launch_noise<T>(h->ureal[0], h->npts, seed, h->stream));
     for (int c = 1; c < h->nd; ++c) LAUNCHCHK(rp::launch_zero<T>(h->ureal[c], h->npts, h->stream));
@@ -1325,6 +1459,7 @@ extern "C" {
 
 int bastr_init_random_phase(bastr_handle* h, double k0, uint64_t seed, double energy) {
     if (!h) return fail(BASTR_EINVAL, "bastr_init_random_phase: NULL handle");
+    if (h->lw > 1) return fail(BASTR_EINVAL, "bastr_init_random_phase: an ensemble handle takes the bastr_*_member entry points");
     if (!(k0 > 0) || !(energy > 0)) return fail(BASTR_EINVAL, "bastr_init_random_phase: k0 and energy must be > 0");
     int rc = h->prec == BASTR_PREC_F32 ? init_random_phase_impl<float>(h, k0, seed, energy)
                                        : init_random_phase_impl<double>(h, k0, seed, energy);
@@ -1340,13 +1475,91 @@ int bastr_set_time(bastr_handle* h, int64_t nstep, double time) {
     if (!h) return fail(BASTR_EINVAL, "bastr_set_time: NULL handle");
     h->nstep = nstep;
     h->time = time;
+    for (int m = 0; m < h->nmem; ++m) {
+        h->mnstep[m] = nstep;
+        h->mtime[m] = time;
+    }
     return BASTR_OK;
 }
 
 int bastr_get_time(const bastr_handle* h, int64_t* nstep, double* time) {
     if (!h) return fail(BASTR_EINVAL, "bastr_get_time: NULL handle");
-    if (nstep) *nstep = h->nstep;
-    if (time) *time = h->time;
+    if (nstep) *nstep = h->nmem > 1 ? h->mnstep[0] : h->nstep;
+    if (time) *time = h->nmem > 1 ? h->mtime[0] : h->time;
+    return BASTR_OK;
+}
+
+// One member of an ensemble handle through the one-field entry points: the
+// handle's buffers point at the member for the call, and the member's spectral
+// state is formed at once (the entry dealias of mainloop2D :41-52).
+static int member_check(bastr_handle* h, int member, const char* what) {
+    if (!h) return fail(BASTR_EINVAL, "%s: NULL handle", what);
+    if (member < 0 || member >= h->nmem)
+        return fail(BASTR_EINVAL, "%s: member %d not in [0, %d)", what, member, h->nmem);
+    return 0;
+}
+
+static void member_done(bastr_handle* h, int member) {
+    h->mset[member] = 1;
+    h->have_fields = std::all_of(h->mset.begin(), h->mset.end(), [](char c) { return c != 0; });
+}
+
+int bastr_set_member_fields(bastr_handle* h, int32_t member, const double* u1, const double* u2, const double* u3) {
+    int rc = member_check(h, member, "bastr_set_member_fields");
+    if (rc) return rc;
+    if (h->nmem == 1) return bastr_set_fields(h, u1, u2, u3);
+    {
+        MemberView v(h, member);
+        rc = bastr_set_fields(h, u1, u2, u3);
+        if (!rc) rc = prologue_any(h);
+        h->prologue_pending = false;
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
+            rc = fail(BASTR_EDEVICE, "bastr_set_member_fields: stream synchronisation failed");
+    }
+    if (rc) return rc;
+    member_done(h, member);
+    return BASTR_OK;
+}
+
+int bastr_get_member_fields(bastr_handle* h, int32_t member, double* u1, double* u2, double* u3) {
+    int rc = member_check(h, member, "bastr_get_member_fields");
+    if (rc) return rc;
+    if (h->nmem == 1) return bastr_get_fields(h, u1, u2, u3);
+    if (!h->mset[member]) return fail(BASTR_EINVAL, "bastr_get_member_fields: member %d not set", member);
+    MemberView v(h, member);
+    const bool hf = h->have_fields;
+    h->have_fields = true;
+    rc = bastr_get_fields(h, u1, u2, u3);
+    h->have_fields = hf;
+    return rc;
+}
+
+int bastr_init_random_phase_member(bastr_handle* h, int32_t member, double k0, uint64_t seed, double energy) {
+    int rc = member_check(h, member, "bastr_init_random_phase_member");
+    if (rc) return rc;
+    if (h->nmem == 1) return bastr_init_random_phase(h, k0, seed, energy);
+    {
+        MemberView v(h, member);
+        rc = bastr_init_random_phase(h, k0, seed, energy);
+        if (!rc) rc = prologue_any(h);
+        h->prologue_pending = false;
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
+            rc = fail(BASTR_EDEVICE, "bastr_init_random_phase_member: stream synchronisation failed");
+    }
+    if (rc) return rc;
+    h->mnstep[member] = 0;
+    h->mtime[member] = 0.0;
+    member_done(h, member);
+    return BASTR_OK;
+}
+
+int bastr_get_member_time(const bastr_handle* h, int32_t member, int64_t* nstep, double* time) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_get_member_time: NULL handle");
+    if (member < 0 || member >= h->nmem)
+        return fail(BASTR_EINVAL, "bastr_get_member_time: member %d not in [0, %d)", member, h->nmem);
+    if (h->nmem == 1) return bastr_get_time(h, nstep, time);
+    if (nstep) *nstep = h->mnstep[member];
+    if (time) *time = h->mtime[member];
     return BASTR_OK;
 }
 
@@ -1407,6 +1620,7 @@ static int fft_c2c3(bastr_handle* h, bool inverse, const cx<double>* tw) {
 }
 
 static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
+    if (h && h->nmem > 1) return fail(BASTR_EINVAL, "bastr_fft_*: not available on an ensemble handle");
     if (!h || !inout) return fail(BASTR_EINVAL, "bastr_fft: NULL argument");
     const size_t bytes = sizeof(double) * 2 * h->npts;
     if (!h->cbuf) HIPCHK(hipMalloc(&h->cbuf, bytes));
@@ -1422,8 +1636,7 @@ static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
         tw1 = (const cx<double>*)h->tw1d;
     }
     HIPCHK(hipMemcpyAsync(h->cbuf, inout, bytes, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, h->status, INT_MAX);
-    HIPCHK(hipGetLastError());
+    if (reset_status(h)) return fail(BASTR_EDEVICE, "status reset launch failed");
     const Gate open{h->status, 0, 0};
     Geo<double> g = geo<double>(h);
     cx<double>* z = (cx<double>*)h->cbuf;
@@ -1515,7 +1728,8 @@ int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* tot
         units[BASTR_K_COL_FWD_B] = 3 * fc + 6 * fc * fy + 12 * fc * fyz;
         units[BASTR_K_COL_FWD_C] = 3 * fc + 6 * fc * fy + 12 * fc * fyz;
     }
-    const double half = (double)h->nh * h->n1 * h->n2 * 2.0 * h->esz;
+    // (an ensemble launch covers every member)
+    const double half = (double)h->nh * h->n1 * h->n2 * 2.0 * h->esz * h->nmem;
     if (launches) *launches = h->prof_launch[c];
     if (total_ms) *total_ms = h->prof_ms[c];
     if (bytes) *bytes = units[c] * half;
@@ -1805,6 +2019,7 @@ int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const 
                      double* totals) {
     (void)u3;
     if (!h || !u1 || !u2 || !shells || !totals) return fail(BASTR_EINVAL, "bastr_pp_spectra: NULL argument");
+    if (h->nmem > 1) return fail(BASTR_EINVAL, "bastr_pp_spectra: not available on an ensemble handle");
     if (h->nd != 2) return fail(BASTR_EINVAL, "bastr_pp_spectra: spectra2d is 2-D only");
     int rc = pp_arena(h);
     if (rc) return rc;
@@ -1815,6 +2030,7 @@ int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const 
 
 int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* out) {
     if (!h || !u1 || !u2 || !out || (h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_pp_transfer: NULL argument");
+    if (h->nmem > 1) return fail(BASTR_EINVAL, "bastr_pp_transfer: not available on an ensemble handle");
     int rc = pp_arena(h);
     if (rc) return rc;
     const double* src[3] = {u1, u2, u3};

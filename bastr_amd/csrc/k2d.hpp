@@ -75,17 +75,25 @@ struct Geo {
     int lproject;
     int kxlim;       // x passes: half-row entries kx >= kxlim are zero (not read, not written)
     double nu, dt, inv_n;
+    // ensemble members: blockIdx.z is the member; member m's fields start
+    // m*hstride complex (half spectra) or m*rstride real (physical) elements
+    // after member 0's
+    long hstride, rstride;
+    int nmem;        // launch width along z (0 or 1: one member)
 };
+
+__host__ __device__ __forceinline__ unsigned zdim(int nmem) { return nmem > 1 ? (unsigned)nmem : 1u; }
 
 // Iteration gate: kernels of the diagnostic phase of iteration `it` run while
 // no NaN was found in an earlier iteration; kernels of the step phase also
-// stop for the iteration whose diagnostics were NaN (solution.F90:81).
+// stop for the iteration whose diagnostics were NaN (solution.F90:81).  Each
+// ensemble member (blockIdx.z) has its own status word.
 struct Gate {
-    const int* status;  // first iteration index with NaN energy, INT_MAX if none
+    const int* status;  // [member] first iteration index with NaN energy, INT_MAX if none
     int it;
     int step_phase;
     __device__ __forceinline__ bool closed() const {
-        const int s = *status;
+        const int s = status[blockIdx.z];
         return step_phase ? (s <= it) : (s < it);
     }
 };
@@ -160,6 +168,10 @@ struct FinArgs {
     bastr_diag* diag;   // slot of this iteration
     double* spec;       // [3][allkmax] slot of this iteration (Es at +allkmax, Ed at +2*allkmax)
     int* status;
+    // ensemble members (blockIdx.z): member m's partials, sums, diagnostics
+    // and spectra start m*dstride diag slots / m*sstride doubles after member 0's
+    int nmem;
+    long dstride, sstride;
 };
 
 template <typename T>
@@ -172,8 +184,10 @@ struct ForceArgs {
     double target_energy;
     const double* part_en;    // [nblk_en] partial sums of |u_hat|^2 over all modes (methods 1, 5)
     int nblk_en;
-    double* factor_out;       // method 1: the factor for scale_kernel
+    double* factor_out;       // method 1: the factor for scale_kernel ([member])
     bastr_diag* diag;
+    long dstride;             // diag slots between members
+    const int* mforce;        // [member] 0: this member is not forced (null: every member is)
 };
 
 template <typename T>
@@ -189,7 +203,8 @@ struct ScaleArgs {
     Geo<T> g;
     Gate gate;
     cx<T>* U[2];
-    const double* factor;
+    const double* factor;     // [member]
+    const int* mforce;        // [member] 0: this member is not forced (null: every member is)
 };
 
 // launchers (k2d.hip); n is the transform length along the pass

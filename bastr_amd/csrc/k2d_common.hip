@@ -14,6 +14,13 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
     if (a.gate.closed()) return;
     __shared__ double sh[256];
     const int K = a.allkmax;
+    {  // ensemble member blockIdx.z: its partials and reduced sums
+        const size_t m = blockIdx.z;
+        a.part_bins += m * (size_t)a.nblk_spec * (K + 1) * 2;
+        a.part_tot += m * (size_t)a.nblk_spec * 4;
+        a.part_vg += m * (size_t)a.nblk_vg * 8;
+        a.red += m * (2 * (size_t)K + 12);
+    }
     const int q = blockIdx.x, tid = threadIdx.x;
     const double* src;
     int n;
@@ -58,6 +65,13 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
 __global__ void finalize_kernel(FinArgs a) {
     if (a.gate.closed()) return;
     const int K = a.allkmax;
+    {  // ensemble member blockIdx.z
+        const size_t m = blockIdx.z;
+        a.red += m * (2 * (size_t)K + 12);
+        a.diag += m * (size_t)a.dstride;
+        a.spec += m * (size_t)a.sstride;
+        a.status += m;
+    }
     const double* r = a.red;
     for (int b = threadIdx.x; b < K; b += blockDim.x) {
         a.spec[K + b] = r[b];
@@ -92,8 +106,8 @@ __global__ void finalize_kernel(FinArgs a) {
 }
 
 int launch_finalize(const FinArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(2 * a.allkmax + 12), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(2 * a.allkmax + 12, 1, zdim(a.nmem)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1, 1, zdim(a.nmem)), dim3(256), 0, s, a);
     return 0;
 }
 

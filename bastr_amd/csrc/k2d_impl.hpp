@@ -21,6 +21,70 @@ namespace k2d {
 
 __device__ __forceinline__ int ksigned(int m, int n) { return m <= n / 2 ? m : m - n; }
 
+// Ensemble members: blockIdx.z selects the member; these move an argument
+// set's field pointers (and per-block partial sums, per-member scalars) from
+// member 0's to that member's.  Member 0 (every single-field launch) is
+// unchanged.
+template <typename T>
+__device__ __forceinline__ void at_member(ColFwdArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    for (int c = 0; c < 2; ++c) {
+        a.nx[c] += o;
+        a.U[c] += o;
+        a.UB[c] += o;
+        a.KA[c] += o;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void at_member(RowArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride, r = (size_t)blockIdx.z * (size_t)a.g.rstride;
+    for (int q = 0; q < 4; ++q) a.hin[q] += o;
+    for (int c = 0; c < 2; ++c) {
+        a.hout[c] += o;
+        a.rin[c] += r;
+        a.rout[c] += r;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void at_member(VgArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    for (int q = 0; q < 7; ++q) a.w[q] += o;
+    a.hout[0] += o;
+    a.hout[1] += o;
+    a.part += (size_t)blockIdx.z * 8 * gridDim.x;
+}
+template <typename T>
+__device__ __forceinline__ void at_member(SpecArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    a.U[0] += o;
+    a.U[1] += o;
+    a.part_bins += (size_t)blockIdx.z * (size_t)(a.allkmax + 1) * 2 * gridDim.x;
+    a.part_tot += (size_t)blockIdx.z * 4 * gridDim.x;
+}
+template <typename T>
+__device__ __forceinline__ void at_member(EnergyArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    a.U[0] += o;
+    a.U[1] += o;
+    a.part_en += (size_t)blockIdx.z * gridDim.x;
+}
+template <typename T>
+__device__ __forceinline__ void at_member(ScaleArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    a.U[0] += o;
+    a.U[1] += o;
+    a.factor += blockIdx.z;
+}
+template <typename T>
+__device__ __forceinline__ void at_member(ForceArgs<T>& a) {
+    const size_t o = (size_t)blockIdx.z * (size_t)a.g.hstride;
+    a.U[0] += o;
+    a.U[1] += o;
+    a.part_en += (size_t)blockIdx.z * a.nblk_en;
+    a.factor_out += blockIdx.z;
+    a.diag += (size_t)blockIdx.z * a.dstride;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
 // b % 8), each XCD with its own L2.  Column blocks share 128-byte lines with
 // their neighbours, so map consecutive column blocks to the same XCD:
@@ -170,6 +234,7 @@ __device__ __forceinline__ void unpack_store(cx<T>* v, int j, const LdsLine<T, f
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void update_kernel(ColFwdArgs<T> a) {
     if (a.gate.closed()) return;
+    at_member(a);
     const int kx = blockIdx.x * 256 + threadIdx.x;
     const int m = (int)blockIdx.y >= a.ky_skip_at ? (int)blockIdx.y + a.ky_skip : (int)blockIdx.y;
     if (kx >= a.nkx) return;
@@ -259,6 +324,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T>
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
+    if (MODE != R_C2C_FWD && MODE != R_C2C_INV) at_member(a);
     __shared__ cx<T> lds[LR * lds_line_cx(N)];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
@@ -366,6 +432,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
+    at_member(a);
     __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
@@ -437,6 +504,7 @@ void vg_dis_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
+    at_member(a);
     __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
     const int tid = threadIdx.x;
@@ -500,6 +568,7 @@ void vg_dis_kernel(VgArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     if (a.gate.closed()) return;
+    at_member(a);
     extern __shared__ double sh[];
     const int K = a.allkmax, nb = K + 1;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -603,6 +672,7 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
     if (a.gate.closed()) return;
+    at_member(a);
     __shared__ double red[4];
     const int n0 = a.g.n0, nh = a.g.nh;
     const size_t hp = (size_t)a.g.hp;
@@ -629,6 +699,8 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
     if (a.gate.closed()) return;
+    if (a.mforce && !a.mforce[blockIdx.z]) return;  // an unforced ensemble member
+    at_member(a);
     __shared__ double red[4];
     __shared__ double fac;
     const int tid = threadIdx.x;
@@ -714,6 +786,8 @@ __global__ __launch_bounds__(256) void force_band_kernel(ForceArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void scale_kernel(ScaleArgs<T> a) {
     if (a.gate.closed()) return;
+    if (a.mforce && !a.mforce[blockIdx.z]) return;  // an unforced ensemble member
+    at_member(a);
     const T f = (T)*a.factor;
     const size_t total = (size_t)a.g.hp * a.g.n1;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
@@ -732,7 +806,7 @@ static inline int xcd_grid_h(int nb) { return ((nb + 7) / 8) * 8; }
 template <typename T, int N>
 static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
     using C = RowCfg<T, N>;
-    dim3 grid((nrows + C::L - 1) / C::L), block(C::THREADS);
+    dim3 grid((nrows + C::L - 1) / C::L, 1, zdim(a.g.nmem)), block(C::THREADS);
     switch (mode) {
         case R_FWD_PACK: hipLaunchKernelGGL((row_kernel<T, N, R_FWD_PACK>), grid, block, 0, s, a); break;
         case R_INV_PACK: hipLaunchKernelGGL((row_kernel<T, N, R_INV_PACK>), grid, block, 0, s, a); break;
@@ -750,8 +824,8 @@ static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
     const int g = (nrows + C::L - 1) / C::L;
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
     if (!a.hout[0] || !a.hout[1]) return -1;
-    hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
-    hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
+    hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
@@ -759,7 +833,7 @@ template <typename T>
 int launch_update(int mode, const ColFwdArgs<T>& a, hipStream_t s) {
     const int last_row = a.nky - 1 >= a.ky_skip_at ? a.nky - 1 + a.ky_skip : a.nky - 1;
     if (a.nkx < 1 || a.nkx > a.g.nh || a.nky < 1 || a.ky_skip < 0 || last_row >= a.g.n1) return -1;
-    dim3 grid((a.nkx + 255) / 256, a.nky), block(256);
+    dim3 grid((a.nkx + 255) / 256, a.nky, zdim(a.g.nmem)), block(256);
     switch (mode) {
 #define M(MM) case MM: hipLaunchKernelGGL((update_kernel<T, MM>), grid, block, 0, s, a); break;
         M(UP_PROLOGUE) M(UP_A) M(UP_B) M(UP_C) M(UP_CN_MID) M(UP_CN_LAST)
@@ -789,22 +863,22 @@ int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nbl
 template <typename T>
 int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s) {
     const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
-    hipLaunchKernelGGL((spectra_kernel<T>), dim3(nblk), dim3(256), shm, s, a);
+    hipLaunchKernelGGL((spectra_kernel<T>), dim3(nblk, 1, zdim(a.g.nmem)), dim3(256), shm, s, a);
     return 0;
 }
 template <typename T>
 int launch_force_band(const ForceArgs<T>& a, hipStream_t s) {
-    hipLaunchKernelGGL((force_band_kernel<T>), dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((force_band_kernel<T>), dim3(1, 1, zdim(a.g.nmem)), dim3(256), 0, s, a);
     return 0;
 }
 template <typename T>
 int launch_energy(const EnergyArgs<T>& a, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((energy_kernel<T>), dim3(nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((energy_kernel<T>), dim3(nblk, 1, zdim(a.g.nmem)), dim3(256), 0, s, a);
     return 0;
 }
 template <typename T>
 int launch_scale(const ScaleArgs<T>& a, hipStream_t s) {
-    hipLaunchKernelGGL((scale_kernel<T>), dim3(1024), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((scale_kernel<T>), dim3(1024, 1, zdim(a.g.nmem)), dim3(256), 0, s, a);
     return 0;
 }
 

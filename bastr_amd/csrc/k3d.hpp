@@ -50,6 +50,8 @@ struct ColXArgs {
                          // pass does not read them, a forward pass does not write them
     int pskip_at, pskip; // plane p >= pskip_at is plane p + pskip (0, 0: planes in order)
     const cx<T>* tw;
+    long mstride;        // 2-D ensembles: elements between members (blockIdx.z)
+    int nmem;            // launch width along z (0 or 1: one member)
 };
 
 template <typename T>

@@ -84,8 +84,9 @@ void colx_kernel(ColXArgs<T> a) {
     const int kcut = a.kcut > 0 ? a.kcut : N;
     auto live = [&](int m) { return (m <= N / 2 ? m : N - m) < kcut; };
     cx<T> v[E];
+    const size_t mo = (size_t)blockIdx.z * (size_t)a.mstride;  // ensemble member offset
     if (MT == 1) {
-        const cx<T>* __restrict__ src = a.in[jb.in[0]];
+        const cx<T>* __restrict__ src = a.in[jb.in[0]] + mo;
         const int mult = jb.mult[0];
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -107,7 +108,7 @@ void colx_kernel(ColXArgs<T> a) {
             // one term's loads at a time: keeps the next term's loads from
             // being hoisted over this one's accumulation (register pressure)
             __builtin_amdgcn_sched_barrier(0);
-            const cx<T>* __restrict__ src = a.in[jb.in[t]];
+            const cx<T>* __restrict__ src = a.in[jb.in[t]] + mo;
             const int mult = jb.mult[t];
 #pragma unroll
             for (int k = 0; k < E; ++k) {
@@ -120,7 +121,7 @@ void colx_kernel(ColXArgs<T> a) {
         __builtin_amdgcn_sched_barrier(0);
     }
     line_fft<T, N, E, INV, true>(v, j, L, a.tw);
-    cx<T>* __restrict__ dst = a.out[jb.out];
+    cx<T>* __restrict__ dst = a.out[jb.out] + mo;
     const T sc = (T)a.scale;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -281,6 +282,7 @@ void cf2_kernel(k2d::ColFwdArgs<T> a) {
     __shared__ T lds[CB * lds_line(N)];
     const int blk = xcd_block(blockIdx.x, a.nkx / CB);
     if (blk < 0) return;
+    k2d::at_member(a);
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
     const int kx = blk * CB + l;
@@ -931,7 +933,7 @@ static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
         const int cb = v == 0 ? C::CB : C4::CB;
         const long nbl = (long)b.njob * (b.ncols / cb) * b.nplanes;
         if (nbl > INT_MAX - 8) return -1;
-        dim3 grid(k2d::xcd_grid_h((int)nbl)), block(v == 0 ? C::THREADS : C4::THREADS);
+        dim3 grid(k2d::xcd_grid_h((int)nbl), 1, k2d::zdim(b.nmem)), block(v == 0 ? C::THREADS : C4::THREADS);
         if (v == 0) {
             if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1>), grid, block, 0, s, b);
             else hipLaunchKernelGGL((colx_kernel<T, N, false, 1>), grid, block, 0, s, b);
@@ -998,7 +1000,7 @@ template <typename T, int N>
 static int cf2_n(int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s) {
     using C = Fwd2Cfg<T, N>;
     if (a.nkx < C::CB || a.nkx % C::CB || a.nkx > a.pitch || a.g.n1 != N) return -1;
-    dim3 grid(k2d::xcd_grid_h(a.nkx / C::CB)), block(C::THREADS);
+    dim3 grid(k2d::xcd_grid_h(a.nkx / C::CB), 1, k2d::zdim(a.g.nmem)), block(C::THREADS);
     switch (mode) {
 #define M(MM) case MM: hipLaunchKernelGGL((cf2_kernel<T, N, MM>), grid, block, 0, s, a); break;
         M(k2d::UP_A) M(k2d::UP_B) M(k2d::UP_C) M(k2d::UP_CN_MID) M(k2d::UP_CN_LAST)

@@ -150,5 +150,56 @@ class Solver:
         return self._fft("fft_backward", a)
 
 
+class Ensemble(Solver):
+    """nmembers independent 2-D runs of one grid in one handle
+    (bastr_create_ensemble): every launch of the step covers all members."""
+
+    def __init__(self, lib, params, nmembers, forced=None):
+        self._lib = lib
+        self._pre = "bastr"
+        self.params = params
+        self.nd = 2
+        self.shape = (params.ia, params.ja)
+        self.nmembers = int(nmembers)
+        flags = None
+        if forced is not None:
+            f = np.ascontiguousarray(np.asarray(forced, dtype=np.int32))
+            if f.shape != (self.nmembers,):
+                raise ValueError("forced needs one flag per member")
+            flags = f.ctypes.data_as(C.POINTER(C.c_int32))
+        h = C.c_void_p()
+        self._check(self._fn("create_ensemble")(C.byref(params), self.nmembers, flags, C.byref(h)))
+        self._h = h
+        self.allkmax = self._fn("allkmax")(self._h)
+
+    def set_member_fields(self, m, *u):
+        fs = [self._field(x) for x in u[:2]]
+        self._check(self._fn("set_member_fields")(self._h, int(m), _dptr(fs[0]), _dptr(fs[1]), None))
+
+    def init_random_phase_member(self, m, k0=4.0, seed=1, energy=1.0):
+        self._check(self._fn("init_random_phase_member")(self._h, int(m), float(k0), int(seed), float(energy)))
+
+    def get_member_fields(self, m):
+        fs = [np.empty(self.shape, dtype=np.float64, order="F") for _ in range(2)]
+        self._check(self._fn("get_member_fields")(self._h, int(m), _dptr(fs[0]), _dptr(fs[1]), None))
+        return fs
+
+    def get_member_time(self, m):
+        n = C.c_int64()
+        t = C.c_double()
+        self._check(self._fn("get_member_time")(self._h, int(m), C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def step(self, nsteps, want_spectra=True):
+        """Run nsteps iterations of every member; returns (status,
+        diags[nmembers][nsteps], spectra[nmembers, nsteps, 3, allkmax])."""
+        M = self.nmembers
+        flat = (abi.Diag * (M * nsteps))()
+        spec = np.zeros((M, nsteps, 3, self.allkmax), dtype=np.float64) if want_spectra else None
+        rc = self._check(self._fn("step")(self._h, nsteps, flat, _dptr(spec) if spec is not None else None))
+        diags = [flat[m * nsteps:(m + 1) * nsteps] for m in range(M)]
+        return rc, diags, spec
+
+
 def diag_table(diags, names):
     return np.array([[getattr(d, n) for n in names] for d in diags], dtype=np.float64)

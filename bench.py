@@ -167,12 +167,160 @@ def rel_l2(a, b):
     return float(np.sqrt(num / sum(float(np.sum(y * y)) for y in b)))
 
 
+# --config c4: the batched ensemble, one handle per member size, every member
+# of a size advanced by the same launches (bastr_create_ensemble)
+ENSEMBLE = [(256, 48), (512, 32), (1024, 16)]
+
+
+def bench_ensemble(args):
+    """96 independent 2-D runs (48 x 256^2, 32 x 512^2, 16 x 1024^2), FP32 with
+    FP64 energy/spectrum reductions, band forcing (k=5, E_t=0.075) on every
+    other member and free decay on the rest, projection on, each member its own
+    seed with k0 drawn uniformly from {3..8}; dt = 1e-4."""
+    import numpy as np
+
+    import bastr_amd
+    from bastr_amd.solver import make_params
+
+    lib = bastr_amd.load()
+    prof_api(lib)
+    rng = np.random.default_rng(2024)
+    steps = args.steps if args.steps is not None else 1000
+    groups = []
+    seed = 1
+    for n, M in ENSEMBLE:
+        p = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
+                        target_energy=0.075, lproject=True, kmax=n // 2, precision=1)
+        forced = [m % 2 for m in range(M)]
+        e = bastr_amd.ensemble(p, M, forced)
+        k0s = rng.integers(3, 9, size=M)
+        for m in range(M):
+            e.init_random_phase_member(m, k0=float(k0s[m]), seed=seed + m, energy=1.0)
+        groups.append(dict(n=n, M=M, ens=e, forced=forced, k0=k0s, seed0=seed, params=p))
+        seed += M
+    total_pts = sum(n * n * M for n, M in ENSEMBLE)
+
+    def advance(n_it, keep=False):
+        out = []
+        done = 0
+        while done < n_it:
+            k = min(args.chunk, n_it - done)
+            for g in groups:
+                rc, _, sp = g["ens"].step(k, want_spectra=True)
+                if rc != 0:
+                    raise RuntimeError(f"bastr_step returned {rc} ({g['n']}^2 ensemble)")
+                if keep:
+                    out.append((g["n"], done, sp))
+            done += k
+        return out
+
+    # warm-up with every kernel class timed; its first iterations give the
+    # FP32 spectra for the accuracy check
+    for g in groups:
+        lib.bastr_prof_enable(g["ens"]._h, 0x3FF)
+    nchk = max(args.warmup, 1)
+    warm_sp = advance(nchk, keep=True)
+    best = None
+    for g in groups:
+        w = read_prof(lib, g["ens"]._h)
+        for name, (c, n_l, ms, by) in w.items():
+            if n_l and (best is None or ms > best[3]):
+                best = (g, name, c, ms)
+    dg, dom_name, dom_cls, _ = best
+    for g in groups:
+        lib.bastr_prof_enable(g["ens"]._h, (1 << dom_cls) if g is dg else 0)
+    t0 = time.perf_counter()
+    advance(steps)
+    wall = time.perf_counter() - t0
+    print(f"  timed {steps} ensemble steps in {wall:.2f} s", file=sys.stderr, flush=True)
+    _, n_l, ms_l, by_l = read_prof(lib, dg["ens"]._h)[dom_name]
+    mean_s = ms_l / n_l * 1e-3
+    achieved = by_l / mean_s / 1e9
+    copy_gbps = C.c_double()
+    if lib.bastr_prof_copy_gbps(groups[0]["ens"]._h, 1 << 30, 10, C.byref(copy_gbps)) != 0:
+        copy_gbps.value = float("nan")
+
+    # accuracy: member 0 (forced) and member 1 (free decay) of each size, FP32
+    # ensemble spectra at the last warm-up iteration against one-field FP64 runs
+    accuracy = {}
+    for g in groups:
+        for m in (0, 1):
+            fp = make_params(g["n"], g["n"], reynolds=5000.0, deltat=1e-4, timemethod=1,
+                             forcemethod=2 if g["forced"][m] else 0, forcek=5, target_energy=0.075,
+                             lproject=True, kmax=g["n"] // 2, precision=0)
+            ref = bastr_amd.solver(fp)
+            ref.init_random_phase(k0=float(g["k0"][m]), seed=g["seed0"] + m, energy=1.0)
+            _, _, rsp = ref.step(nchk)
+            ref.close()
+            sp32 = [sp for (n, d0, sp) in warm_sp if n == g["n"] and d0 + sp.shape[1] == nchk][0]
+            e_abs, e_shell = ek_err(sp32[m, -1], rsp[-1])
+            accuracy[f"{g['n']}^2 member {m} ({'forced' if g['forced'][m] else 'free decay'})"] = {
+                "iterations": nchk, "ek_rel_l2_fp32_vs_fp64": e_abs, "ek_err_max_per_shell": e_shell}
+    for g in groups:
+        g["ens"].close()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1), 16)
+        oracle.set_threads(threads)
+        t_iter, parts = 0.0, []
+        for n, M in ENSEMBLE:
+            its = {256: 40, 512: 10, 1024: 3}[n]
+            fp = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
+                             target_energy=0.075, lproject=True, kmax=n // 2, precision=0)
+            g0 = bastr_amd.solver(fp)
+            g0.init_random_phase(k0=4.0, seed=1, energy=1.0)
+            u = g0.get_fields()
+            g0.close()
+            o = oracle.solver(fp)
+            o.set_fields(*u)
+            o.get_fields()
+            t1 = time.perf_counter()
+            o.step(its)
+            dt = time.perf_counter() - t1
+            o.close()
+            t_iter += M * dt / its
+            parts.append(f"{its} iterations of one {n}^2 member ({dt:.1f} s)")
+        model, ncpu = cpu_model()
+        cpu = {"value": total_pts / t_iter, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
+               "cpu_model": model, "host_logical_cpus": ncpu,
+               "sample": "; ".join(parts) + ", FP64 C restatement, OpenMP; ensemble rate = all points / "
+                         "sum over sizes of members x seconds per member iteration"}
+    out = {
+        "metric": METRIC,
+        "value": total_pts * steps / wall,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": nchk,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded random-phase fields (hitgen law), k0 uniform in {3..8}, one seed per member, u_rms=1",
+        "config": {"workload": "ensemble of 96 independent 2-D Burgers runs: 48 x 256^2, 32 x 512^2, 16 x 1024^2 "
+                               "(28,311,552 points), Re=5000, dt=1e-4, RK3, band forcing k=5 E_t=0.075 on every "
+                               "other member, free decay on the rest, lproject=t, spectra kmax=n/2",
+                   "members": [list(x) for x in ENSEMBLE], "precision": "fp32"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                     "ensemble_size": dg["n"], "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
+                     "copy_measured": round(copy_gbps.value, 1),
+                     "frac_of_copy": round(achieved / copy_gbps.value, 4)},
+        "cpu_baseline": cpu,
+        "accuracy": accuracy,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -181,6 +329,8 @@ def main():
         print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
               "MI355X's HBM); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        return bench_ensemble(args)
 
     import bastr_amd
     from bastr_amd.solver import make_params

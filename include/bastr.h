@@ -177,6 +177,37 @@ int bastr_shell_counts(const bastr_handle* h, int64_t* counts);
 int bastr_fft_forward(bastr_handle* h, double* inout);
 int bastr_fft_backward(bastr_handle* h, double* inout);
 
+/*
+ * Ensembles of independent 2-D runs on one grid.  No counterpart in the
+ * reference, which advances one field set (src/commvar.F90:27-29); each member
+ * is exactly the run a one-field handle with the same parameters would make
+ * (same kernels, same per-member summation order), and all members share
+ * every launch of the step.
+ *
+ * bastr_create_ensemble: p as for bastr_create (ka must be 0); nmembers in
+ *   [1, 65535]; forced: NULL (every member forced as p->forcemethod says) or
+ *   nmembers flags, 0 = this member is not forced (free decay), 1 = forced.
+ * bastr_set_member_fields / bastr_init_random_phase_member: as the one-field
+ *   calls, for one member, whose entry dealias (solution.F90:41-52) is applied
+ *   at once; every member must be set before the first bastr_step.
+ * bastr_get_member_fields / bastr_get_member_time: one member's field / clock.
+ * bastr_step on an ensemble handle: diags holds nmembers * nsteps entries,
+ *   member m's at diags + m*nsteps; spectra nmembers * nsteps * 3 * allkmax
+ *   doubles, member m's at spectra + m*nsteps*3*allkmax.  A member whose Es+Ed
+ *   is NaN stops there as a one-field run would (its later entries keep
+ *   zeros with NaN forcing values) while the others go on; the call then
+ *   returns BASTR_STOP_NAN.
+ * The one-field entry points bastr_set_fields, bastr_get_fields,
+ * bastr_init_random_phase, bastr_fft_*, bastr_pp_* reject ensemble handles of
+ * more than one member.
+ */
+int bastr_create_ensemble(const bastr_params* p, int32_t nmembers, const int32_t* forced, bastr_handle** out);
+int bastr_nmembers(const bastr_handle* h);
+int bastr_set_member_fields(bastr_handle* h, int32_t member, const double* u1, const double* u2, const double* u3);
+int bastr_get_member_fields(bastr_handle* h, int32_t member, double* u1, double* u2, double* u3);
+int bastr_init_random_phase_member(bastr_handle* h, int32_t member, double k0, uint64_t seed, double energy);
+int bastr_get_member_time(const bastr_handle* h, int32_t member, int64_t* nstep, double* time);
+
 #ifdef __cplusplus
 }
 #endif
