@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bastr.h"
@@ -27,6 +28,14 @@ using namespace k2d;
 namespace {
 
 thread_local char g_err[512] = "";
+
+// Graph capture (bastr_step) is kept apart from the library's blocking
+// runtime calls (allocations, synchronous copies) made on other threads for
+// other handles: the runtime invalidates a capture in progress when such a
+// call runs meanwhile.  Stream-ordered work (launches, async copies, stream and
+// event waits) does not take the lock.
+std::recursive_mutex g_sync_mu;
+using SyncLock = std::lock_guard<std::recursive_mutex>;
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
@@ -80,6 +89,7 @@ __global__ void copy16_kernel(const double2* __restrict__ in, double2* __restric
 }
 
 void* dalloc(size_t bytes, bool* ok) {
+    SyncLock lk(g_sync_mu);
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) { *ok = false; return nullptr; }
     if (hipMemset(p, 0, bytes) != hipSuccess) { *ok = false; }
@@ -223,6 +233,7 @@ void make_twiddles(int n, std::vector<cx<T>>& tw) {
 
 template <typename T>
 int upload_twiddles(int n, void** out) {
+    SyncLock lk(g_sync_mu);
     std::vector<cx<T>> tw;
     make_twiddles<T>(n, tw);
     HIPCHK(hipMalloc(out, sizeof(cx<T>) * n));
@@ -984,6 +995,7 @@ constexpr int kMaxGraphSteps = 256;
 
 int ensure_capacity(bastr_handle* h, int n) {
     if (n <= h->cap) return 0;
+    SyncLock lk(g_sync_mu);
     drop_graph(h);  // the captured launches point at the old buffers
     if (h->ddiag) (void)hipFree(h->ddiag);
     if (h->dspec) (void)hipFree(h->dspec);
@@ -992,7 +1004,8 @@ int ensure_capacity(bastr_handle* h, int n) {
     const size_t slots = (size_t)n * h->nmem;  // member-major: member m's slots start at m*n
     HIPCHK(hipMalloc(&h->ddiag, sizeof(bastr_diag) * slots));
     HIPCHK(hipMalloc(&h->dspec, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * slots));
-    HIPCHK(hipMemset(h->dspec, 0, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * slots));
+    HIPCHK(hipMemsetAsync(h->dspec, 0, sizeof(double) * 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * slots,
+                          h->stream));
     h->cap = n;
     return 0;
 }
@@ -1009,16 +1022,19 @@ int collect_ensemble(bastr_handle* h, int nsteps, bastr_diag* diags, double* spe
     prof_collect(h);
     bool any_nan = false;
     for (int m = 0; m < M; ++m) {
+        if (diags)
+            HIPCHK(hipMemcpyAsync(diags + (size_t)m * nsteps, h->ddiag + (size_t)m * h->cap,
+                                  sizeof(bastr_diag) * nsteps, hipMemcpyDeviceToHost, h->stream));
+        if (spectra && K > 0)
+            HIPCHK(hipMemcpyAsync(spectra + (size_t)m * nsteps * 3 * K, h->dspec + (size_t)m * h->cap * 3 * K,
+                                  sizeof(double) * 3 * K * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int m = 0; m < M; ++m) {
         const int done = status[m] == INT_MAX ? nsteps : status[m] + 1;
         any_nan = any_nan || status[m] != INT_MAX;
-        if (diags) {
-            HIPCHK(hipMemcpy(diags + (size_t)m * nsteps, h->ddiag + (size_t)m * h->cap, sizeof(bastr_diag) * nsteps,
-                             hipMemcpyDeviceToHost));
-        }
         if (spectra && K > 0) {
             double* sp = spectra + (size_t)m * nsteps * 3 * K;
-            HIPCHK(hipMemcpy(sp, h->dspec + (size_t)m * h->cap * 3 * K, sizeof(double) * 3 * K * (size_t)nsteps,
-                             hipMemcpyDeviceToHost));
             for (int it = 0; it < nsteps; ++it)
                 for (int b = 1; b <= K; ++b) sp[(size_t)it * 3 * K + b - 1] = h->kn[b];
         }
@@ -1056,6 +1072,7 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     // events stay out of the capture.
     if (h->use_graph && h->prof_mask == 0 && nsteps <= kMaxGraphSteps) {
         if (!h->gexec || h->gsteps != nsteps) {
+            SyncLock lk(g_sync_mu);
             drop_graph(h);
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1086,11 +1103,13 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     prof_collect(h);
     const int done = status == INT_MAX ? nsteps : status + 1;  // iterations with diagnostics
     const int K = h->allkmax;
-    if (diags) {
-        HIPCHK(hipMemcpy(diags, h->ddiag, sizeof(bastr_diag) * done, hipMemcpyDeviceToHost));
-    }
+    if (diags)
+        HIPCHK(hipMemcpyAsync(diags, h->ddiag, sizeof(bastr_diag) * done, hipMemcpyDeviceToHost, h->stream));
+    if (spectra && K > 0)
+        HIPCHK(hipMemcpyAsync(spectra, h->dspec, sizeof(double) * 3 * K * (size_t)done, hipMemcpyDeviceToHost,
+                              h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     if (spectra && K > 0) {
-        HIPCHK(hipMemcpy(spectra, h->dspec, sizeof(double) * 3 * K * (size_t)done, hipMemcpyDeviceToHost));
         for (int it = 0; it < done; ++it)
             for (int b = 1; b <= K; ++b) spectra[(size_t)it * 3 * K + b - 1] = h->kn[b];
     }
@@ -1214,7 +1233,16 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     h->dstage = (double*)dalloc(h->npts * sizeof(double), &ok);
     const int nbx = (h->nh + 63) / 64;
     const long ntask = (long)h->n1 * h->n2 * nbx;
-    h->nblk_spec = (int)std::max(1L, std::min(512L, ntask / 8));
+    if (nd == 2) {
+        // 64-mode tasks the (dealias-limited) spectra pass sums; each block
+        // writes (allkmax+1)*2 partial bins, so a block takes at least
+        // (allkmax+1)/8 tasks: its partials stay under a quarter of its reads
+        const long ntask2 = h->prune ? (long)h->rows_n * ((h->cut + 63) / 64) : ntask;
+        const long per = std::max(8L, (long)(h->allkmax + 1) / 8);
+        h->nblk_spec = (int)std::max(1L, std::min(512L, ntask2 / per));
+    } else {
+        h->nblk_spec = (int)std::max(1L, std::min(512L, ntask / 8));
+    }
     {
         const int L = vg_block_rows(h->n0, h->prec, nd);
         h->nblk_vg = (h->n1 * h->n2 + L - 1) / L;
@@ -1235,8 +1263,11 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     }
     int rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n0, &h->tw0) : upload_twiddles<double>(h->n0, &h->tw0);
     if (!rc) rc = h->prec == BASTR_PREC_F32 ? upload_twiddles<float>(h->n1, &h->tw1) : upload_twiddles<double>(h->n1, &h->tw1);
-    if (!rc && h->mforce && hipMemcpy(h->mforce, forced, sizeof(int) * nmem, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(BASTR_EDEVICE, "bastr_create_ensemble: copy of the forcing flags failed");
+    if (!rc && h->mforce) {
+        SyncLock lk(g_sync_mu);
+        if (hipMemcpy(h->mforce, forced, sizeof(int) * nmem, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(BASTR_EDEVICE, "bastr_create_ensemble: copy of the forcing flags failed");
+    }
     if (rc) {
         bastr_destroy(h);
         return rc;
@@ -1308,6 +1339,7 @@ int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const 
     if (!h || !u1 || !u2 || (h && h->nd == 3 && !u3)) return fail(BASTR_EINVAL, "bastr_set_fields: NULL argument");
     if (h->lw > 1) return fail(BASTR_EINVAL, "bastr_set_fields: an ensemble handle takes the bastr_*_member entry points");
     const double* src[3] = {u1, u2, u3};
+    SyncLock lk(g_sync_mu);
     for (int c = 0; c < h->nd; ++c) {
         if (h->prec == BASTR_PREC_F64) {
             HIPCHK(hipMemcpy(h->ureal[c], src[c], sizeof(double) * h->npts, hipMemcpyHostToDevice));
@@ -1621,6 +1653,7 @@ static int fft_c2c3(bastr_handle* h, bool inverse, const cx<double>* tw) {
 
 static int fft_c2c(bastr_handle* h, double* inout, bool inverse) {
     if (h && h->nmem > 1) return fail(BASTR_EINVAL, "bastr_fft_*: not available on an ensemble handle");
+    SyncLock lk(g_sync_mu);
     if (!h || !inout) return fail(BASTR_EINVAL, "bastr_fft: NULL argument");
     const size_t bytes = sizeof(double) * 2 * h->npts;
     if (!h->cbuf) HIPCHK(hipMalloc(&h->cbuf, bytes));
@@ -1795,6 +1828,7 @@ int pp_base(const bastr_handle* h) { return h->nd == 2 ? 6 : 18; }
 
 int pp_arena(bastr_handle* h) {
     if (h->pp_status) return 0;
+    SyncLock lk(g_sync_mu);
     const int nbuf = h->nd == 2 ? 14 : 30;
     const size_t cbytes = h->nhalf * 2 * h->esz;
     const long nbx = (h->nh + 63) / 64;
@@ -1869,6 +1903,7 @@ int pp_fwd_inplace(bastr_handle* h, const Gate& g, int first, int nf, int axis, 
 // u_hat = fft(u) (normalised, no dealias) into arena [0..nd-1]
 template <typename T>
 int pp_load(bastr_handle* h, const double* const* src) {
+    SyncLock lk(g_sync_mu);
     const int rb = pp_base(h);
     for (int c = 0; c < h->nd; ++c) {
         if (h->prec == BASTR_PREC_F64) {

@@ -200,17 +200,29 @@ def bench_ensemble(args):
         seed += M
     total_pts = sum(n * n * M for n, M in ENSEMBLE)
 
+    # each size's handle has its own HIP stream: the three are stepped from
+    # three host threads (ctypes releases the GIL during the call), so their
+    # kernels overlap on the GPU; --serial steps them one after another
+    from concurrent.futures import ThreadPoolExecutor
+    pool = ThreadPoolExecutor(max_workers=len(groups))
+
+    def step_group(g, k):
+        rc, _, sp = g["ens"].step(k, want_spectra=True)
+        if rc != 0:
+            raise RuntimeError(f"bastr_step returned {rc} ({g['n']}^2 ensemble)")
+        return sp
+
     def advance(n_it, keep=False):
         out = []
         done = 0
         while done < n_it:
             k = min(args.chunk, n_it - done)
-            for g in groups:
-                rc, _, sp = g["ens"].step(k, want_spectra=True)
-                if rc != 0:
-                    raise RuntimeError(f"bastr_step returned {rc} ({g['n']}^2 ensemble)")
-                if keep:
-                    out.append((g["n"], done, sp))
+            if args.serial:
+                sps = [step_group(g, k) for g in groups]
+            else:
+                sps = list(pool.map(lambda g: step_group(g, k), groups))
+            if keep:
+                out.extend((g["n"], done, sp) for g, sp in zip(groups, sps))
             done += k
         return out
 
@@ -227,6 +239,11 @@ def bench_ensemble(args):
             if n_l and (best is None or ms > best[3]):
                 best = (g, name, c, ms)
     dg, dom_name, dom_cls, _ = best
+    # one untimed unprofiled chunk per size, one after another: captures each
+    # handle's iteration graph for the chunk length before the timed region
+    for g in groups:
+        lib.bastr_prof_enable(g["ens"]._h, 0)
+        step_group(g, min(args.chunk, steps))
     for g in groups:
         lib.bastr_prof_enable(g["ens"]._h, (1 << dom_cls) if g is dg else 0)
     t0 = time.perf_counter()
@@ -266,7 +283,7 @@ def bench_ensemble(args):
         oracle.set_threads(threads)
         t_iter, parts = 0.0, []
         for n, M in ENSEMBLE:
-            its = {256: 40, 512: 10, 1024: 3}[n]
+            its = {256: 400, 512: 100, 1024: 25}[n]  # about 2-3 s of CPU work each
             fp = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
                              target_energy=0.075, lproject=True, kmax=n // 2, precision=0)
             g0 = bastr_amd.solver(fp)
@@ -303,7 +320,9 @@ def bench_ensemble(args):
         "config": {"workload": "ensemble of 96 independent 2-D Burgers runs: 48 x 256^2, 32 x 512^2, 16 x 1024^2 "
                                "(28,311,552 points), Re=5000, dt=1e-4, RK3, band forcing k=5 E_t=0.075 on every "
                                "other member, free decay on the rest, lproject=t, spectra kmax=n/2",
-                   "members": [list(x) for x in ENSEMBLE], "precision": "fp32"},
+                   "members": [list(x) for x in ENSEMBLE], "precision": "fp32",
+                   "launch": "one handle per size (every member of a size in each launch), the three on their "
+                             "own streams " + ("one after another" if args.serial else "concurrently")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
                      "ensemble_size": dg["n"], "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
@@ -324,6 +343,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serial", action="store_true", help="c4: step the size groups one after another")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
