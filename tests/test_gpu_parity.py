@@ -128,6 +128,8 @@ def test_step_parity_large_grids(hip, orc, n, nsteps):
     ((4096, 64), 0, 2, 3),    # 4096-point rows (one row per workgroup, largest LDS line)
     ((64, 4096), 0, 0, 3),    # 4096-point columns
     ((4096, 32), 1, 0, 3),    # FP32 at the largest row length
+    ((2048, 64), 1, 2, 3),    # FP32 rows of 16 points per thread
+    ((1024, 128), 1, 2, 3),   # FP32 rows of 16 points per thread in one wavefront
 ])
 def test_step_parity_longest_lines(hip, orc, shape, prec, force, nsteps):
     """ia or ja at the 4096 limit of bastr_create: the longest row and
