@@ -581,21 +581,24 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     const int nrow = a.nky > 0 ? a.nky : n1;
     const size_t hp = (size_t)a.g.hp;
     const int nbx = (nh + 63) / 64;
-    const long ntask = (long)nrow * nbx;
+    const int ntask = nrow * nbx;  // < 2^31 (launch_spectra checks)
     // tasks are dealt round-robin over blocks and waves, so the few tasks
     // with many shells <= allkmax (small |k|) are spread over the machine
-    const long tstride = 4L * gridDim.x;
-    const long t1 = ntask;
+    const int tstride = 4 * gridDim.x;
+    const int t1 = ntask;
     double tot[4] = {0, 0, 0, 0};
     constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
-    for (long t = (long)blockIdx.x * 4 + wave; t < t1; t += tstride * TB) {
+    for (int t = blockIdx.x * 4 + wave; t < t1; t += tstride * TB) {
         cx<T> u1[TB], u2[TB];
+        int r_[TB], kx_[TB];  // each task's row and kx, formed once (32-bit)
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const long tq = t + tstride * q;
-            const int r0 = (int)(tq / nbx);
+            const int tq = t + tstride * q;
+            const int r0 = tq / nbx;
             const int r = r0 >= a.ky_at ? r0 + a.ky_skip : r0;
-            const int kx = (int)(tq % nbx) * 64 + lane;
+            const int kx = (tq - r0 * nbx) * 64 + lane;
+            r_[q] = r;
+            kx_[q] = kx;
             if (tq < t1 && kx < nh) {
                 u1[q] = a.U[0][(size_t)r * hp + kx];
                 u2[q] = a.U[1][(size_t)r * hp + kx];
@@ -606,11 +609,10 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
         }
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const long tq = t + tstride * q;
+            const int tq = t + tstride * q;
             if (tq >= t1) break;  // wave-uniform
-            const int r0 = (int)(tq / nbx);
-            const int r = r0 >= a.ky_at ? r0 + a.ky_skip : r0;
-            const int kx = (int)(tq % nbx) * 64 + lane;
+            const int r = r_[q];
+            const int kx = kx_[q];
             double es = 0, ed = 0, w = 0;
             int s = INT_MAX;
             if (kx < nh) {
@@ -862,6 +864,10 @@ int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nbl
 }
 template <typename T>
 int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s) {
+    {  // the kernel's task index is 32-bit
+        const long nh = a.nkx > 0 ? a.nkx : a.g.nh, nrow = a.nky > 0 ? a.nky : a.g.n1;
+        if (nrow * ((nh + 63) / 64) + 16L * nblk >= INT_MAX) return -1;
+    }
     const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
     hipLaunchKernelGGL((spectra_kernel<T>), dim3(nblk, 1, zdim(a.g.nmem)), dim3(256), shm, s, a);
     return 0;

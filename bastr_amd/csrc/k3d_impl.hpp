@@ -720,25 +720,24 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
     const int ny = a.nky > 0 ? a.nky : n1, nz = a.nkz > 0 ? a.nkz : n2;
     const size_t hp = (size_t)a.g.hp;
     const int nbx = (nh + 63) / 64;
-    const long ntask = (long)ny * nz * nbx;
-    // (ky index, kz index) of task row r0
-    auto row_of = [&](long r0, int& iy, int& iz) {
-        iy = (int)(r0 % ny);
-        iz = (int)(r0 / ny);
-        if (iy >= a.ky_at) iy += a.ky_skip;
-        if (iz >= a.kz_at) iz += a.kz_skip;
-    };
-    const long tstride = 4L * gridDim.x;
+    const int ntask = ny * nz * nbx;  // < 2^31 (launch_spec3 checks)
+    const int tstride = 4 * gridDim.x;
     double tot[4] = {0, 0, 0, 0};
     constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
-    for (long t = (long)blockIdx.x * 4 + wave; t < ntask; t += tstride * TB) {
+    for (int t = blockIdx.x * 4 + wave; t < ntask; t += tstride * TB) {
         cx<T> u[TB][3];
+        int ky_[TB], kz_[TB], kx_[TB];  // each task's indices, formed once (32-bit)
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const long tq = t + tstride * q;
-            int iy, iz;
-            row_of(tq / nbx, iy, iz);
-            const int kx = (int)(tq % nbx) * 64 + lane;
+            const int tq = t + tstride * q;
+            const int r0 = tq / nbx;
+            int iy = r0 % ny, iz = r0 / ny;
+            if (iy >= a.ky_at) iy += a.ky_skip;
+            if (iz >= a.kz_at) iz += a.kz_skip;
+            const int kx = (tq - r0 * nbx) * 64 + lane;
+            ky_[q] = ksigned(iy, n1);
+            kz_[q] = ksigned(iz, n2);
+            kx_[q] = kx;
             const bool ok = tq < ntask && kx < nh;
             const size_t o = ok ? ((size_t)iz * n1 + iy) * hp + kx : 0;
 #pragma unroll
@@ -746,15 +745,13 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
         }
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const long tq = t + tstride * q;
+            const int tq = t + tstride * q;
             if (tq >= ntask) break;  // wave-uniform
-            int iy, iz;
-            row_of(tq / nbx, iy, iz);
-            const int kx = (int)(tq % nbx) * 64 + lane;
+            const int kx = kx_[q];
             double es = 0, ed = 0, w = 0;
             int s = INT_MAX;
             if (kx < nh) {
-                const int ky = ksigned(iy, n1), kz = ksigned(iz, n2);
+                const int ky = ky_[q], kz = kz_[q];
                 w = (kx == 0 || kx == n0 / 2) ? 1.0 : 2.0;
                 const double kk = sqrt((double)(kx * kx + ky * ky + kz * kz));
                 if (kk > 0.5) {
@@ -1056,6 +1053,10 @@ int launch_vg3(int n, const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
 }
 template <typename T>
 int launch_spec3(const Spec3Args<T>& a, int nblk, hipStream_t s) {
+    {  // the kernel's task index is 32-bit
+        const long nh = a.nkx > 0 ? a.nkx : a.g.nh, ny = a.nky > 0 ? a.nky : a.g.n1, nz = a.nkz > 0 ? a.nkz : a.g.n2;
+        if (ny * nz * ((nh + 63) / 64) + 4L * nblk * 4 >= INT_MAX) return -1;
+    }
     const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
     hipLaunchKernelGGL((spec3_kernel<T>), dim3(nblk), dim3(256), shm, s, a);
     return 0;
