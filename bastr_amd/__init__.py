@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 from . import abi
-from .solver import BastrError, Ensemble, Solver, diag_table, make_params  # noqa: F401
+from .solver import BastrError, Ensemble, Solver, diag_table, make_params, step_group  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbastr_hip.so")

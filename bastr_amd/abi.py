@@ -81,6 +81,7 @@ ENSEMBLE_SIGNATURES = [
     ("get_member_fields", C.c_int, [_P, C.c_int32, _D, _D, _D]),
     ("init_random_phase_member", C.c_int, [_P, C.c_int32, C.c_double, C.c_uint64, C.c_double]),
     ("get_member_time", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    ("step_group", C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(C.POINTER(Diag)), C.POINTER(_D)]),
 ]
 
 

@@ -1052,8 +1052,8 @@ int collect_ensemble(bastr_handle* h, int nsteps, bastr_diag* diags, double* spe
 }
 
 template <typename T>
-int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
-    int rc;
+int step_launch(bastr_handle* h, int nsteps) {
+    int rc = 0;
     if (h->prologue_pending) {
         if ((rc = (h->nd == 3 ? run_prologue3<T>(h) : run_prologue<T>(h)))) return rc;
         h->prologue_pending = false;
@@ -1096,6 +1096,12 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
         for (int it = 0; it < nsteps; ++it)
             if ((rc = (h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it)))) return rc;
     }
+    return 0;
+}
+
+// Wait for a step call's launches and return its results (the one-field
+// layout, or the ensemble layout of collect_ensemble).
+int step_collect(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     if (h->nmem > 1) return collect_ensemble(h, nsteps, diags, spectra);
     int status = INT_MAX;
     HIPCHK(hipMemcpyAsync(&status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1124,6 +1130,12 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
         h->time = h->time + h->p.deltat;
     }
     return status == INT_MAX ? BASTR_OK : BASTR_STOP_NAN;
+}
+
+template <typename T>
+int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
+    const int rc = step_launch<T>(h, nsteps);
+    return rc ? rc : step_collect(h, nsteps, diags, spectra);
 }
 
 }  // namespace
@@ -1602,6 +1614,39 @@ int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spect
     if (nsteps == 0) return BASTR_OK;
     return h->prec == BASTR_PREC_F32 ? step_impl<float>(h, nsteps, diags, spectra)
                                      : step_impl<double>(h, nsteps, diags, spectra);
+}
+
+int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_diag* const* diags,
+                     double* const* spectra) {
+    if (!hs || nh < 1) return fail(BASTR_EINVAL, "bastr_step_group: no handles");
+    if (nsteps < 0) return fail(BASTR_EINVAL, "bastr_step_group: nsteps < 0");
+    for (int i = 0; i < nh; ++i) {
+        if (!hs[i]) return fail(BASTR_EINVAL, "bastr_step_group: NULL handle %d", i);
+        if (!hs[i]->have_fields) return fail(BASTR_EINVAL, "bastr_step_group: fields of handle %d not set", i);
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(BASTR_EINVAL, "bastr_step_group: handle %d given twice", i);
+    }
+    if (nsteps == 0) return BASTR_OK;
+    // every handle's launches go to its own stream first, so they overlap on
+    // the device; then each is waited for and its results collected
+    for (int i = 0; i < nh; ++i) {
+        bastr_handle* h = hs[i];
+        const int rc = h->prec == BASTR_PREC_F32 ? step_launch<float>(h, nsteps) : step_launch<double>(h, nsteps);
+        if (rc) {
+            for (int j = 0; j < i; ++j) (void)hipStreamSynchronize(hs[j]->stream);
+            return rc;
+        }
+    }
+    int ret = BASTR_OK;
+    for (int i = 0; i < nh; ++i) {
+        const int rc = step_collect(hs[i], nsteps, diags ? diags[i] : nullptr, spectra ? spectra[i] : nullptr);
+        if (rc < 0) {
+            for (int j = i + 1; j < nh; ++j) (void)hipStreamSynchronize(hs[j]->stream);
+            return rc;
+        }
+        if (rc == BASTR_STOP_NAN) ret = BASTR_STOP_NAN;
+    }
+    return ret;
 }
 
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts) {

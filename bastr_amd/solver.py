@@ -201,5 +201,30 @@ class Ensemble(Solver):
         return rc, diags, spec
 
 
+def step_group(handles, nsteps, want_spectra=True):
+    """bastr_step_group over Solver / Ensemble objects of one library: every
+    handle's iterations are launched before any is waited for.  Returns
+    (status, [(diags, spectra) per handle]) in each object's own step layout."""
+    lib = handles[0]._lib
+    n = len(handles)
+    hs = (C.c_void_p * n)(*[h._h.value for h in handles])
+    flats, specs = [], []
+    for h in handles:
+        M = getattr(h, "nmembers", 1)
+        flats.append((abi.Diag * (M * nsteps))())
+        shape = (M, nsteps, 3, h.allkmax) if isinstance(h, Ensemble) else (nsteps, 3, h.allkmax)
+        specs.append(np.zeros(shape, dtype=np.float64) if want_spectra else None)
+    dp = (C.POINTER(abi.Diag) * n)(*[C.cast(f, C.POINTER(abi.Diag)) for f in flats])
+    sp = (C.POINTER(C.c_double) * n)(*[_dptr(x) if x is not None else None for x in specs])
+    rc = handles[0]._check(lib.bastr_step_group(hs, n, nsteps, dp, sp))
+    out = []
+    for h, f, x in zip(handles, flats, specs):
+        if isinstance(h, Ensemble):
+            out.append(([f[m * nsteps:(m + 1) * nsteps] for m in range(h.nmembers)], x))
+        else:
+            out.append((f[:], x))
+    return rc, out
+
+
 def diag_table(diags, names):
     return np.array([[getattr(d, n) for n in names] for d in diags], dtype=np.float64)

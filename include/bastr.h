@@ -147,6 +147,19 @@ int bastr_get_time(const bastr_handle* h, int64_t* nstep, double* time);
 int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra);
 
 /*
+ * bastr_step on several handles at once (no counterpart in the reference):
+ * every handle's iterations are launched on its own stream before any is
+ * waited for, so runs of different sizes (e.g. the ensembles of one
+ * workload) overlap on the device.  diags[i] / spectra[i]: NULL or handle
+ * i's arrays, laid out as bastr_step lays them out for that handle (the
+ * pointer arrays themselves may be NULL).  Returns a negative code on the
+ * first failure, else BASTR_STOP_NAN if any handle stopped on a NaN, else 0.
+ * A handle may appear only once.
+ */
+int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_diag* const* diags,
+                     double* const* spectra);
+
+/*
  * Post-processing of a field, e.g. a dump (Bastrpp's commands; the solver state,
  * nstep and time of the handle are left as they were).  Fields are Fortran-order
  * real arrays as for bastr_set_fields, used as given (no entry dealias).

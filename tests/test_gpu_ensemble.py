@@ -162,3 +162,47 @@ def test_handles_stepped_from_threads(hip):
             else:
                 same_diags(od, rd)
             assert np.array_equal(os_, rs)
+
+
+def test_step_group_equals_separate_steps(hip):
+    """bastr_step_group (every handle launched before any is waited for)
+    gives each handle exactly the results of its own bastr_step calls."""
+    print("case step_group", flush=True)
+    from bastr_amd.solver import step_group
+
+    def build():
+        hs = []
+        for shape, M in (((64, 64), 3), ((128, 128), 2)):
+            p = make_params(*shape, forcemethod=2, lproject=True, kmax=128, precision=abi.PREC_F32)
+            e = Ensemble(hip, p, M, [m % 2 for m in range(M)])
+            for m in range(M):
+                e.init_random_phase_member(m, k0=3 + m, seed=40 + m)
+            hs.append(e)
+        s = Solver(hip, "bastr", make_params(128, 64, forcemethod=2, lproject=True))
+        s.set_fields(*hitgen((128, 64), k0=4, seed=19))
+        hs.append(s)
+        return hs
+
+    sep = build()
+    ref = [[h.step(n)[1:] for n in (4, 4, 6)] for h in sep]
+    grp = build()
+    got = []
+    for n in (4, 4, 6):
+        rc, out = step_group(grp, n)
+        assert rc == 0
+        got.append(out)
+    for i, h in enumerate(grp):
+        for c in range(3):
+            (od, os_), (rd, rs) = got[c][i], ref[i][c]
+            if isinstance(h, Ensemble):
+                for a, b in zip(od, rd):
+                    same_diags(a, b)
+            else:
+                same_diags(od, rd)
+            assert np.array_equal(os_, rs)
+    for a, b in zip(grp, sep):
+        if isinstance(a, Ensemble):
+            for m in range(a.nmembers):
+                assert all(np.array_equal(x, y) for x, y in zip(a.get_member_fields(m), b.get_member_fields(m)))
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(a.get_fields(), b.get_fields()))
