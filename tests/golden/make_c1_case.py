@@ -1,0 +1,63 @@
+"""Writes the C1 reference-check case (2-D 256^2, FP64, Re = 5000, dt = 1e-4,
+maxstep = 200, free decay, no projection, spectra to kmax = 128) into OUTDIR:
+
+  OUTDIR/datin/input.dat       the reference's input format (readwrite.F90:19-112)
+  OUTDIR/datin/flowini2d.h5    seeded random-phase initial field (hitgen law,
+                               k0 = 4, seed 1, u_rms = 1), written by bastr_run hitgen
+  OUTDIR/expected/log/*.dat    the logs bastr_run writes for this case
+
+Needs the GPU (the driver runs the step on the device).  Usage:
+  python tests/golden/make_c1_case.py OUTDIR
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+DRIVER = os.path.join(ROOT, "bastr_amd", "bin", "bastr_run")
+
+INPUT = """########################################################
+#  Bastr input: 2-D viscous Burgers 256^2, free decay  #
+########################################################
+
+# ia,ja,ka
+256,256,0
+
+# ref_t,reynolds
+273.15d0, 5000.d0
+
+# maxstep,deltat,lwsequ,feqwsequ,lwspectra,feqwspe,kmax,timemethod
+200, 1.d-4, t, 200, t, 100, 128, 1
+
+# forcemethod, forcek, targetenergy, lproject
+0,5,0.075d0,f
+
+# initialmethod
+1
+"""
+
+
+def main():
+    out = os.path.abspath(sys.argv[1])
+    os.makedirs(os.path.join(out, "datin"), exist_ok=True)
+    with open(os.path.join(out, "datin", "input.dat"), "w") as f:
+        f.write(INPUT)
+    work = out + ".run"
+    if os.path.exists(work):
+        shutil.rmtree(work)
+    shutil.copytree(out, work)
+    for args in (["hitgen", "1", "4", "1.0"], []):
+        r = subprocess.run([DRIVER, *args], cwd=work, capture_output=True, text=True, timeout=600)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        if r.returncode != 0:
+            sys.exit(r.returncode)
+    shutil.copy(os.path.join(work, "datin", "flowini2d.h5"), os.path.join(out, "datin", "flowini2d.h5"))
+    os.makedirs(os.path.join(out, "expected", "log"), exist_ok=True)
+    for name in sorted(os.listdir(os.path.join(work, "log"))):
+        shutil.copy(os.path.join(work, "log", name), os.path.join(out, "expected", "log", name))
+    shutil.rmtree(work)
+
+
+if __name__ == "__main__":
+    main()
