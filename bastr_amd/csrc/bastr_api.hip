@@ -115,6 +115,14 @@ struct bastr_handle {
     size_t npts = 0, nhalf = 0;
     size_t esz = 8;  // sizeof(T)
     hipStream_t stream = nullptr;
+    // 2-D overlap: the spectra, the dissipation moment's x pass and the
+    // diagnostics reduction run on `side` beside the main chain (forked and
+    // joined with the events fork[0..2] each iteration); stage A's y-forward
+    // then writes WA, since the dissipation pass still reads W
+    bool overlap = false;
+    hipStream_t side = nullptr;
+    hipEvent_t fork[3] = {};
+    void* WA[2] = {};
     void* U[3] = {};
     void* UB[3] = {};
     void* KA[3] = {};
@@ -191,18 +199,20 @@ struct ProfScope {
     bastr_handle* h;
     int cls;
     bool on;
-    ProfScope(bastr_handle* h_, int c) : h(h_), cls(c), on((h_->prof_mask >> c) & 1u) {
+    hipStream_t s;
+    ProfScope(bastr_handle* h_, int c, hipStream_t st = nullptr)
+        : h(h_), cls(c), on((h_->prof_mask >> c) & 1u), s(st ? st : h_->stream) {
         if (on) {
             hipEvent_t e = pool_event(h);
             if (!e) { on = false; return; }
             h->ev_pending.push_back({cls, h->ev_used - 1});
-            (void)hipEventRecord(e, h->stream);
+            (void)hipEventRecord(e, s);
         }
     }
     ~ProfScope() {
         if (on) {
             hipEvent_t e = pool_event(h);
-            if (e) (void)hipEventRecord(e, h->stream);
+            if (e) (void)hipEventRecord(e, s);
             else h->ev_pending.pop_back();
         }
     }
@@ -388,12 +398,12 @@ int run_yfwd2(bastr_handle* h, const Gate& g, void* const* src, void* const* dst
 // spectral update (dealias/stage combination) from the y-forward outputs in
 // W[0..1] (prune: the modes inside the dealias limits)
 template <typename T>
-int run_update2(bastr_handle* h, const Gate& g, int mode, bool first, bool prune) {
+int run_update2(bastr_handle* h, const Gate& g, int mode, bool first, bool prune, void* const* src = nullptr) {
     ColFwdArgs<T> c{};
     c.g = geo<T>(h);
     c.gate = g;
-    c.nx[0] = C<T>(h->W[0]);
-    c.nx[1] = C<T>(h->W[1]);
+    c.nx[0] = C<T>(src ? src[0] : h->W[0]);
+    c.nx[1] = C<T>(src ? src[1] : h->W[1]);
     for (int q = 0; q < 2; ++q) {
         c.U[q] = C<T>(h->U[q]);
         c.UB[q] = C<T>(h->UB[q]);
@@ -510,11 +520,16 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
         if (k3d::cf2_full_rows<T>(h->n1)) {
             rc = run_cf2<T>(h, gstep, modes[stage], first);
         } else {
-            // the fused kernel would need narrower column blocks here: y-forward, then the update
+            // the fused kernel would need narrower column blocks here: y-forward, then the
+            // update (the first stage writes WA while the dissipation pass may still read W)
             void* nx[2] = {h->NX[0], h->NX[1]};
             void* w[2] = {h->W[0], h->W[1]};
+            if (first && h->overlap) {
+                w[0] = h->WA[0];
+                w[1] = h->WA[1];
+            }
             rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
-            if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true);
+            if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true, w);
         }
         if (rc) return rc;
     }
@@ -522,10 +537,23 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
 }
 
 // One iteration of mainloop2D's body (solution.F90:54-97).
+// With h->overlap the diagnostics that only feed the log run on h->side:
+//   main: fork0 | col_inv_vg, velgrad (stage A's x pass) | fork1 | stage A | wait fork2 | stages B, C, forcing
+//   side: wait fork0 | spectra | wait fork1 | dissipation x pass, reduction | fork2
+// The dissipation pass reads W, which stage A leaves alone (its y-forward
+// writes WA); nothing writes u_hat before the join; a NaN found by the
+// reduction stops stages B and C and the forcing (stage A writes only its
+// scratch KA, UB), as in the one-stream order.
 template <typename T>
 int run_iteration(bastr_handle* h, int it) {
     const Gate gdiag{h->status, it, 0};
     const Gate gstep{h->status, it, 1};
+    const bool ov = h->overlap;
+    hipStream_t ss = ov ? h->side : h->stream;
+    if (ov) {
+        HIPCHK(hipEventRecord(h->fork[0], h->stream));
+        HIPCHK(hipStreamWaitEvent(h->side, h->fork[0], 0));
+    }
     // spectra_compute2D on u_hat
     SpecArgs<T> sa{};
     sa.g = geo<T>(h);
@@ -542,8 +570,8 @@ int run_iteration(bastr_handle* h, int it) {
         sa.ky_skip = h->rows_skip;
     }
     {
-        ProfScope ps(h, BASTR_K_SPECTRA);
-        LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, h->stream));
+        ProfScope ps(h, BASTR_K_SPECTRA, ss);
+        LAUNCHCHK(launch_spectra<T>(sa, h->nblk_spec, ss));
     }
     // velgrad_calculate2D: derivative fields shared with RK stage A
     {
@@ -563,7 +591,15 @@ int run_iteration(bastr_handle* h, int it) {
     va.tw = C<T>(h->tw0);
     {
         ProfScope ps(h, BASTR_K_VELGRAD);
-        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg));
+        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg, 1));
+    }
+    if (ov) {
+        HIPCHK(hipEventRecord(h->fork[1], h->stream));
+        HIPCHK(hipStreamWaitEvent(h->side, h->fork[1], 0));
+    }
+    {
+        ProfScope ps(h, BASTR_K_VG_DIS, ss);
+        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, ss, &h->nblk_vg, 2));
     }
     FinArgs fa{};
     fa.gate = gdiag;
@@ -584,21 +620,25 @@ int run_iteration(bastr_handle* h, int it) {
     fa.dstride = h->cap;
     fa.sstride = (long)h->cap * 3 * h->allkmax;
     {
-        ProfScope ps(h, BASTR_K_FINALIZE);
-        LAUNCHCHK(launch_finalize(fa, h->stream));
+        ProfScope ps(h, BASTR_K_FINALIZE, ss);
+        LAUNCHCHK(launch_finalize(fa, ss));
     }
+    if (ov) HIPCHK(hipEventRecord(h->fork[2], h->side));
     const void* Ua[2] = {h->U[0], h->U[1]};
     const void* Ub[2] = {h->UB[0], h->UB[1]};
     int rc;
     if (h->p.timemethod == 1) {
         // RK32D (solution.F90:172-203)
         if ((rc = run_stage<T>(h, 0, gstep, Ua, true))) return rc;
+        if (ov) HIPCHK(hipStreamWaitEvent(h->stream, h->fork[2], 0));
         if ((rc = run_stage<T>(h, 1, gstep, Ub))) return rc;
         if ((rc = run_stage<T>(h, 2, gstep, Ub))) return rc;
     } else {
         // CN2D (solution.F90:205-231): six sweeps u = uold + dt*F(0.5*(u + uold))
-        for (int sweep = 1; sweep <= 6; ++sweep)
+        for (int sweep = 1; sweep <= 6; ++sweep) {
             if ((rc = run_stage<T>(h, sweep == 6 ? 4 : 3, gstep, Ub, sweep == 1))) return rc;
+            if (sweep == 1 && ov) HIPCHK(hipStreamWaitEvent(h->stream, h->fork[2], 0));
+        }
     }
     // forcing2D (solution.F90:303-540)
     const int fm = h->p.forcemethod;
@@ -1216,6 +1256,21 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
         delete h;
         return fail(BASTR_EDEVICE, "bastr_create: hipStreamCreate failed");
     }
+    if (nd == 2) {
+        // "1": the two-stream order (measured: graph replay at 2048^2 FP64 0.575 -> 0.553 ms
+        // per iteration; direct launches and small grids slower, so not the default)
+        const char* ov = std::getenv("BASTR_OVERLAP");
+        h->overlap = ov && std::strcmp(ov, "1") == 0;
+    }
+    if (h->overlap) {
+        bool oks = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) == hipSuccess;
+        for (int i = 0; i < 3 && oks; ++i)
+            oks = hipEventCreateWithFlags(&h->fork[i], hipEventDisableTiming) == hipSuccess;
+        if (!oks) {
+            bastr_destroy(h);
+            return fail(BASTR_EDEVICE, "bastr_create: second stream or its events failed");
+        }
+    }
     bool ok = true;
     const size_t cbytes = h->nhalf * 2 * h->esz;
     if (nd == 2) {
@@ -1228,6 +1283,8 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
             h->ureal[c] = dalloc(h->npts * h->esz * M, &ok);
         }
         for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes * M, &ok);
+        if (h->overlap)
+            for (int c = 0; c < 2; ++c) h->WA[c] = dalloc(cbytes * M, &ok);
     } else {
         for (int c = 0; c < 3; ++c) {
             h->U[c] = dalloc(cbytes, &ok);
@@ -1317,6 +1374,12 @@ int bastr_nmembers(const bastr_handle* h) { return h ? h->nmem : -1; }
 int bastr_destroy(bastr_handle* h) {
     if (!h) return BASTR_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->side) (void)hipStreamSynchronize(h->side);
+    for (int c = 0; c < 2; ++c)
+        if (h->WA[c]) (void)hipFree(h->WA[c]);
+    for (hipEvent_t e : h->fork)
+        if (e) (void)hipEventDestroy(e);
+    if (h->side) (void)hipStreamDestroy(h->side);
     void* bufs[] = {h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf, h->part_bins, h->part_tot,
                     h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec, h->mforce};
     for (void* b : bufs)
@@ -1789,7 +1852,8 @@ int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* tot
     if (h->nd == 2) {
         units[BASTR_K_SPECTRA] = 2 * fc * fy;
         units[BASTR_K_COL_INV_VG] = fc * (2 * fy + 7);   // y-inverse: u_hat -> 7 fields
-        units[BASTR_K_VELGRAD] = 9 * fc;                 // x passes: r7 w2
+        units[BASTR_K_VELGRAD] = 6 * fc;                 // stage A's x pass: r4 w2
+        units[BASTR_K_VG_DIS] = 6 * fc;                  // dissipation x pass: r6
         units[BASTR_K_COL_INV] = fc * (2 * fy + 4);      // y-inverse: 2 -> 4
         units[BASTR_K_ROW_STAGE] = 6 * fc;               // r4 w2
         units[BASTR_K_COL_FWD_A] = 2 * fc + 6 * fc * fy; // y-forward of 2, update r U w KA UB
@@ -1851,7 +1915,8 @@ int bastr_prof_copy_gbps(bastr_handle* h, uint64_t bytes, int reps, double* gbps
 
 const char* bastr_prof_name(int c) {
     static const char* names[BASTR_K_NCLASS] = {"spectra", "col_inv_vg", "velgrad", "finalize", "col_inv",
-                                                "row_stage", "col_fwd_a", "col_fwd_b", "col_fwd_c", "force"};
+                                                "row_stage", "col_fwd_a", "col_fwd_b", "col_fwd_c", "force",
+                                                "vg_dis"};
     return (c >= 0 && c < BASTR_K_NCLASS) ? names[c] : "";
 }
 int bastr_fft_backward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, true); }

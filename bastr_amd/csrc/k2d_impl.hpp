@@ -821,13 +821,15 @@ static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
 }
 
 template <typename T, int N>
-static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
+static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk, int which) {
     using C = VgCfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
-    if (!a.hout[0] || !a.hout[1]) return -1;
-    hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
-    hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
+    if (!a.hout[0] || !a.hout[1] || which < 0 || which > 2) return -1;
+    if (which != 2)
+        hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
+    if (which != 1)
+        hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
@@ -854,9 +856,9 @@ int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
     return -1;
 }
 template <typename T>
-int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
+int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk, int which) {
     switch (n) {
-#define X(NN) case NN: return vg_n<T, NN>(a, nrows, s, nblk);
+#define X(NN) case NN: return vg_n<T, NN>(a, nrows, s, nblk, which);
         K2D_SIZES(X)
 #undef X
     }
@@ -891,7 +893,7 @@ int launch_scale(const ScaleArgs<T>& a, hipStream_t s) {
 #define K2D_INSTANTIATE(T)                                                                     \
     template int launch_update<T>(int, const ColFwdArgs<T>&, hipStream_t);                      \
     template int launch_row<T>(int, int, const RowArgs<T>&, int, hipStream_t);                 \
-    template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*);             \
+    template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*, int);        \
     template int launch_spectra<T>(const SpecArgs<T>&, int, hipStream_t);                      \
     template int launch_force_band<T>(const ForceArgs<T>&, hipStream_t);                      \
     template int launch_energy<T>(const EnergyArgs<T>&, int, hipStream_t);                     \

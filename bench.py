@@ -74,7 +74,10 @@ def cpu_model():
     return name, count or os.cpu_count()
 
 
-def read_prof(lib, h, ncls=10):
+NCLASS = 11  # BASTR_K_NCLASS (include/bastr_prof.h)
+
+
+def read_prof(lib, h, ncls=NCLASS):
     out = {}
     for c in range(ncls):
         n = C.c_int64()
@@ -231,7 +234,7 @@ def bench_ensemble(args):
     # warm-up with every kernel class timed; its first iterations give the
     # FP32 spectra for the accuracy check
     for g in groups:
-        lib.bastr_prof_enable(g["ens"]._h, 0x3FF)
+        lib.bastr_prof_enable(g["ens"]._h, (1 << NCLASS) - 1)
     nchk = max(args.warmup, 1)
     warm_sp = advance(nchk, keep=True)
     best = None
@@ -376,7 +379,7 @@ def main():
     s.init_random_phase(k0=4.0, seed=1, energy=1.0)
 
     # warmup, with every kernel class timed to find the dominant one
-    lib.bastr_prof_enable(s._h, 0x3FF)
+    lib.bastr_prof_enable(s._h, (1 << NCLASS) - 1)
     run_steps(s, max(args.warmup, 1), args.chunk)
     warm = read_prof(lib, s._h)
     dom_name, (dom_cls, _, _, _) = max(warm.items(), key=lambda kv: kv[1][2])

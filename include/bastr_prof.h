@@ -2,7 +2,9 @@
  * bastr_prof.h -- per-kernel timing of a bastr handle (no counterpart in the
  * reference, which has no timers beyond an unused MPI_Wtime,
  * parallel.F90:2658-2664).  For benchmarks: when a kernel class is enabled,
- * every launch of it is bracketed by HIP events on the handle's stream and
+ * every launch of it is bracketed by HIP events on the stream it runs on (a
+ * 2-D handle created with BASTR_OVERLAP=1 in the environment runs the spectra,
+ * the dissipation moment and the diagnostics reduction on a second stream) and
  * its elapsed time is accumulated at the end of each bastr_step call.
  */
 #ifndef BASTR_PROF_H
@@ -26,6 +28,9 @@ enum bastr_kclass {
     BASTR_K_COL_FWD_B,
     BASTR_K_COL_FWD_C,
     BASTR_K_FORCE,           /* band forcing                                    */
+    BASTR_K_VG_DIS,          /* 2-D: the dissipation moment's x pass (VELGRAD is
+                                then stage A's x pass); 3-D: unused (VELGRAD
+                                covers both velocity-gradient kernels)           */
     BASTR_K_NCLASS
 };
 
