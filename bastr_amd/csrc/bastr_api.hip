@@ -1325,7 +1325,7 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     h->nblk_en = 512;
     h->part_en = (double*)dalloc(sizeof(double) * h->nblk_en * M, &ok);
     h->dfactor = (double*)dalloc(sizeof(double) * M, &ok);
-    if (forced && nmem > 1) h->mforce = (int*)dalloc(sizeof(int) * M, &ok);
+    if (forced) h->mforce = (int*)dalloc(sizeof(int) * M, &ok);
     if (!ok) {
         bastr_destroy(h);
         return fail(BASTR_ENOMEM, "bastr_create: device allocation failed");

@@ -113,6 +113,23 @@ def test_random_phase_members_and_nan_isolation(hip):
         assert s.get_time()[0] == 4
 
 
+def test_single_member_flags(hip):
+    """A one-member ensemble keeps its forcing flag: forced=[0] under band
+    forcing is the free-decay run."""
+    print("case ensemble of one, unforced", flush=True)
+    shape = (64, 64)
+    p = make_params(*shape, forcemethod=2, lproject=True, kmax=128)
+    u = hitgen(shape, k0=4, seed=8)
+    e = Ensemble(hip, p, 1, [0])
+    e.set_member_fields(0, *u)
+    s = Solver(hip, "bastr", member_params(p, False))
+    s.set_fields(*u)
+    _, de, se = e.step(3)
+    _, ds, ss = s.step(3)
+    same_diags(de[0], ds)
+    assert np.array_equal(se[0], ss)
+
+
 def test_one_field_entry_points_reject_ensembles(hip):
     print("case ensemble argument checks", flush=True)
     p = make_params(32, 32)
