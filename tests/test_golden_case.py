@@ -58,7 +58,10 @@ def test_expected_logs_match_oracle(orc, tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_reproduces_expected_logs(tmp_path):
+def test_driver_reproduces_expected_logs(tmp_path, orc):
+    """The driver's run of the case reproduces the expected logs, and its dump
+    at nstep = 200 (written before that iteration's step, solution.F90:69-79)
+    is the oracle's field after 200 steps to 1e-11 relative L2."""
     print("case golden c1 driver", flush=True)
     work = tmp_path / "c1"
     shutil.copytree(os.path.join(CASE, "datin"), work / "datin")
@@ -66,3 +69,16 @@ def test_driver_reproduces_expected_logs(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:]
     worst = compare(str(work / "log"), EXP, 1e-12)
     assert all(v <= 1e-12 for v in worst.values()), worst
+    f = tmp_path / "flowini2d.h5"
+    shutil.copy(os.path.join(CASE, "datin", "flowini2d.h5"), f)
+    u0 = [h5read(f, v, (256, 256)) for v in ("u1", "u2")]
+    got = [h5read(work / "outdat" / "flowfield0001.h5", v, (256, 256)) for v in ("u1", "u2")]
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    o = orc.solver(make_params(256, 256, forcemethod=0, lproject=False, kmax=128))
+    o.set_fields(*u0)
+    assert o.step(200)[0] == 0
+    ref = o.get_fields()
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(got, ref))
+    err = np.sqrt(num / sum(float(np.sum(b * b)) for b in ref))
+    print("c1 200-step field rel-L2", err)
+    assert err < 1e-11

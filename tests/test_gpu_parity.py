@@ -124,6 +124,24 @@ def test_step_parity_large_grids(hip, orc, n, nsteps):
     assert np.abs(sg - so).max() / np.abs(so[:, 1:, :]).max() < 1e-10
 
 
+def test_long_run_parity_512(hip, orc):
+    """200 loop iterations at 512^2 with band forcing and projection (the C2
+    semantics): field and shell spectrum within 1e-11 of the oracle."""
+    print("case long run 512^2 x 200", flush=True)
+    orc.set_threads(16)
+    p = make_params(512, 512, forcemethod=2, lproject=True, kmax=200)
+    u = mixed_field((512, 512), seed=12)
+    g, o, out = run_pair(hip, orc, p, u, 200)
+    (rg, dg, sg), (ro, do, so) = out[0]
+    assert rg == 0 and ro == 0
+    err = rel_l2(g.get_fields(), o.get_fields())
+    print("512^2 200-step field rel-L2", err)
+    assert err < 1e-11
+    E_g, E_o = sg[-1, 1] + sg[-1, 2], so[-1, 1] + so[-1, 2]
+    assert np.linalg.norm(E_g - E_o) / np.linalg.norm(E_o) < 1e-11
+    check_diags(dg, do)
+
+
 @pytest.mark.parametrize("shape,prec,force,nsteps", [
     ((4096, 64), 0, 2, 3),    # 4096-point rows (one row per workgroup, largest LDS line)
     ((64, 4096), 0, 0, 3),    # 4096-point columns
