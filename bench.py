@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the Bastr pseudo-spectral time step on one MI355X.
 
-Workload (default, --config c2): 2-D viscous Burgers on a 2048^2 periodic
+Workload (default, --config 2d-2048): 2-D viscous Burgers on a 2048^2 periodic
 grid, FP64, Re = 5000, dt = 1e-4, Kutta RK3, band forcing at forcek = 5 with
 target energy 0.075, dilatational projection on, spectra to kmax = 200;
 initial field: Bastrpp's hitgen random-phase construction with k0 = 4
 (E(k) = k^4 exp(-2 (k/4)^2) from k = 4 up, 4^6 k^-2 e^-2 below), seed 1,
-rescaled to u_rms = 1, built on the device.  --config c3 is the 3-D 512^3 case,
-c2-fp32 / c3-fp32 the FP32 variants, c1 the 256^2 free-decay case.
+rescaled to u_rms = 1, built on the device.  --config 3d-512 is the 3-D 512^3
+case, 2d-2048-fp32 / 3d-512-fp32 the FP32 variants, 2d-256-decay the 256^2
+free-decay case, 2d-ensemble96 the 96-member ensemble of 2-D runs.
 A "step" is one iteration of the reference main loop (solution.F90:54-97,
 124-169): spectra + velocity-gradient diagnostics, RK3 step, forcing.  The
 field stays resident in HBM for the whole timed region; bastr_step is called
@@ -41,11 +42,11 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # grid, precision, forcemethod, lproject, kmax, default timed steps, oracle sample steps
-    "c1": dict(grid=(256, 256), prec=0, force=0, proj=False, kmax=128, steps=2000, cpu=200),
-    "c2": dict(grid=(2048, 2048), prec=0, force=2, proj=True, kmax=200, steps=1000, cpu=10),
-    "c2-fp32": dict(grid=(2048, 2048), prec=1, force=2, proj=True, kmax=200, steps=1000, cpu=10),
-    "c3": dict(grid=(512, 512, 512), prec=0, force=2, proj=True, kmax=256, steps=100, cpu=1),
-    "c3-fp32": dict(grid=(512, 512, 512), prec=1, force=2, proj=True, kmax=256, steps=100, cpu=1),
+    "2d-256-decay": dict(grid=(256, 256), prec=0, force=0, proj=False, kmax=128, steps=2000, cpu=200),
+    "2d-2048": dict(grid=(2048, 2048), prec=0, force=2, proj=True, kmax=200, steps=1000, cpu=10),
+    "2d-2048-fp32": dict(grid=(2048, 2048), prec=1, force=2, proj=True, kmax=200, steps=1000, cpu=10),
+    "3d-512": dict(grid=(512, 512, 512), prec=0, force=2, proj=True, kmax=256, steps=100, cpu=1),
+    "3d-512-fp32": dict(grid=(512, 512, 512), prec=1, force=2, proj=True, kmax=256, steps=100, cpu=1),
 }
 
 
@@ -170,7 +171,7 @@ def rel_l2(a, b):
     return float(np.sqrt(num / sum(float(np.sum(y * y)) for y in b)))
 
 
-# --config c4: the batched ensemble, one handle per member size, every member
+# --config 2d-ensemble96: the batched ensemble, one handle per member size, every member
 # of a size advanced by the same launches (bastr_create_ensemble)
 ENSEMBLE = [(256, 48), (512, 32), (1024, 16)]
 
@@ -345,17 +346,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--config", default="2d-2048", choices=sorted(CONFIGS) + ["2d-ensemble96"])
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serial", action="store_true", help="c4: step the size groups one after another")
+    ap.add_argument("--serial", action="store_true", help="2d-ensemble96: step the size groups one after another")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
               "MI355X's HBM); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
-    if args.config == "c4":
+    if args.config == "2d-ensemble96":
         return bench_ensemble(args)
 
     import bastr_amd

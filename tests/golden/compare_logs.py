@@ -1,9 +1,9 @@
-"""Compares the log/*.dat files of a run of the C1 case (tests/golden/c1_case)
-with the expected ones, record by record.  For a maintainer who can build the
-reference (gfortran + FFTW-MPI + HDF5): copy tests/golden/c1_case, run Bastr
-in it, then
+"""Compares the log/*.dat files of a run of the 256^2 free-decay case
+(tests/golden/decay2d_256) with the expected ones, record by record.  For a
+maintainer who can build the reference (gfortran + FFTW-MPI + HDF5): copy
+tests/golden/decay2d_256, run Bastr in it, then
 
-  python tests/golden/compare_logs.py <case>/log tests/golden/c1_case/expected/log
+  python tests/golden/compare_logs.py <case>/log tests/golden/decay2d_256/expected/log
 
 Energies are compared relative to Es+Ed of the same record (Es of this purely
 dilatational field is roundoff), every other column relative to its own

@@ -1,4 +1,4 @@
-"""Writes the C1 reference-check case (2-D 256^2, FP64, Re = 5000, dt = 1e-4,
+"""Writes the 256^2 free-decay reference-check case (2-D 256^2, FP64, Re = 5000, dt = 1e-4,
 maxstep = 200, free decay, no projection, spectra to kmax = 128) into OUTDIR:
 
   OUTDIR/datin/input.dat       the reference's input format (readwrite.F90:19-112)
@@ -7,7 +7,7 @@ maxstep = 200, free decay, no projection, spectra to kmax = 128) into OUTDIR:
   OUTDIR/expected/log/*.dat    the logs bastr_run writes for this case
 
 Needs the GPU (the driver runs the step on the device).  Usage:
-  python tests/golden/make_c1_case.py OUTDIR
+  python tests/golden/make_decay2d_256_case.py OUTDIR
 """
 import os
 import shutil

@@ -1,5 +1,5 @@
-"""The C1 reference-check case (tests/golden/c1_case, written by
-tests/golden/make_c1_case.py): 256^2 FP64 free decay, 201 loop iterations.
+"""The 256^2 free-decay reference-check case (tests/golden/decay2d_256,
+written by tests/golden/make_decay2d_256_case.py): 256^2 FP64 free decay, 201 loop iterations.
 Its expected logs are pinned here against the oracle on the CPU (so the kit a
 maintainer runs the reference binary on is itself checked), and the driver
 reproduces them on the GPU."""
@@ -14,7 +14,7 @@ import pytest
 from bastr_amd.solver import make_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CASE = os.path.join(ROOT, "tests", "golden", "c1_case")
+CASE = os.path.join(ROOT, "tests", "golden", "decay2d_256")
 EXP = os.path.join(CASE, "expected", "log")
 sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 from compare_logs import compare, read_log  # noqa: E402
@@ -62,8 +62,8 @@ def test_driver_reproduces_expected_logs(tmp_path, orc):
     """The driver's run of the case reproduces the expected logs, and its dump
     at nstep = 200 (written before that iteration's step, solution.F90:69-79)
     is the oracle's field after 200 steps to 1e-11 relative L2."""
-    print("case golden c1 driver", flush=True)
-    work = tmp_path / "c1"
+    print("case golden decay2d_256 driver", flush=True)
+    work = tmp_path / "decay2d_256"
     shutil.copytree(os.path.join(CASE, "datin"), work / "datin")
     r = subprocess.run([DRIVER], cwd=work, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:]
@@ -80,5 +80,5 @@ def test_driver_reproduces_expected_logs(tmp_path, orc):
     ref = o.get_fields()
     num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(got, ref))
     err = np.sqrt(num / sum(float(np.sum(b * b)) for b in ref))
-    print("c1 200-step field rel-L2", err)
+    print("256^2 decay 200-step field rel-L2", err)
     assert err < 1e-11

@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through the C-ABI) against the CPU oracle.
 
-Tolerances (FP64 path): fields 1e-11 relative L2 (the north-star bar);
+Tolerances (FP64 path): fields 1e-11 relative L2;
 scalar diagnostics 1e-10 relative; shell spectra 1e-10 relative to the
 largest shell value.  FP32 path: fields and spectra 1e-5 relative to FP64.
 """
@@ -82,8 +82,8 @@ def test_set_get_fields_is_entry_dealias(hip, orc):
 
 
 @pytest.mark.parametrize("shape,force,proj,nsteps", [
-    ((256, 256), 0, False, 20),   # C1 configuration, shortened
-    ((128, 128), 2, True, 15),    # band forcing + projection (C2 semantics)
+    ((256, 256), 0, False, 20),   # 256^2 free decay, shortened
+    ((128, 128), 2, True, 15),    # band forcing + projection
     ((64, 128), 0, True, 10),     # non-square grid
     ((128, 64), 2, False, 10),
 ])
@@ -109,7 +109,7 @@ def test_step_parity_fp64(hip, orc, shape, force, proj, nsteps):
 
 @pytest.mark.parametrize("n,nsteps", [(512, 3), (1024, 2), (2048, 2)])
 def test_step_parity_large_grids(hip, orc, n, nsteps):
-    """Benchmark-size grids (C2 semantics: band forcing + projection)."""
+    """Benchmark-size grids (band forcing + projection, kmax = 200)."""
     print(f"case step_large {n}^2 n={nsteps}", flush=True)
     orc.set_threads(16)
     p = make_params(n, n, forcemethod=2, lproject=True, kmax=200)
@@ -125,8 +125,8 @@ def test_step_parity_large_grids(hip, orc, n, nsteps):
 
 
 def test_long_run_parity_512(hip, orc):
-    """200 loop iterations at 512^2 with band forcing and projection (the C2
-    semantics): field and shell spectrum within 1e-11 of the oracle."""
+    """200 loop iterations at 512^2 with band forcing and projection: field
+    and shell spectrum within 1e-11 of the oracle."""
     print("case long run 512^2 x 200", flush=True)
     orc.set_threads(16)
     p = make_params(512, 512, forcemethod=2, lproject=True, kmax=200)
