@@ -320,7 +320,8 @@ __global__ __launch_bounds__(256) void update_kernel(ColFwdArgs<T> a) {
 // x-direction passes
 // ---------------------------------------------------------------------------
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row_kernel(RowArgs<T> a) {
+__global__ __launch_bounds__((RowCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(RowCfg<T, N>::WPE)))
+void row_kernel(RowArgs<T> a) {
     using C = RowCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
     if (a.gate.closed()) return;
