@@ -15,6 +15,8 @@ module bastr_hip
   public :: bastr_init_random_phase, bastr_set_time, bastr_get_time, bastr_step
   public :: bastr_shell_counts, bastr_fft_forward, bastr_fft_backward
   public :: bastr_pp_spectra, bastr_pp_transfer
+  public :: bastr_create_ensemble, bastr_nmembers, bastr_set_member_fields, bastr_get_member_fields
+  public :: bastr_init_random_phase_member, bastr_get_member_time, bastr_step_group
 
   integer(c_int), parameter, public :: BASTR_OK = 0, BASTR_STOP_NAN = 1
   integer(c_int), parameter, public :: BASTR_EINVAL = -1, BASTR_EDEVICE = -2, BASTR_ENOMEM = -3
@@ -125,6 +127,58 @@ module bastr_hip
       real(c_double), intent(in) :: u1(*), u2(*)
       real(c_double), intent(in), optional :: u3(*)
       real(c_double), intent(out) :: out(*)
+    end function
+    ! Ensembles of nmembers independent 2-D runs of one grid (no counterpart in
+    ! the reference): forced(m) = 0 free decay, 1 forced as p says (absent: all
+    ! forced); members are numbered from 0.  bastr_step on an ensemble fills
+    ! diags(m*n + i) and spectra(m*n*3*allkmax + ...) for member m.
+    integer(c_int) function bastr_create_ensemble(p, nmembers, forced, h) bind(C, name='bastr_create_ensemble')
+      import
+      type(bastr_params), intent(in) :: p
+      integer(c_int32_t), value :: nmembers
+      integer(c_int32_t), intent(in), optional :: forced(*)
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function bastr_nmembers(h) bind(C, name='bastr_nmembers')
+      import
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function bastr_set_member_fields(h, m, u1, u2, u3) bind(C, name='bastr_set_member_fields')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: m
+      real(c_double), intent(in) :: u1(*), u2(*)
+      real(c_double), intent(in), optional :: u3(*)
+    end function
+    integer(c_int) function bastr_get_member_fields(h, m, u1, u2, u3) bind(C, name='bastr_get_member_fields')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: m
+      real(c_double), intent(out) :: u1(*), u2(*)
+      real(c_double), intent(out), optional :: u3(*)
+    end function
+    integer(c_int) function bastr_init_random_phase_member(h, m, k0, seed, energy) &
+        bind(C, name='bastr_init_random_phase_member')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: m
+      real(c_double), value :: k0, energy
+      integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function bastr_get_member_time(h, m, nstep, time) bind(C, name='bastr_get_member_time')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: m
+      integer(c_int64_t), intent(out) :: nstep
+      real(c_double), intent(out) :: time
+    end function
+    ! bastr_step on hs(1:nh) at once; diags(i) / spectra(i): c_loc of handle i's
+    ! arrays (or c_null_ptr); the arrays themselves may be absent
+    integer(c_int) function bastr_step_group(hs, nh, n, diags, spectra) bind(C, name='bastr_step_group')
+      import
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: nh, n
+      type(c_ptr), intent(in), optional :: diags(*), spectra(*)
     end function
   end interface
 

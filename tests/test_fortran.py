@@ -105,3 +105,44 @@ def test_fftwlink_dropin_matches_numpy(tmp_path, shape):
     ref = np.fft.fftn(u, axes=axes) / (ia * ja * (ka if ka else 1))  # fft2d/3d: forward / N
     got = np.fromfile(tmp_path / "fftwlink_fwd.bin", dtype=np.complex128).reshape((ia, ja, nz), order="F")
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+ENS_DEMO = os.path.join(ROOT, "bastr_amd", "bin", "bastr_ensemble_demo")
+
+
+@pytest.mark.gpu
+def test_fortran_ensemble_demo_matches_library(hip):
+    """bastr_ensemble_demo drives a 2-member ensemble (forced, free decay) and a
+    one-field run through the Fortran module and bastr_step_group; its
+    diagnostics equal those of the same runs made through the C-ABI from Python."""
+    print("case fortran ensemble demo 64x64", flush=True)
+    from bastr_amd.solver import Ensemble
+    n, nsteps = 64, 4
+    if not os.path.exists(ENS_DEMO):
+        pytest.fail("Fortran ensemble demo not built; run __graft_entry__.build()")
+    r = subprocess.run([ENS_DEMO, str(n), str(nsteps)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ens, one, times = {}, [], {}
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if f and f[0] == "ens":
+            ens.setdefault(int(f[1]), []).append((float(f[3]), float(f[4])))
+        elif f and f[0] == "one":
+            one.append((float(f[2]), float(f[3])))
+        elif f and f[0] == "time":
+            times[int(f[1])] = (int(f[2]), float(f[3]))
+    p = make_params(n, n, forcemethod=2, lproject=True, kmax=128)
+    e = Ensemble(hip, p, 2, [1, 0])
+    for m in range(2):
+        e.init_random_phase_member(m, k0=4.0, seed=m + 1, energy=1.0)
+    _, de, _ = e.step(nsteps)
+    s = Solver(hip, "bastr", p)
+    s.init_random_phase(k0=4.0, seed=3, energy=1.0)
+    _, ds, _ = s.step(nsteps)
+    for m in range(2):
+        assert len(ens[m]) == nsteps
+        for i in range(nsteps):
+            assert ens[m][i] == pytest.approx((de[m][i].Esspe, de[m][i].Edspe), rel=1e-15, abs=1e-300)
+        assert times[m] == e.get_member_time(m)
+    for i in range(nsteps):
+        assert one[i] == pytest.approx((ds[i].Esspe, ds[i].Edspe), rel=1e-15, abs=1e-300)

@@ -112,3 +112,26 @@ def test_device_random_phase_3d(hip, orc):
     o = orc.solver(p)
     o.init_random_phase(k0=4.0, seed=5, energy=1.0)
     assert rel_l2(g.get_fields(), o.get_fields()) < 1e-12
+
+
+def test_long_run_parity_3d(hip, orc):
+    """30 loop iterations at 128^3 with band forcing and projection: field and
+    shell spectrum within 1e-11 of the oracle after the whole run."""
+    print("case step3d long 128^3 x 30", flush=True)
+    orc.set_threads(16)
+    n = 128
+    p = make_params(n, n, ka=n, forcemethod=2, lproject=True, kmax=256)
+    u = mixed3(n, seed=23)
+    g = Solver(hip, "bastr", p)
+    o = orc.solver(p)
+    g.set_fields(*u)
+    o.set_fields(*u)
+    rg, dg, sg = g.step(30)
+    ro, do, so = o.step(30)
+    assert rg == 0 and ro == 0
+    err = rel_l2(g.get_fields(), o.get_fields())
+    print("128^3 30-step field rel-L2", err)
+    assert err < 1e-11
+    E_g, E_o = sg[-1, 1] + sg[-1, 2], so[-1, 1] + so[-1, 2]
+    assert np.linalg.norm(E_g - E_o) / np.linalg.norm(E_o) < 1e-11
+    check_diags(dg, do)
