@@ -15,6 +15,13 @@
  * Parseval for the spectra, single-mode velocity-gradient moments and the
  * band-forcing energy identity.
  *
+ * Third-party boundary: the reference's transforms are FFTW3 / fftw3-mpi
+ * (not vendored; no version pinned in its Makefile:15,27; call sites
+ * fftwlink.F90:65,72,294-298,333,346,356,366).  This file uses its own
+ * radix-2 FFT with the same sign and normalisation conventions, pinned as
+ * above; the rounding of FFTW's own plans (below 1e-14 relative) is parity
+ * unpinned.
+ *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off
  * -fcx-fortran-rules; no fast-math, no FMA contraction, to keep the
  * reference's statement-level rounding).
