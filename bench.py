@@ -42,9 +42,9 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # grid, precision, forcemethod, lproject, kmax, default timed steps, oracle sample steps
-    "2d-256-decay": dict(grid=(256, 256), prec=0, force=0, proj=False, kmax=128, steps=2000, cpu=200),
-    "2d-2048": dict(grid=(2048, 2048), prec=0, force=2, proj=True, kmax=200, steps=1000, cpu=10),
-    "2d-2048-fp32": dict(grid=(2048, 2048), prec=1, force=2, proj=True, kmax=200, steps=1000, cpu=10),
+    "2d-256-decay": dict(grid=(256, 256), prec=0, force=0, proj=False, kmax=128, steps=2000, cpu=2000),
+    "2d-2048": dict(grid=(2048, 2048), prec=0, force=2, proj=True, kmax=200, steps=1000, cpu=20),
+    "2d-2048-fp32": dict(grid=(2048, 2048), prec=1, force=2, proj=True, kmax=200, steps=1000, cpu=20),
     "3d-512": dict(grid=(512, 512, 512), prec=0, force=2, proj=True, kmax=256, steps=100, cpu=1),
     "3d-512-fp32": dict(grid=(512, 512, 512), prec=1, force=2, proj=True, kmax=256, steps=100, cpu=1),
 }
@@ -289,7 +289,7 @@ def bench_ensemble(args):
         oracle.set_threads(threads)
         t_iter, parts = 0.0, []
         for n, M in ENSEMBLE:
-            its = {256: 400, 512: 100, 1024: 25}[n]  # about 2-3 s of CPU work each
+            its = {256: 1200, 512: 300, 1024: 75}[n]  # about 8 s of CPU work each
             fp = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
                              target_energy=0.075, lproject=True, kmax=n // 2, precision=0)
             g0 = bastr_amd.solver(fp)
