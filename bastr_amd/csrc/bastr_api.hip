@@ -1606,8 +1606,12 @@ static int member_check(bastr_handle* h, int member, const char* what) {
     return 0;
 }
 
-static void member_done(bastr_handle* h, int member) {
-    h->mset[member] = 1;
+// have_fields of an ensemble is "every member is set"; the one-field calls a
+// member operation makes set it on their own, so it is recomputed after each
+// member operation, and a failed one leaves that member unset (its buffers may
+// hold part of the new field)
+static void member_done(bastr_handle* h, int member, bool ok = true) {
+    h->mset[member] = ok ? 1 : 0;
     h->have_fields = std::all_of(h->mset.begin(), h->mset.end(), [](char c) { return c != 0; });
 }
 
@@ -1623,8 +1627,8 @@ int bastr_set_member_fields(bastr_handle* h, int32_t member, const double* u1, c
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
             rc = fail(BASTR_EDEVICE, "bastr_set_member_fields: stream synchronisation failed");
     }
+    member_done(h, member, rc == 0);
     if (rc) return rc;
-    member_done(h, member);
     return BASTR_OK;
 }
 
@@ -1653,10 +1657,10 @@ int bastr_init_random_phase_member(bastr_handle* h, int32_t member, double k0, u
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
             rc = fail(BASTR_EDEVICE, "bastr_init_random_phase_member: stream synchronisation failed");
     }
+    member_done(h, member, rc == 0);
     if (rc) return rc;
     h->mnstep[member] = 0;
     h->mtime[member] = 0.0;
-    member_done(h, member);
     return BASTR_OK;
 }
 

@@ -621,11 +621,13 @@ int main(int argc, char** argv) {
     const int K = r.allkmax;
     std::vector<bastr_diag> diags;
     std::vector<double> spec;
+    int64_t dumped_at = -1;  // nstep of the last scheduled dump
     while (nstep <= maxstep) {
         if (in.lwsequ && nstep == nxtwsequ) {
             // the reference dumps inside iteration nstep, after its diagnostics
             // and before its step: the device field is exactly that field
             writeflfed(r, filenumb, nstep, time);
+            dumped_at = nstep;
             filenumb = filenumb + 1;
             nxtwsequ = std::min(nxtwsequ + in.feqwsequ, maxstep);
             std::printf(" Next print is %d\n", nxtwsequ);
@@ -657,8 +659,10 @@ int main(int argc, char** argv) {
             std::printf(" nstep, Es, Ed, Eall= %20lld %24.16E %24.16E %24.16E\n", (long long)g.nstep, g.Esspe,
                         g.Edspe, g.Esspe + g.Edspe);
             if (nan) {
-                // writeflfed then stop (solution.F90:69-81)
-                writeflfed(r, filenumb, g.nstep, g.time);
+                // writeflfed then stop (solution.F90:69-81); one condition covers
+                // both the scheduled dump and the NaN, so an iteration that was
+                // just dumped is not written a second time
+                if (g.nstep != dumped_at) writeflfed(r, filenumb, g.nstep, g.time);
                 std::fflush(nullptr);
                 stop_msg("\"E\" is a NaN", 0);
             }

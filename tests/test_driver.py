@@ -183,6 +183,21 @@ def test_driver_3d_and_nan_stop(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_nan_on_a_dump_iteration_writes_one_file(tmp_path):
+    """The reference's single condition '(lwsequ .and. nstep==nxtwsequ) .or.
+    isnan(E)' (solution.F90:69-79) writes one file when the NaN falls on a
+    scheduled dump: u = 0 with band forcing is NaN from nstep 1, the first
+    dump is at min(feqwsequ, maxstep) = 1."""
+    print("case driver 2d 32^2 nan on dump", flush=True)
+    n = 32
+    write_input(tmp_path, n, n, 0, 5, "1.d-4", True, 1, False, 2, 128, 1, 2, 5, "0.075d0", False, 0)
+    r = run(tmp_path)
+    assert r.returncode == 0 and 'STOP "E" is a NaN' in r.stdout
+    files = sorted(os.listdir(tmp_path / "outdat"))
+    assert files == ["flowfield0001.h5"], files
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("shape", [(64, 32), (16, 16, 16)])
 def test_driver_postprocessing_commands(tmp_path, orc, shape):
     """spectra2d / transfer2d / transfer3d NNNN (Bastrpp.F90:174-599): read

@@ -302,7 +302,7 @@ def test_band_forcing_against_numpy(orc):
     assert abs(np.mean(fw[0] ** 2 + fw[1] ** 2) - 0.075) < 1e-14
 
 
-def test_3d_shear_flow_and_solenoidal_projection(orc):
+def test_3d_shear_flow_decays_by_exact_rk3_gain(orc):
     """3-D path (compute_ut3D, RK33D): u = (f(z), g(z), 0) is an exact
     nonlinear-free shear flow decaying by the RK3 gain."""
     n, nsteps, re, dt = 16, 6, 20.0, 1e-2
