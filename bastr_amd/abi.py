@@ -85,9 +85,18 @@ ENSEMBLE_SIGNATURES = [
 ]
 
 
+# The step without device allocation (HIP library only; no counterpart in the
+# reference or the oracle).
+NOALLOC_SIGNATURES = [
+    ("step_noalloc", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
+]
+
+
 def bind(lib, prefix, ensemble=False):
-    """Attach argtypes/restype for every entry point; raise if one is missing."""
-    for name, res, args in SIGNATURES + (ENSEMBLE_SIGNATURES if ensemble else []):
+    """Attach argtypes/restype for every entry point; raise if one is missing.
+    ensemble=True also binds the HIP library's own entry points (ensembles,
+    bastr_step_group, bastr_step_noalloc)."""
+    for name, res, args in SIGNATURES + (ENSEMBLE_SIGNATURES + NOALLOC_SIGNATURES if ensemble else []):
         fn = getattr(lib, f"{prefix}_{name}")  # AttributeError = missing export
         fn.restype = res
         fn.argtypes = args

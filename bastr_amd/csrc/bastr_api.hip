@@ -152,6 +152,7 @@ struct bastr_handle {
     // the current launch width: nmem, or 1 while a MemberView points the
     // buffers at one member.
     int nmem = 1, lw = 1;
+    int dev = 0;                      // device current at bastr_create
     int* mforce = nullptr;            // [nmem] device: member is band-forced (null: all are)
     std::vector<char> mset;           // member's field has been set
     std::vector<int64_t> mnstep;      // per-member nstep / time (ensembles)
@@ -1180,6 +1181,109 @@ int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Step without allocation (bastr_step_noalloc).  The step's per-call device
+// scratch -- the NaN status word, the partial sums of the spectra, the
+// velocity-gradient moments and the forcing energy, the reduced sums, the
+// forcing factor and the per-iteration diagnostic and spectrum slots, which
+// bastr_step keeps in handle buffers (the slots grow with the step count by
+// hipMalloc) -- is carved out of this fixed array instead.  Each device holds
+// its own copy for the life of the process; its address on a device is
+// resolved once, with that device current, when the first handle is created
+// there.  Calls on one device share the array, so they are serialised by a
+// per-device lock (a call is synchronous and holds it until its results are
+// copied out).
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_init_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // scratch's device address per device
+std::mutex g_scratch_mu[kMaxDevices];     // one noalloc call per device at a time
+
+// Resolves scratch's address on the current device (once per device).
+int scratch_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(BASTR_EDEVICE, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(BASTR_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+    return 0;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte offsets of a one-field handle's step scratch for a call of nsteps
+// iterations (the sizes of the handle's own buffers, create_impl / ensure_capacity).
+struct ScratchLayout {
+    size_t status, bins, tot, vg, red, en, factor, diag, spec, total;
+};
+
+ScratchLayout scratch_layout(const bastr_handle* h, int nsteps) {
+    const size_t K = (size_t)h->allkmax;
+    ScratchLayout L{};
+    size_t o = 0;
+    auto take = [&o](size_t bytes) { const size_t at = o; o = align256(o + bytes); return at; };
+    L.status = take(sizeof(int));
+    L.bins = take(sizeof(double) * (size_t)h->nblk_spec * (K + 1) * 2);
+    L.tot = take(sizeof(double) * (size_t)h->nblk_spec * 4);
+    L.vg = take(sizeof(double) * (size_t)h->nblk_vg * 8);
+    L.red = take(sizeof(double) * (2 * K + 12));
+    L.en = take(sizeof(double) * (size_t)h->nblk_en);
+    L.factor = take(sizeof(double));
+    L.diag = take(sizeof(bastr_diag) * (size_t)nsteps);
+    L.spec = take(sizeof(double) * 3 * (K > 0 ? K : 1) * (size_t)nsteps);
+    L.total = o;
+    return L;
+}
+
+// Points the handle's step scratch at the array for one call: direct launches
+// (no graph instantiation), no profiling events, slot capacity = nsteps.
+struct ScratchView {
+    bastr_handle* h;
+    int* status;
+    double *part_bins, *part_tot, *part_vg, *red, *part_en, *dfactor, *dspec;
+    bastr_diag* ddiag;
+    int cap;
+    bool use_graph;
+    uint32_t prof_mask;
+    ScratchView(bastr_handle* h_, char* base, const ScratchLayout& L, int nsteps) : h(h_) {
+        status = h->status; part_bins = h->part_bins; part_tot = h->part_tot; part_vg = h->part_vg;
+        red = h->red; part_en = h->part_en; dfactor = h->dfactor; ddiag = h->ddiag; dspec = h->dspec;
+        cap = h->cap; use_graph = h->use_graph; prof_mask = h->prof_mask;
+        h->status = (int*)(base + L.status);
+        h->part_bins = (double*)(base + L.bins);
+        h->part_tot = (double*)(base + L.tot);
+        h->part_vg = (double*)(base + L.vg);
+        h->red = (double*)(base + L.red);
+        h->part_en = (double*)(base + L.en);
+        h->dfactor = (double*)(base + L.factor);
+        h->ddiag = (bastr_diag*)(base + L.diag);
+        h->dspec = (double*)(base + L.spec);
+        h->cap = nsteps;
+        h->use_graph = false;
+        h->prof_mask = 0;
+    }
+    ~ScratchView() {
+        h->status = status; h->part_bins = part_bins; h->part_tot = part_tot; h->part_vg = part_vg;
+        h->red = red; h->part_en = part_en; h->dfactor = dfactor; h->ddiag = ddiag; h->dspec = dspec;
+        h->cap = cap; h->use_graph = use_graph; h->prof_mask = prof_mask;
+    }
+};
+
+__global__ void zero_doubles_kernel(double* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bastr_last_error(void) { return g_err; }
@@ -1211,7 +1315,11 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     if ((p->forcemethod == 2 || p->forcemethod == 5) && 2 * p->forcek + 2 > p->ja)
         return fail(BASTR_EINVAL, "bastr_create: forcek %d too large for ja=%d", p->forcek, p->ja);
 
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(BASTR_EDEVICE, "bastr_create: hipGetDevice failed");
+    if (int rs = scratch_init(dev)) return rs;
     bastr_handle* h = new bastr_handle();
+    h->dev = dev;
     h->p = *p;
     h->nd = nd;
     h->nmem = h->lw = nmem;
@@ -1681,6 +1789,34 @@ int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spect
     if (nsteps == 0) return BASTR_OK;
     return h->prec == BASTR_PREC_F32 ? step_impl<float>(h, nsteps, diags, spectra)
                                      : step_impl<double>(h, nsteps, diags, spectra);
+}
+
+int bastr_step_noalloc(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_step_noalloc: NULL handle");
+    if (nsteps < 0) return fail(BASTR_EINVAL, "bastr_step_noalloc: nsteps < 0");
+    if (h->nmem > 1) return fail(BASTR_EINVAL, "bastr_step_noalloc: ensemble handles take bastr_step");
+    if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_step_noalloc: fields not set");
+    if (nsteps == 0) return BASTR_OK;
+    void* base = g_scratch_ptr[h->dev];
+    if (!base) return fail(BASTR_EDEVICE, "bastr_step_noalloc: scratch address of device %d not resolved", h->dev);
+    const ScratchLayout L = scratch_layout(h, nsteps);
+    if (L.total > kScratchBytes)
+        return fail(BASTR_EINVAL,
+                    "bastr_step_noalloc: %d iterations need %zu bytes of scratch, more than its %zu "
+                    "(call with fewer iterations)", nsteps, L.total, kScratchBytes);
+    std::lock_guard<std::mutex> lk(g_scratch_mu[h->dev]);
+    int rc;
+    {
+        ScratchView v(h, (char*)base, L, nsteps);
+        // the spectrum slots start at zero, as ensure_capacity leaves them
+        const size_t nspec = 3 * (size_t)(h->allkmax > 0 ? h->allkmax : 1) * nsteps;
+        hipLaunchKernelGGL(zero_doubles_kernel, dim3(64), dim3(256), 0, h->stream, h->dspec, nspec);
+        rc = hipGetLastError() == hipSuccess ? 0 : fail(BASTR_EDEVICE, "bastr_step_noalloc: zero launch failed");
+        if (!rc) rc = h->prec == BASTR_PREC_F32 ? step_launch<float>(h, nsteps) : step_launch<double>(h, nsteps);
+        if (!rc) rc = step_collect(h, nsteps, diags, spectra);
+        if (rc < 0) (void)hipStreamSynchronize(h->stream);  // nothing still reads the array
+    }
+    return rc;
 }
 
 int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_diag* const* diags,

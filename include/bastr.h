@@ -160,6 +160,25 @@ int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_
                      double* const* spectra);
 
 /*
+ * bastr_step without device allocation.  No counterpart in the reference:
+ * for callers that may not allocate device memory once set up (tight serving
+ * loops, harnesses that forbid hipMalloc).  Results, layout, NaN semantics
+ * and nstep/time bookkeeping are those of bastr_step on the same handle,
+ * which must be a one-field handle (ensembles take bastr_step).  The step's
+ * per-call scratch (NaN status, partial and reduced sums, forcing factor,
+ * per-iteration diagnostic and spectrum slots) lives in a fixed 16 MiB device
+ * array of the library, one per device, instead of handle buffers; nothing is
+ * allocated or freed and no graph is instantiated (launches are direct).
+ * Needs about 256 B per iteration of diagnostics, 24*allkmax B per iteration
+ * of spectra and 16*(allkmax+1) B per spectra block (at most 512 blocks);
+ * a call whose total exceeds 16 MiB (e.g. more than ~3300 iterations at
+ * allkmax = 200) returns BASTR_EINVAL: split it.  Calls on one device share
+ * the array and are serialised by the library; the array keeps the last
+ * call's contents, which no call reads before writing.
+ */
+int bastr_step_noalloc(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spectra);
+
+/*
  * Post-processing of a field, e.g. a dump (Bastrpp's commands; the solver state,
  * nstep and time of the handle are left as they were).  Fields are Fortran-order
  * real arrays as for bastr_set_fields, used as given (no entry dealias).

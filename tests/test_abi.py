@@ -21,7 +21,8 @@ def declared_entry_points():
 
 def test_header_declares_expected_entry_points():
     names = declared_entry_points()
-    expected = sorted("bastr_" + n for n, _, _ in abi.SIGNATURES + abi.ENSEMBLE_SIGNATURES)
+    expected = sorted("bastr_" + n for n, _, _ in abi.SIGNATURES + abi.ENSEMBLE_SIGNATURES
+                      + abi.NOALLOC_SIGNATURES)
     assert names == expected
 
 
