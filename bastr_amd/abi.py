@@ -15,6 +15,9 @@ BASTR_ENOMEM = -3
 PREC_F64 = 0
 PREC_F32 = 1
 
+LAW_HITGEN = 0   # Bastrpp's piecewise hitgen law (Bastrpp.F90:654-658)
+LAW_PURE = 1     # E(k) = k^4 exp(-2 (k/k0)^2) at every k
+
 
 class Params(C.Structure):
     """bastr_params: run parameters of datin/input.dat (readwrite.F90:35-44)."""
@@ -66,6 +69,7 @@ SIGNATURES = [
     ("step", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
     ("shell_counts", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("init_random_phase", C.c_int, [_P, C.c_double, C.c_uint64, C.c_double]),
+    ("set_spectrum_law", C.c_int, [_P, C.c_int32]),
     ("pp_spectra", C.c_int, [_P, _D, _D, _D, _D, _D]),
     ("pp_transfer", C.c_int, [_P, _D, _D, _D, _D]),
     ("fft_forward", C.c_int, [_P, _D]),

@@ -153,6 +153,7 @@ struct bastr_handle {
     // buffers at one member.
     int nmem = 1, lw = 1;
     int dev = 0;                      // device current at bastr_create
+    int law = BASTR_LAW_HITGEN;       // spectrum law of the random-phase generator
     int* mforce = nullptr;            // [nmem] device: member is band-forced (null: all are)
     std::vector<char> mset;           // member's field has been set
     std::vector<int64_t> mnstep;      // per-member nstep / time (ensembles)
@@ -1621,6 +1622,7 @@ static int init_random_phase_impl(bastr_handle* h, double k0, uint64_t seed, dou
     b.nh = h->nh;
     b.hp = h->hp;
     b.k0 = k0;
+    b.law = h->law;
     LAUNCHCHK(rp::launch_build<T>(b, h->stream));
     // 4. rescale to <sum u_i^2> = energy (Parseval over the half spectrum)
     std::vector<double> part(h->nblk_en);
@@ -1683,6 +1685,14 @@ int bastr_init_random_phase(bastr_handle* h, double k0, uint64_t seed, double en
     h->have_fields = true;
     h->nstep = 0;
     h->time = 0.0;
+    return BASTR_OK;
+}
+
+int bastr_set_spectrum_law(bastr_handle* h, int32_t law) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_set_spectrum_law: NULL handle");
+    if (law != BASTR_LAW_HITGEN && law != BASTR_LAW_PURE)
+        return fail(BASTR_EINVAL, "bastr_set_spectrum_law: law %d is neither BASTR_LAW_HITGEN nor BASTR_LAW_PURE", law);
+    h->law = law;
     return BASTR_OK;
 }
 

@@ -44,10 +44,11 @@ __global__ __launch_bounds__(256) void build_kernel(RpArgs a) {
         cx<T> o[3] = {{0, 0}, {0, 0}, {0, 0}};
         if (kx < a.nh && kk > 0) {
             const double t = atan2((double)nz.y, (double)nz.x);
-            // E(k) = k^4 exp(-2 (k/k0)^2) from k0 up, k0^6 k^-2 e^-2 below (Bastrpp.F90:654-658)
+            // hitgen law: E(k) = k^4 exp(-2 (k/k0)^2) from k0 up, k0^6 k^-2 e^-2 below
+            // (Bastrpp.F90:654-658); pure law: k^4 exp(-2 (k/k0)^2) at every k
             const double q = kk / a.k0, kk2 = kk * kk, k0 = a.k0;
-            const double E = kk >= k0 ? kk2 * kk2 * exp(-2.0 * (q * q))
-                                      : k0 * k0 * k0 * k0 * k0 * k0 / kk2 * exp(-2.0);
+            const double E = (kk >= k0 || a.law == 1) ? kk2 * kk2 * exp(-2.0 * (q * q))
+                                                     : k0 * k0 * k0 * k0 * k0 * k0 / kk2 * exp(-2.0);
             const double A = a.nd == 2 ? sqrt(E / (2.0 * M_PI * kk)) : sqrt(E / (4.0 * M_PI * kk * kk));
             const double st = sin(t), ct = cos(t);
             const double kc[3] = {(double)kx, (double)ky, (double)kz};

@@ -9,6 +9,7 @@ struct RpArgs {
     void* U[3];       // half spectra; U[0] holds the noise spectrum on entry
     int nd, n0, n1, n2, nh, hp;
     double k0;
+    int law;          // BASTR_LAW_HITGEN or BASTR_LAW_PURE (include/bastr.h)
 };
 
 template <typename T> int launch_noise(void* out, size_t n, uint64_t seed, hipStream_t s);

@@ -36,12 +36,16 @@ def splitmix_noise(shape, seed):
     return ((x >> np.uint64(11)).astype(np.float64) * 2.0 ** -53).reshape(shape, order="F")
 
 
-def spectrum_law(kk, k0):
-    """E(k) of Bastrpp.F90:654-658 (kk > 0)."""
-    return np.where(kk >= k0, kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2), k0 ** 6 / kk ** 2 * np.exp(-2.0))
+def spectrum_law(kk, k0, law=0):
+    """E(k) of Bastrpp.F90:654-658 (law 0, kk > 0), or k^4 exp(-2 (k/k0)^2) at
+    every k (law 1, the pure law)."""
+    pure = kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
+    if law == 1:
+        return pure
+    return np.where(kk >= k0, pure, k0 ** 6 / kk ** 2 * np.exp(-2.0))
 
 
-def hitgen(shape, k0=4.0, seed=0, energy=1.0):
+def hitgen(shape, k0=4.0, seed=0, energy=1.0, law=0):
     shape = tuple(int(s) for s in shape)
     nd = len(shape)
     noise = splitmix_noise(shape, seed)
@@ -49,7 +53,7 @@ def hitgen(shape, k0=4.0, seed=0, energy=1.0):
     ks = np.meshgrid(*[_wavenumbers(n) for n in shape], indexing="ij")
     kk = np.sqrt(sum(k * k for k in ks))
     safe = np.where(kk > 0, kk, 1.0)
-    E = spectrum_law(safe, float(k0))
+    E = spectrum_law(safe, float(k0), law)
     A = np.sqrt(E / (2 * np.pi * safe)) if nd == 2 else np.sqrt(E / (4 * np.pi * safe ** 2))
     A = np.where(kk > 0, A, 0.0)
     phase = np.sin(theta) + 1j * np.cos(theta)

@@ -17,10 +17,12 @@ module bastr_hip
   public :: bastr_pp_spectra, bastr_pp_transfer
   public :: bastr_create_ensemble, bastr_nmembers, bastr_set_member_fields, bastr_get_member_fields
   public :: bastr_init_random_phase_member, bastr_get_member_time, bastr_step_group
+  public :: bastr_set_spectrum_law, bastr_step_noalloc
 
   integer(c_int), parameter, public :: BASTR_OK = 0, BASTR_STOP_NAN = 1
   integer(c_int), parameter, public :: BASTR_EINVAL = -1, BASTR_EDEVICE = -2, BASTR_ENOMEM = -3
   integer(c_int32_t), parameter, public :: BASTR_PREC_F64 = 0, BASTR_PREC_F32 = 1
+  integer(c_int32_t), parameter, public :: BASTR_LAW_HITGEN = 0, BASTR_LAW_PURE = 1
 
   ! commvar.F90:16-66 run parameters (bastr.h: bastr_params)
   type, bind(C) :: bastr_params
@@ -75,6 +77,11 @@ module bastr_hip
       type(c_ptr), value :: h
       real(c_double), value :: k0, energy
       integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function bastr_set_spectrum_law(h, law) bind(C, name='bastr_set_spectrum_law')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: law
     end function
     integer(c_int) function bastr_set_time(h, nstep, time) bind(C, name='bastr_set_time')
       import
@@ -174,6 +181,13 @@ module bastr_hip
     end function
     ! bastr_step on hs(1:nh) at once; diags(i) / spectra(i): c_loc of handle i's
     ! arrays (or c_null_ptr); the arrays themselves may be absent
+    integer(c_int) function bastr_step_noalloc(h, n, diags, spectra) bind(C, name='bastr_step_noalloc')
+      import
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      type(bastr_diag), intent(out) :: diags(*)
+      real(c_double), intent(out), optional :: spectra(*)
+    end function
     integer(c_int) function bastr_step_group(hs, nh, n, diags, spectra) bind(C, name='bastr_step_group')
       import
       type(c_ptr), intent(in) :: hs(*)

@@ -87,6 +87,10 @@ class Solver:
         spectrum peak k0 (the reference's forcek), <sum u_i^2> = energy."""
         self._check(self._fn("init_random_phase")(self._h, float(k0), int(seed), float(energy)))
 
+    def set_spectrum_law(self, law):
+        """Law of later init_random_phase calls: abi.LAW_HITGEN (default) or abi.LAW_PURE."""
+        self._check(self._fn("set_spectrum_law")(self._h, int(law)))
+
     def get_fields(self):
         fs = [np.empty(self.shape, dtype=np.float64, order="F") for _ in range(self.nd)]
         ptrs = [_dptr(f) for f in fs] + [None] * (3 - self.nd)

@@ -4,11 +4,16 @@
 Workload (default, --config 2d-2048): 2-D viscous Burgers on a 2048^2 periodic
 grid, FP64, Re = 5000, dt = 1e-4, Kutta RK3, band forcing at forcek = 5 with
 target energy 0.075, dilatational projection on, spectra to kmax = 200;
-initial field: Bastrpp's hitgen random-phase construction with k0 = 4
-(E(k) = k^4 exp(-2 (k/4)^2) from k = 4 up, 4^6 k^-2 e^-2 below), seed 1,
-rescaled to u_rms = 1, built on the device.  --config 3d-512 is the 3-D 512^3
-case, 2d-2048-fp32 / 3d-512-fp32 the FP32 variants, 2d-256-decay the 256^2
-free-decay case, 2d-ensemble96 the 96-member ensemble of 2-D runs.
+initial field: Bastrpp's hitgen random-phase construction with the spectrum
+law E(k) = k^4 exp(-2 (k/4)^2) at every k (k0 = 4), seed 1, rescaled to
+u_rms = 1, built on the device.  The default run also times the rest of the
+metric ("legs"): the FP32 variant of the 2048^2 workload and the 3-D 512^3
+workload (band forcing, projection, kmax = 256) in FP64 and FP32, each with
+its own dominant-kernel roofline, and reports FP32 over FP64 throughput and
+the FP32 E(k) error against the FP64 device run of the same iterations.
+--config 3d-512 is the 3-D 512^3 case alone, 2d-2048-fp32 / 3d-512-fp32 the
+FP32 variants, 2d-256-decay the 256^2 free-decay case, 2d-ensemble96 the
+96-member ensemble of 2-D runs.
 A "step" is one iteration of the reference main loop (solution.F90:54-97,
 124-169): spectra + velocity-gradient diagnostics, RK3 step, forcing.  The
 field stays resident in HBM for the whole timed region; bastr_step is called
@@ -19,7 +24,9 @@ would.
 Prints one JSON line.  `value` = grid points x steps / wall seconds.
 `roofline` is the dominant kernel class: its compulsory HBM bytes per launch
 over its mean duration, timed with HIP events on the library's stream during
-the timed region.  `step_roofline` prices the whole step at the reference
+the timed region (whose step calls therefore launch directly);
+`graph_replay` times the same number of iterations again with no events, the
+step calls then replayed as captured HIP graphs.  `step_roofline` prices the whole step at the reference
 algorithm's transform count (one compulsory real round trip, 2*s*N^d bytes,
 per whole-field transform).  `cpu_baseline` is the FP64 CPU restatement
 (oracle/) timed on this host's cores over a bounded sample of the same
@@ -36,6 +43,8 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+LAW_PURE = 1  # bastr_set_spectrum_law: E(k) = k^4 exp(-2 (k/k0)^2) at every k
 
 METRIC = "grid-pt·steps/s at 2048² & 512³ (FP64 vs FP32), % HBM roofline, E(k) err"
 HBM_PEAK_GBS = 8000.0
@@ -111,13 +120,18 @@ def traffic_from_profile(kclass, prec, grid):
 
 
 def run_steps(s, n, chunk):
+    """n iterations in calls of at most `chunk`; returns the last iteration's
+    (kn, Es, Ed)."""
     done = 0
+    last = None
     while done < n:
         k = min(chunk, n - done)
-        rc, _, _ = s.step(k, want_spectra=True)
+        rc, _, sp = s.step(k, want_spectra=True)
         if rc != 0:
             raise RuntimeError(f"bastr_step returned {rc}")
+        last = sp[-1]
         done += k
+    return last
 
 
 def cpu_baseline(params, fields, sample_steps):
@@ -197,6 +211,7 @@ def bench_ensemble(args):
                         target_energy=0.075, lproject=True, kmax=n // 2, precision=1)
         forced = [m % 2 for m in range(M)]
         e = bastr_amd.ensemble(p, M, forced)
+        e.set_spectrum_law(LAW_PURE)
         k0s = rng.integers(3, 9, size=M)
         for m in range(M):
             e.init_random_phase_member(m, k0=float(k0s[m]), seed=seed + m, energy=1.0)
@@ -272,6 +287,7 @@ def bench_ensemble(args):
                              forcemethod=2 if g["forced"][m] else 0, forcek=5, target_energy=0.075,
                              lproject=True, kmax=g["n"] // 2, precision=0)
             ref = bastr_amd.solver(fp)
+            ref.set_spectrum_law(LAW_PURE)
             ref.init_random_phase(k0=float(g["k0"][m]), seed=g["seed0"] + m, energy=1.0)
             _, _, rsp = ref.step(nchk)
             ref.close()
@@ -292,8 +308,7 @@ def bench_ensemble(args):
             its = {256: 1200, 512: 300, 1024: 75}[n]  # about 8 s of CPU work each
             fp = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
                              target_energy=0.075, lproject=True, kmax=n // 2, precision=0)
-            g0 = bastr_amd.solver(fp)
-            g0.init_random_phase(k0=4.0, seed=1, energy=1.0)
+            g0 = new_field(bastr_amd, fp)
             u = g0.get_fields()
             g0.close()
             o = oracle.solver(fp)
@@ -322,7 +337,8 @@ def bench_ensemble(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded random-phase fields (hitgen law), k0 uniform in {3..8}, one seed per member, u_rms=1",
+        "data": "synthetic: seeded random-phase fields, E(k) = k^4 exp(-2 (k/k0)^2) (pure law), k0 uniform in "
+                "{3..8}, one seed per member, u_rms=1, built on the device",
         "config": {"workload": "ensemble of 96 independent 2-D Burgers runs: 48 x 256^2, 32 x 512^2, 16 x 1024^2 "
                                "(28,311,552 points), Re=5000, dt=1e-4, RK3, band forcing k=5 E_t=0.075 on every "
                                "other member, free decay on the rest, lproject=t, spectra kmax=n/2",
@@ -341,6 +357,110 @@ def bench_ensemble(args):
     print(json.dumps(out))
 
 
+def cfg_params(name, precision=None):
+    from bastr_amd.solver import make_params
+    cfg = CONFIGS[name]
+    grid = cfg["grid"]
+    return make_params(grid[0], grid[1], grid[2] if len(grid) == 3 else 0, reynolds=5000.0, deltat=1e-4,
+                       timemethod=1, forcemethod=cfg["force"], forcek=5, target_energy=0.075,
+                       lproject=cfg["proj"], kmax=cfg["kmax"],
+                       precision=cfg["prec"] if precision is None else precision)
+
+
+def new_field(bastr_amd, params):
+    """A handle holding the workload's initial field: the pure-law random-phase
+    field (k0 = 4, seed 1, u_rms = 1) built on the device."""
+    from bastr_amd import abi
+    s = bastr_amd.solver(params)
+    s.set_spectrum_law(abi.LAW_PURE)
+    s.init_random_phase(k0=4.0, seed=1, energy=1.0)
+    return s
+
+
+def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False, graph=False):
+    """Time `steps` iterations of workload `name` after `warmup` untimed ones.
+    The timed region brackets the dominant kernel class (the largest share of
+    the warm-up, all classes timed) with HIP events on the library's stream;
+    `graph`: afterwards time as many iterations again with no events, which
+    replays the step call as a captured HIP graph (the path a caller gets).
+    Returns the record and the last iteration's spectrum."""
+    cfg = CONFIGS[name]
+    grid, prec = cfg["grid"], cfg["prec"]
+    s = new_field(bastr_amd, cfg_params(name))
+    lib.bastr_prof_enable(s._h, (1 << NCLASS) - 1)
+    run_steps(s, max(warmup, 1), chunk)
+    warm = read_prof(lib, s._h)
+    dom_name, (dom_cls, _, _, _) = max(warm.items(), key=lambda kv: kv[1][2])
+    for cname, (c, n, ms, by) in sorted(warm.items(), key=lambda kv: -kv[1][2]):
+        if n:
+            print(f"  [{name}] warmup {cname:12s} launches={n:6d} mean={ms / n * 1e3:9.2f} us "
+                  f"bytes/launch={by / 1e6:8.2f} MB  {by / (ms / n * 1e-3) / 1e9:8.1f} GB/s",
+                  file=sys.stderr, flush=True)
+    lib.bastr_prof_enable(s._h, 1 << dom_cls)
+    t0 = time.perf_counter()
+    last = run_steps(s, steps, chunk)
+    wall = time.perf_counter() - t0
+    print(f"  [{name}] timed {steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    prof = read_prof(lib, s._h)
+    _, n_l, ms_l, by_l = prof[dom_name]
+    mean_s = ms_l / n_l * 1e-3
+    achieved = by_l / mean_s / 1e9
+    copy_gbps = float("nan")
+    if copy:  # measured device copy bandwidth (1 GiB, read + write), beside the nominal peak
+        cg = C.c_double()
+        if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(cg)) == 0:
+            copy_gbps = cg.value
+    s.close()
+    graph_ms = None
+    if graph:
+        # a second run from the same initial field (the workload's field blows up
+        # after about 1860 iterations at 2048^2), unprofiled: the warm-up's last
+        # call captures the graph of the chunk length the timed calls replay
+        g = new_field(bastr_amd, cfg_params(name))
+        run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
+        t1 = time.perf_counter()
+        run_steps(g, steps, chunk)
+        graph_ms = (time.perf_counter() - t1) / steps * 1e3
+        g.close()
+    npts = 1
+    for n in grid:
+        npts *= n
+    T = reference_transforms(len(grid), 1, cfg["force"])
+    b_step = T * 2 * (4 if prec == 1 else 8) * npts
+    rec = {
+        "value": npts * steps / wall, "unit": "grid-pt·steps/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": wall / steps * 1e3, "dtype": "f32" if prec == 1 else "f64",
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(dom_name, prec, grid), "kernel": dom_name,
+                     "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)},
+        "step_roofline": {"reference_transforms": T, "bytes_per_step": b_step,
+                          "achieved": round(b_step * steps / wall / 1e9, 1), "unit": "GB/s",
+                          "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if copy:
+        rec["roofline"]["copy_measured"] = round(copy_gbps, 1)
+        rec["roofline"]["frac_of_copy"] = round(achieved / copy_gbps, 4)
+        rec["step_roofline"]["frac_of_copy"] = round(b_step * steps / wall / 1e9 / copy_gbps, 4)
+    if graph_ms is not None:
+        rec["graph_replay"] = {"ms_per_step": graph_ms, "value": npts / (graph_ms * 1e-3)}
+    return rec, last
+
+
+def workload_desc(name):
+    cfg = CONFIGS[name]
+    grid = cfg["grid"]
+    dims = "x".join(map(str, grid))
+    return (f"{len(grid)}-D viscous Burgers {dims}, Re=5000, dt=1e-4, RK3, "
+            f"{'band forcing k=5 E_t=0.075' if cfg['force'] == 2 else 'free decay'}, "
+            f"lproject={'t' if cfg['proj'] else 'f'}, spectra kmax={cfg['kmax']}")
+
+
+# the other workloads of the metric, timed in the default run beside the
+# headline 2048^2 FP64 one: (name, its FP64 counterpart for the accuracy check)
+LEGS = [("2d-2048-fp32", "2d-2048"), ("3d-512", None), ("3d-512-fp32", "3d-512")]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,8 +470,23 @@ def main():
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="2d-2048: time the headline workload only")
     ap.add_argument("--serial", action="store_true", help="2d-ensemble96: step the size groups one after another")
+    ap.add_argument("--copy-only", type=int, default=None, metavar="MIB",
+                    help="only run the library's device copy (bastr_prof_copy_gbps) of MIB MiB, 5 times: the "
+                         "known-size kernel the HBM counter summaries are calibrated against")
     args = ap.parse_args()
+    if args.copy_only is not None:
+        import bastr_amd
+        from bastr_amd.solver import make_params
+        lib = bastr_amd.load()
+        prof_api(lib)
+        s = bastr_amd.solver(make_params(16, 16))
+        g = C.c_double()
+        rc = lib.bastr_prof_copy_gbps(s._h, args.copy_only << 20, 5, C.byref(g))
+        s.close()
+        print(f"copy of {args.copy_only} MiB x 5: {g.value:.1f} GB/s (rc {rc})")
+        sys.exit(0 if rc == 0 else 1)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
               "MI355X's HBM); --gpus must be 1", file=sys.stderr)
@@ -359,95 +494,66 @@ def main():
     if args.config == "2d-ensemble96":
         return bench_ensemble(args)
 
-    import bastr_amd
-    from bastr_amd.solver import make_params
+    import numpy as np
 
-    cfg = CONFIGS[args.config]
-    grid, prec, force, proj, kmax = cfg["grid"], cfg["prec"], cfg["force"], cfg["proj"], cfg["kmax"]
+    import bastr_amd
+
+    name = args.config
+    cfg = CONFIGS[name]
+    grid, prec = cfg["grid"], cfg["prec"]
     steps = args.steps if args.steps is not None else cfg["steps"]
     cpu_steps = args.cpu_steps if args.cpu_steps is not None else cfg["cpu"]
-    ia, ja = grid[0], grid[1]
-    ka = grid[2] if len(grid) == 3 else 0
     dims = "x".join(map(str, grid))
-
-    def params(precision):
-        return make_params(ia, ja, ka, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=force, forcek=5,
-                           target_energy=0.075, lproject=proj, kmax=kmax, precision=precision)
-
     lib = bastr_amd.load()
     prof_api(lib)
-    s = bastr_amd.solver(params(prec))
-    s.init_random_phase(k0=4.0, seed=1, energy=1.0)
 
-    # warmup, with every kernel class timed to find the dominant one
-    lib.bastr_prof_enable(s._h, (1 << NCLASS) - 1)
-    run_steps(s, max(args.warmup, 1), args.chunk)
-    warm = read_prof(lib, s._h)
-    dom_name, (dom_cls, _, _, _) = max(warm.items(), key=lambda kv: kv[1][2])
-    for name, (c, n, ms, by) in sorted(warm.items(), key=lambda kv: -kv[1][2]):
-        if n:
-            print(f"  warmup {name:12s} launches={n:6d} mean={ms / n * 1e3:9.2f} us "
-                  f"bytes/launch={by / 1e6:8.2f} MB  {by / (ms / n * 1e-3) / 1e9:8.1f} GB/s",
-                  file=sys.stderr, flush=True)
-
-    # timed region: only the dominant kernel class is bracketed by events
-    lib.bastr_prof_enable(s._h, 1 << dom_cls)
-    t0 = time.perf_counter()
-    run_steps(s, steps, args.chunk)
-    wall = time.perf_counter() - t0
-    print(f"  timed {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
-    prof = read_prof(lib, s._h)
-    _, n_l, ms_l, by_l = prof[dom_name]
-    mean_s = ms_l / n_l * 1e-3
-    achieved = by_l / mean_s / 1e9
-
-    # measured device copy bandwidth (1 GiB, read + write), beside the nominal peak
-    copy_gbps = C.c_double()
-    if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(copy_gbps)) != 0:
-        copy_gbps.value = float("nan")
-    s.close()
-
-    npts = ia * ja * max(ka, 1)
-    value = npts * steps / wall
-    # the whole step against the reference algorithm's transform count, each
-    # transform one compulsory real round trip of 2 * s * N^d bytes
-    T = reference_transforms(len(grid), 1, force)
-    b_step = T * 2 * (4 if prec == 1 else 8) * npts
-    step_rd = {"reference_transforms": T, "bytes_per_step": b_step,
-               "achieved": round(b_step * steps / wall / 1e9, 1), "unit": "GB/s",
-               "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4),
-               "frac_of_copy": round(b_step * steps / wall / 1e9 / copy_gbps.value, 4)}
-    rd = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom_name, prec, grid),
-          "kernel": dom_name, "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
-          "copy_measured": round(copy_gbps.value, 1), "frac_of_copy": round(achieved / copy_gbps.value, 4)}
-    # accuracy of E(k): an FP32 run against the FP64 device path over the same
-    # iterations (the FP64 path itself is checked against the oracle below)
+    rec, last = measure(lib, bastr_amd, name, steps, args.warmup, args.chunk, copy=True, graph=True)
+    finals = {name: last}
     accuracy = {}
-    if prec == 1:
-        nchk = min(steps, 200 if ka == 0 else 20)
+    legs = {}
+    if name == "2d-2048" and not args.no_legs:
+        # the rest of the metric: the FP32 variant and the 512^3 workloads in
+        # both precisions, each from the same initial field as its FP64 twin
+        # and as many iterations, so their last spectra compare directly
+        for leg, twin in LEGS:
+            nd3 = len(CONFIGS[leg]["grid"]) == 3
+            lsteps = min(steps, CONFIGS[leg]["steps"]) if nd3 else steps
+            lwarm = min(args.warmup, 10) if nd3 else args.warmup
+            lrec, llast = measure(lib, bastr_amd, leg, lsteps, lwarm, args.chunk)
+            finals[leg] = llast
+            lrec["config"] = {"workload": workload_desc(leg), "grid": list(CONFIGS[leg]["grid"])}
+            if twin is not None and twin in finals and (twin != name or (lsteps, lwarm) == (steps, args.warmup)):
+                e_abs, e_shell = ek_err(llast, finals[twin])
+                lrec["accuracy_vs_fp64_device"] = {"iterations": lwarm + lsteps, "ek_rel_l2": e_abs,
+                                                   "ek_err_max_per_shell": e_shell}
+            legs[leg] = lrec
+        ratios = {}
+        for a32, a64 in (("2d-2048-fp32", None), ("3d-512-fp32", "3d-512")):
+            v64 = rec["value"] if a64 is None else legs[a64]["value"]
+            ratios["2048^2" if a64 is None else "512^3"] = round(legs[a32]["value"] / v64, 3)
+        accuracy["fp32_vs_fp64_device"] = {k: v.get("accuracy_vs_fp64_device") for k, v in legs.items()
+                                           if "accuracy_vs_fp64_device" in v}
+    elif prec == 1:
+        # an FP32 workload alone: its E(k) against the FP64 device path, same iterations
+        nchk = min(steps, 200 if len(grid) == 2 else 20)
         res = []
         for pr in (1, 0):
-            g = bastr_amd.solver(params(pr))
-            g.init_random_phase(k0=4.0, seed=1, energy=1.0)
+            g = new_field(bastr_amd, cfg_params(name, pr))
             _, _, sp = g.step(nchk)
             res.append((sp[-1], g.get_fields()))
             g.close()
         e_abs, e_shell = ek_err(res[0][0], res[1][0])
-        accuracy["fp32_vs_fp64_device"] = {"iterations": nchk, "ek_rel_l2": e_abs,
-                                           "ek_err_max_per_shell": e_shell,
+        accuracy["fp32_vs_fp64_device"] = {"iterations": nchk, "ek_rel_l2": e_abs, "ek_err_max_per_shell": e_shell,
                                            "fields_rel_l2": rel_l2(res[0][1], res[1][1])}
-        print(f"  fp32 vs fp64: {accuracy['fp32_vs_fp64_device']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         # the same initial field, from the library's FP64 generator, on the CPU restatement
-        g0 = bastr_amd.solver(params(0))
-        g0.init_random_phase(k0=4.0, seed=1, energy=1.0)
+        g0 = new_field(bastr_amd, cfg_params(name, 0))
         fields = g0.get_fields()
         _, _, sp0 = g0.step(cpu_steps)
         dev = (sp0[-1], g0.get_fields())
         g0.close()
-        v, thr, dt, osp, ofields = cpu_baseline(params(0), fields, cpu_steps)
+        v, thr, dt, osp, ofields = cpu_baseline(cfg_params(name, 0), fields, cpu_steps)
         model, ncpu = cpu_model()
         cpu = {"value": v, "unit": "grid-pt·steps/s", "cores": thr, "kind": "port",
                "cpu_model": model, "host_logical_cpus": ncpu,
@@ -458,26 +564,28 @@ def main():
                                              "ek_err_max_per_shell": e_shell, "fields_rel_l2": rel_l2(dev[1], ofields)}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": rec["value"],
         "unit": "grid-pt·steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / steps * 1e3,
+        "ms_per_step": rec["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == 1 else "f64",
-        "data": "synthetic: seeded random-phase field (hitgen law, k0=4, seed 1), E(k)~k^4 exp(-2(k/4)^2), u_rms=1",
-        "config": {"workload": f"{len(grid)}-D viscous Burgers {dims}, Re=5000, dt=1e-4, RK3, "
-                               f"{'band forcing k=5 E_t=0.075' if force == 2 else 'free decay'}, "
-                               f"lproject={'t' if proj else 'f'}, spectra kmax={kmax}",
-                   "grid": list(grid), "precision": "fp32" if prec == 1 else "fp64"},
-        "roofline": rd,
+        "dtype": rec["dtype"],
+        "data": "synthetic: seeded random-phase field, E(k) = k^4 exp(-2 (k/4)^2) (pure law, k0=4), seed 1, "
+                "u_rms=1, built on the device",
+        "config": {"workload": workload_desc(name), "grid": list(grid), "precision": "fp32" if prec == 1 else "fp64"},
+        "roofline": rec["roofline"],
         "cpu_baseline": cpu,
-        "step_roofline": step_rd,
+        "step_roofline": rec["step_roofline"],
+        "graph_replay": rec.get("graph_replay"),
         "accuracy": accuracy,
     }
+    if legs:
+        out["legs"] = legs
+        out["fp32_over_fp64"] = ratios
     print(json.dumps(out))
 
 

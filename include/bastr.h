@@ -124,6 +124,18 @@ int bastr_set_fields(bastr_handle* h, const double* u1, const double* u2, const 
  */
 int bastr_init_random_phase(bastr_handle* h, double k0, uint64_t seed, double energy);
 
+/*
+ * Spectrum law of the handle's later bastr_init_random_phase(_member) calls:
+ *   BASTR_LAW_HITGEN (the default): the piecewise law of Bastrpp's hitgen
+ *     above (Bastrpp.F90:654-658);
+ *   BASTR_LAW_PURE: E(k) = k^4 exp(-2 (k/k0)^2) at every k (no k^-2 branch
+ *     below k0), the law the benchmark workloads state.
+ * Everything else in the construction is unchanged.
+ */
+#define BASTR_LAW_HITGEN    0
+#define BASTR_LAW_PURE      1
+int bastr_set_spectrum_law(bastr_handle* h, int32_t law);
+
 /* Physical velocity field (Fortran order) at the current nstep. */
 int bastr_get_fields(bastr_handle* h, double* u1, double* u2, double* u3);
 

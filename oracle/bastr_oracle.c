@@ -212,6 +212,7 @@ struct bo_state {
     cplx* force[3];
     double* Es; double* Ed; double* kn; int64_t* Ecount;
     double Esspe, Edspe;
+    int law;                         /* spectrum law of bo_init_random_phase */
     int64_t nstep;
     double time;
     int prologue_pending;
@@ -849,6 +850,12 @@ static uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+int bo_set_spectrum_law(bo_state* s, int32_t law) {
+    if (law != BASTR_LAW_HITGEN && law != BASTR_LAW_PURE) return fail(BASTR_EINVAL, "bo_set_spectrum_law: bad law");
+    s->law = law;
+    return BASTR_OK;
+}
+
 /* create_initial_field_2d / _3d (Bastrpp.F90:601-696, 698-808), statement by
  * statement, with k0 in the role of forcek and `energy` in the role of
  * target_energy.  Loads the field like bo_set_fields. */
@@ -870,7 +877,7 @@ int bo_init_random_phase(bo_state* s, double k0, uint64_t seed, double energy) {
         }
         const double q = kk / k0, kk2 = kk * kk;
         double var1;
-        if (kk >= k0) var1 = kk2 * kk2 * exp(-2.0 * (q * q));
+        if (kk >= k0 || s->law == BASTR_LAW_PURE) var1 = kk2 * kk2 * exp(-2.0 * (q * q));
         else var1 = k0 * k0 * k0 * k0 * k0 * k0 / kk2 * exp(-2.0);
         const double var2 = s->nd == 2 ? sqrt(var1 / 2.0 / PI / kk) : sqrt(var1 / 4.0 / PI / (kk * kk));
         const double th = atan2(cimag(rc[m]), creal(rc[m]));

@@ -374,6 +374,23 @@ def test_device_random_phase_matches_oracle(hip, orc, shape, k0, seed, energy):
     assert rel_l2(g.get_fields(), o.get_fields()) < 1e-11
 
 
+@pytest.mark.parametrize("shape,prec", [((256, 256), 0), ((128, 64), 0), ((128, 128), 1)])
+def test_device_random_phase_pure_law(hip, orc, shape, prec):
+    """The pure law k^4 exp(-2 (k/k0)^2) (bastr_set_spectrum_law) on the device
+    equals the oracle's, and differs from the hitgen law below k0."""
+    print(f"case init_rp pure law {shape} prec={prec}", flush=True)
+    g = Solver(hip, "bastr", make_params(*shape, precision=prec))
+    g.set_spectrum_law(abi.LAW_PURE)
+    g.init_random_phase(k0=4.0, seed=9, energy=1.0)
+    o = orc.solver(make_params(*shape))
+    o.set_spectrum_law(abi.LAW_PURE)
+    o.init_random_phase(k0=4.0, seed=9, energy=1.0)
+    assert rel_l2(g.get_fields(), o.get_fields()) < (1e-12 if prec == 0 else 1e-5)
+    h = orc.solver(make_params(*shape))
+    h.init_random_phase(k0=4.0, seed=9, energy=1.0)
+    assert rel_l2(h.get_fields(), o.get_fields()) > 1e-3
+
+
 def test_device_random_phase_fp32(hip, orc):
     print("case init_rp fp32 128", flush=True)
     g = Solver(hip, "bastr", make_params(128, 128, precision=1))

@@ -387,22 +387,25 @@ def test_forcing_m5_against_numpy(orc, proj):
         assert np.abs(a - (b + factor * c)).max() < 1e-13
 
 
+@pytest.mark.parametrize("law", [abi.LAW_HITGEN, abi.LAW_PURE])
 @pytest.mark.parametrize("shape,k0,seed,energy", [((64, 48), 4.0, 1, 1.0), ((32, 32), 5.0, 7, 0.075),
                                                    ((16, 16, 16), 4.0, 3, 1.0), ((16, 32, 8), 3.0, 2, 0.5)])
-def test_oracle_random_phase_field(orc, shape, k0, seed, energy):
+def test_oracle_random_phase_field(orc, shape, k0, seed, energy, law):
     """bo_init_random_phase (create_initial_field_2d/3d, Bastrpp.F90:601-808)
     against properties its formula fixes, independently of both restatements:
     <sum u^2> = energy; the field is real and curl-free (u_hat parallel to k
     off the Nyquist lines); per-mode |u_hat|^2 is proportional to
-    E(k)/(2 pi k) (2-D) or E(k)/(4 pi k^2) (3-D) with the piecewise law; and
-    the numpy restatement (another FFT) gives the same field."""
+    E(k)/(2 pi k) (2-D) or E(k)/(4 pi k^2) (3-D) with the piecewise law of
+    hitgen or the pure law k^4 exp(-2 (k/k0)^2); and the numpy restatement
+    (another FFT) gives the same field."""
     nd = len(shape)
     p = make_params(shape[0], shape[1], shape[2] if nd == 3 else 0)
     o = orc.solver(p)
+    o.set_spectrum_law(law)
     o.init_random_phase(k0=k0, seed=seed, energy=energy)
     assert o.get_time() == (0, 0.0)
     # raw (un-dealiased) field: a cutoff beyond the grid keeps every mode
-    u = hitgen(shape, k0=k0, seed=seed, energy=energy)
+    u = hitgen(shape, k0=k0, seed=seed, energy=energy, law=law)
     assert abs(sum(np.mean(x * x) for x in u) - energy) < 1e-13 * energy
     uh = [np.fft.fftn(x) / x.size for x in u]
     ks = np.meshgrid(*[np.fft.fftfreq(n, 1.0 / n) for n in shape], indexing="ij")
@@ -417,9 +420,11 @@ def test_oracle_random_phase_field(orc, shape, k0, seed, energy):
     for i in range(nd):
         for j in range(i + 1, nd):
             assert np.abs((uh[i] * ks[j] - uh[j] * ks[i])[off]).max() < 1e-12 * scale * max(shape)
-    law = np.where(kk >= k0, kk ** 4 * np.exp(-2 * (kk / k0) ** 2), k0 ** 6 / np.where(kk > 0, kk, 1) ** 2 * np.exp(-2))
+    Ek = kk ** 4 * np.exp(-2 * (kk / k0) ** 2)
+    if law == abi.LAW_HITGEN:
+        Ek = np.where(kk >= k0, Ek, k0 ** 6 / np.where(kk > 0, kk, 1) ** 2 * np.exp(-2))
     safe = np.where(kk > 0, kk, 1.0)
-    dens = law / (2 * np.pi * safe) if nd == 2 else law / (4 * np.pi * safe ** 2)
+    dens = Ek / (2 * np.pi * safe) if nd == 2 else Ek / (4 * np.pi * safe ** 2)
     off &= dens > 1e-10 * dens[off].max()  # modes well above the FFT roundoff floor
     ratio = sum(np.abs(x) ** 2 for x in uh)[off] / dens[off]
     assert np.ptp(ratio) < 1e-9 * ratio.mean()
