@@ -96,11 +96,35 @@ NOALLOC_SIGNATURES = [
 ]
 
 
+# The learned subgrid closure (include/bastr_closure.h; HIP library only).
+_F = C.POINTER(C.c_float)
+CLOSURE_NPARAM = 6658
+CLOSURE_NSCALE = 6
+CLOSURE_SIGNATURES = [
+    ("filter_to", C.c_int, [_P, _P]),
+    ("pairs_create", C.c_int, [C.c_int32, C.c_int32, C.POINTER(_P)]),
+    ("pairs_add", C.c_int, [_P, _P]),
+    ("pairs_count", C.c_int, [_P]),
+    ("pairs_get", C.c_int, [_P, C.c_int32, _F, _F]),
+    ("pairs_destroy", C.c_int, [_P]),
+    ("closure_create", C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    ("closure_destroy", C.c_int, [_P]),
+    ("closure_set_params", C.c_int, [_P, _F, _F]),
+    ("closure_get_params", C.c_int, [_P, _F, _F]),
+    ("closure_fit_scales", C.c_int, [_P, _P]),
+    ("closure_loss_grad", C.c_int, [_P, _P, C.c_int32, C.c_int32, _D, _F]),
+    ("closure_train", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _D]),
+    ("closure_apply", C.c_int, [_P, _P, C.c_int32, _F]),
+    ("closure_attach", C.c_int, [_P, _P]),
+]
+
+
 def bind(lib, prefix, ensemble=False):
     """Attach argtypes/restype for every entry point; raise if one is missing.
     ensemble=True also binds the HIP library's own entry points (ensembles,
-    bastr_step_group, bastr_step_noalloc)."""
-    for name, res, args in SIGNATURES + (ENSEMBLE_SIGNATURES + NOALLOC_SIGNATURES if ensemble else []):
+    bastr_step_group, bastr_step_noalloc, the closure)."""
+    for name, res, args in SIGNATURES + (ENSEMBLE_SIGNATURES + NOALLOC_SIGNATURES + CLOSURE_SIGNATURES
+                                         if ensemble else []):
         fn = getattr(lib, f"{prefix}_{name}")  # AttributeError = missing export
         fn.restype = res
         fn.argtypes = args

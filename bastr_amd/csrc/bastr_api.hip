@@ -17,10 +17,13 @@
 #include <vector>
 
 #include "../../include/bastr.h"
+#include "../../include/bastr_closure.h"
 #include "../../include/bastr_prof.h"
+#include "closure.hpp"
 #include "k2d.hpp"
 #include "k3d.hpp"
 #include "init_rp.hpp"
+#include "mlp.hpp"
 #include "pp.hpp"
 
 using namespace k2d;
@@ -98,6 +101,25 @@ void* dalloc(size_t bytes, bool* ok) {
 
 }  // namespace
 
+// A subgrid-closure network: FP32 master parameters, their BF16 operand form,
+// Adam state and the scales of its inputs and outputs.
+struct bastr_closure {
+    hipStream_t stream = nullptr;
+    float* params = nullptr;   // [NPARAM] FP32 master copy
+    float* am = nullptr;       // Adam first and second moments
+    float* av = nullptr;
+    float* grad = nullptr;
+    mlp::Packed* packed = nullptr;
+    float scales[BASTR_CLOSURE_NSCALE] = {1, 1, 1, 1, 1, 1};
+    mlp::Scales sc{};
+    uint64_t version = 1;      // bumped when the scales change (captured graphs hold them)
+    int t = 0;                 // Adam steps taken
+    float* part = nullptr;     // per-wavefront partial gradients
+    float* part_loss = nullptr;
+    int part_w = 0;
+    double* dloss = nullptr;
+};
+
 struct bastr_handle {
     bastr_params p{};
     int nd = 2, n0 = 0, n1 = 0, prec = 0, allkmax = 0;
@@ -154,6 +176,17 @@ struct bastr_handle {
     int nmem = 1, lw = 1;
     int dev = 0;                      // device current at bastr_create
     int law = BASTR_LAW_HITGEN;       // spectrum law of the random-phase generator
+    // learned subgrid closure (2-D one-field RK3 handles): the attached network,
+    // the graph's view of its scales, and the buffers of its force: theta_hat and
+    // omega_hat (CW), the y-inverses of u1, u2, theta, omega (CY), those fields in
+    // physical space (CR), the force (CF) and its x-forward (CX)
+    bastr_closure* net = nullptr;
+    uint64_t net_version = 0;
+    void* CW[2] = {};
+    void* CY[4] = {};
+    void* CR[4] = {};
+    void* CF[2] = {};
+    void* CX[2] = {};
     int* mforce = nullptr;            // [nmem] device: member is band-forced (null: all are)
     std::vector<char> mset;           // member's field has been set
     std::vector<int64_t> mnstep;      // per-member nstep / time (ensembles)
@@ -491,6 +524,75 @@ int run_to_physical(bastr_handle* h) {
     return 0;
 }
 
+// The closure's coarse fields u1, u2, theta, omega in physical space (CR) from
+// the half spectra Uin (prune: the step's dealias-limited passes)
+template <typename T>
+int closure_fields(bastr_handle* h, const Gate& g, const void* const* Uin, bool prune) {
+    const clo::Half hf{h->n0, h->n1, h->nh, h->hp};
+    LAUNCHCHK(clo::launch_grad<T>(hf, g, C<T>((void*)Uin[0]), C<T>((void*)Uin[1]), C<T>(h->CW[0]), C<T>(h->CW[1]),
+                                  h->stream));
+    const bool pr = prune && h->prune;
+    k3d::ColXArgs<T> c{};
+    c.gate = g;
+    c.in[0] = C<T>((void*)Uin[0]);
+    c.in[1] = C<T>((void*)Uin[1]);
+    c.in[2] = C<T>(h->CW[0]);
+    c.in[3] = C<T>(h->CW[1]);
+    for (int q = 0; q < 4; ++q) {
+        c.out[q] = C<T>(h->CY[q]);
+        c.job[q] = job1(q, k3d::M_ONE, q);
+    }
+    c.njob = 4;
+    c.pitch = h->hp;
+    c.ncols = pr ? h->cols : h->hp;
+    c.kcut = pr ? h->cut : 0;
+    c.plane_stride = 0;
+    c.nplanes = 1;
+    c.scale = 1.0;
+    c.tw = C<T>(h->tw1);
+    c.mstride = (long)h->nhalf;
+    c.nmem = 1;
+    LAUNCHCHK(k3d::launch_colx<T>(h->n1, true, c, h->stream));
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.g.kxlim = pr ? h->cut : h->hp;
+    r.gate = g;
+    r.tw = C<T>(h->tw0);
+    for (int q = 0; q < 4; q += 2) {
+        r.hin[0] = C<T>(h->CY[q]);
+        r.hin[1] = C<T>(h->CY[q + 1]);
+        r.rout[0] = (T*)h->CR[q];
+        r.rout[1] = (T*)h->CR[q + 1];
+        LAUNCHCHK(launch_row<T>(h->n0, R_INV_PACK, r, h->n1, h->stream));
+    }
+    return 0;
+}
+
+// The attached network's force on the stage field Uin, added to the stage's
+// x-forwarded nonlinear term NX (before its y-forward, dealiasing and projection)
+template <typename T>
+int run_closure(bastr_handle* h, const Gate& g, const void* const* Uin) {
+    int rc = closure_fields<T>(h, g, Uin, true);
+    if (rc) return rc;
+    mlp::Fields<T> F{};
+    for (int q = 0; q < 4; ++q) F.f[q] = (const T*)h->CR[q];
+    F.n = h->n0;
+    F.shift = __builtin_ctz((unsigned)h->n0);
+    LAUNCHCHK(mlp::launch_forward<T>(h->net->packed, h->net->sc, F, (T*)h->CF[0], (T*)h->CF[1], h->stream));
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.gate = g;
+    r.rin[0] = (const T*)h->CF[0];
+    r.rin[1] = (const T*)h->CF[1];
+    r.hout[0] = C<T>(h->CX[0]);
+    r.hout[1] = C<T>(h->CX[1]);
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(launch_row<T>(h->n0, R_FWD_PACK, r, h->n1, h->stream));
+    LAUNCHCHK(clo::launch_add<T>(g, C<T>(h->NX[0]), C<T>(h->NX[1]), C<T>(h->CX[0]), C<T>(h->CX[1]), h->nhalf,
+                                 h->stream));
+    return 0;
+}
+
 // One RK3 stage (stage 0..2) or CN sweep (stage 3 = middle sweep, 4 = last
 // sweep; cn_first marks sweep 1): y-inverse (derivatives), x pass (nonlinear
 // term), y-forward + update.  The first stage/sweep's y-inverse and x pass ran
@@ -514,6 +616,11 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     if (!first) {  // the first x pass ran fused with velgrad in the diagnostic phase
         ProfScope ps(h, BASTR_K_ROW_STAGE);
         LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
+    }
+    if (h->net) {  // subgrid force of the stage field (RK3 only: Uin is the stage input)
+        ProfScope ps(h, BASTR_K_CLOSURE);
+        const int rc = run_closure<T>(h, gstep, Uin);
+        if (rc) return rc;
     }
     {
         static const int modes[5] = {UP_A, UP_B, UP_C, UP_CN_MID, UP_CN_LAST};
@@ -1112,6 +1219,10 @@ int step_launch(bastr_handle* h, int nsteps) {
     // fixed per handle: buffers, the iteration index within the call, the
     // device-side NaN gate).  Profiled calls launch directly: their timing
     // events stay out of the capture.
+    if (h->net && h->net_version != h->net->version) {
+        drop_graph(h);  // the captured closure launches hold the network's scales
+        h->net_version = h->net->version;
+    }
     if (h->use_graph && h->prof_mask == 0 && nsteps <= kMaxGraphSteps) {
         if (!h->gexec || h->gsteps != nsteps) {
             SyncLock lk(g_sync_mu);
@@ -1492,6 +1603,9 @@ int bastr_destroy(bastr_handle* h) {
     void* bufs[] = {h->dstage, h->tw0, h->tw1, h->tw0d, h->tw1d, h->cbuf, h->part_bins, h->part_tot,
                     h->part_vg, h->red, h->part_en, h->dfactor, h->status, h->ddiag, h->dspec, h->mforce};
     for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (void* b : {h->CW[0], h->CW[1], h->CY[0], h->CY[1], h->CY[2], h->CY[3], h->CR[0], h->CR[1], h->CR[2],
+                    h->CR[3], h->CF[0], h->CF[1], h->CX[0], h->CX[1]})
         if (b) (void)hipFree(b);
     for (int c = 0; c < 3; ++c) {
         if (h->U[c]) (void)hipFree(h->U[c]);
@@ -2009,6 +2123,9 @@ int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* tot
         units[BASTR_K_COL_FWD_A] = 2 * fc + 6 * fc * fy; // y-forward of 2, update r U w KA UB
         units[BASTR_K_COL_FWD_B] = 2 * fc + 8 * fc * fy; // r U KA w UB KA
         units[BASTR_K_COL_FWD_C] = 2 * fc + 8 * fc * fy; // r U KA UB w U
+        // closure force of a stage: theta/omega spectra r2 w2, y-inverse r4 w4,
+        // x-inverse r4 w4 (real), network r4 w2 (real), x-forward r2 w2, add r4 w2
+        units[BASTR_K_CLOSURE] = 36;
     } else {
         const double fyz = fy * fz;
         units[BASTR_K_SPECTRA] = 3 * fc * fyz;
@@ -2066,7 +2183,7 @@ int bastr_prof_copy_gbps(bastr_handle* h, uint64_t bytes, int reps, double* gbps
 const char* bastr_prof_name(int c) {
     static const char* names[BASTR_K_NCLASS] = {"spectra", "col_inv_vg", "velgrad", "finalize", "col_inv",
                                                 "row_stage", "col_fwd_a", "col_fwd_b", "col_fwd_c", "force",
-                                                "vg_dis"};
+                                                "vg_dis", "closure"};
     return (c >= 0 && c < BASTR_K_NCLASS) ? names[c] : "";
 }
 int bastr_fft_backward(bastr_handle* h, double* inout) { return fft_c2c(h, inout, true); }
@@ -2330,6 +2447,482 @@ int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const
     if (rc) return rc;
     const double* src[3] = {u1, u2, u3};
     return h->prec == BASTR_PREC_F32 ? pp_transfer_impl<float>(h, src, out) : pp_transfer_impl<double>(h, src, out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned subgrid closure (bastr_closure.h): the sharp filter between grids,
+// training pairs, the network's host side and its attachment to a handle
+// ---------------------------------------------------------------------------
+struct bastr_pairs {
+    int n = 0, shift = 0, cap = 0, count = 0;
+    float* fld = nullptr;          // [cap][4][n*n] u1, u2, theta, omega
+    float* tgt = nullptr;          // [cap][2][n*n] tau1, tau2
+    bastr_handle* coarse = nullptr;  // n x n, the fine handles' precision (made by the first add)
+    void* FA[2] = {};              // the filtered fine nonlinear term, coarse half spectra
+};
+
+namespace {
+
+clo::Half half_of(const bastr_handle* h) { return clo::Half{h->n0, h->n1, h->nh, h->hp}; }
+
+// the closure buffers of a 2-D one-field handle (allocated once)
+int closure_buffers(bastr_handle* h) {
+    if (h->CR[0]) return 0;
+    bool ok = true;
+    const size_t cbytes = h->nhalf * 2 * h->esz, rbytes = h->npts * h->esz;
+    for (int q = 0; q < 2; ++q) {
+        h->CW[q] = dalloc(cbytes, &ok);
+        h->CF[q] = dalloc(rbytes, &ok);
+        h->CX[q] = dalloc(cbytes, &ok);
+    }
+    for (int q = 0; q < 4; ++q) {
+        h->CY[q] = dalloc(cbytes, &ok);
+        h->CR[q] = dalloc(rbytes, &ok);
+    }
+    return ok ? 0 : fail(BASTR_ENOMEM, "closure buffers: device allocation failed");
+}
+
+bool square_pow2(const bastr_handle* h, int lo, int hi) {
+    return h->nd == 2 && h->n0 == h->n1 && pow2_in(h->n0, lo, hi);
+}
+
+int check_fine_coarse(const bastr_handle* f, const bastr_handle* c, const char* what) {
+    if (!f || !c) return fail(BASTR_EINVAL, "%s: NULL handle", what);
+    if (f->nmem != 1 || c->nmem != 1) return fail(BASTR_EINVAL, "%s: one-field handles only", what);
+    if (!square_pow2(f, 32, 4096) || !square_pow2(c, 32, 4096))
+        return fail(BASTR_EINVAL, "%s: square 2-D grids, powers of two in [32, 4096]", what);
+    if (c->n0 > f->n0) return fail(BASTR_EINVAL, "%s: the coarse grid (%d) is finer than the fine one (%d)", what, c->n0, f->n0);
+    if (f->prec != c->prec) return fail(BASTR_EINVAL, "%s: the two handles differ in precision", what);
+    if (!f->have_fields) return fail(BASTR_EINVAL, "%s: fine fields not set", what);
+    return 0;
+}
+
+// fine's normalised spectrum (its pending entry dealias first) and its open gate
+template <typename T>
+int fine_spectrum(bastr_handle* f) {
+    if (f->prologue_pending) {
+        const int rc = run_prologue<T>(f);
+        if (rc) return rc;
+        f->prologue_pending = false;
+    }
+    return reset_status(f) ? fail(BASTR_EDEVICE, "status reset launch failed") : 0;
+}
+
+// nonlinear term N(u) = -u_j d_j u_i of a handle's state, normalised half
+// spectra in W[0..1] (the step's stage pass without the update; modes inside
+// the dealias limits)
+template <typename T>
+int nonlinear_term(bastr_handle* h) {
+    const Gate open{h->status, 0, 0};
+    const void* Us[2] = {h->U[0], h->U[1]};
+    int rc = run_yinv2<T>(h, open, Us, 1, true);
+    if (rc) return rc;
+    RowArgs<T> r{};
+    r.g = geo<T>(h);
+    r.g.kxlim = step_kxlim(h);
+    r.gate = open;
+    for (int q = 0; q < 4; ++q) r.hin[q] = C<T>(h->W[q]);
+    r.hout[0] = C<T>(h->NX[0]);
+    r.hout[1] = C<T>(h->NX[1]);
+    r.tw = C<T>(h->tw0);
+    LAUNCHCHK(launch_row<T>(h->n0, R_STAGE, r, h->n1, h->stream));
+    void* nx[2] = {h->NX[0], h->NX[1]};
+    void* w[2] = {h->W[0], h->W[1]};
+    return run_yfwd2<T>(h, open, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
+}
+
+// coarse state := sharp filter of fine's (plus up to two more fine half
+// spectra filtered into extra[] of coarse size)
+template <typename T>
+int filter_impl(bastr_handle* f, bastr_handle* c, const void* const* more, void* const* extra, int nmore) {
+    const cx<T>* src[4] = {C<T>(f->U[0]), C<T>(f->U[1]), nullptr, nullptr};
+    cx<T>* dst[4] = {C<T>(c->U[0]), C<T>(c->U[1]), nullptr, nullptr};
+    for (int q = 0; q < nmore; ++q) {
+        src[2 + q] = C<T>((void*)more[q]);
+        dst[2 + q] = C<T>(extra[q]);
+    }
+    LAUNCHCHK(clo::launch_filter<T>(half_of(f), half_of(c), c->cut, src, dst, 2 + nmore, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    c->prologue_pending = false;
+    c->have_fields = true;
+    c->nstep = f->nstep;
+    c->time = f->time;
+    return 0;
+}
+
+template <typename T>
+int pairs_add_impl(bastr_pairs* ps, bastr_handle* f) {
+    bastr_handle* c = ps->coarse;
+    int rc = fine_spectrum<T>(f);
+    if (!rc) rc = nonlinear_term<T>(f);
+    const void* more[2] = {f->W[0], f->W[1]};
+    if (!rc) rc = filter_impl<T>(f, c, more, ps->FA, 2);
+    if (rc) return rc;
+    if (reset_status(c)) return fail(BASTR_EDEVICE, "status reset launch failed");
+    const Gate open{c->status, 0, 0};
+    // D_c N(u_bar) in c->W[0..1]; tau = P_c N(u) - D_c N(u_bar) in KA, then physical (ureal)
+    if ((rc = nonlinear_term<T>(c))) return rc;
+    LAUNCHCHK(clo::launch_tau<T>(half_of(c), c->cut, C<T>(ps->FA[0]), C<T>(ps->FA[1]), C<T>(c->W[0]), C<T>(c->W[1]),
+                                 C<T>(c->KA[0]), C<T>(c->KA[1]), c->stream));
+    const void* K[2] = {c->KA[0], c->KA[1]};
+    if ((rc = run_yinv2<T>(c, open, K, 0, false))) return rc;
+    RowArgs<T> r{};
+    r.g = geo<T>(c);
+    r.gate = open;
+    r.hin[0] = C<T>(c->W[0]);
+    r.hin[1] = C<T>(c->W[1]);
+    r.rout[0] = (T*)c->ureal[0];
+    r.rout[1] = (T*)c->ureal[1];
+    r.tw = C<T>(c->tw0);
+    LAUNCHCHK(launch_row<T>(c->n0, R_INV_PACK, r, c->n1, c->stream));
+    // the network's input fields of u_bar
+    const void* Uc[2] = {c->U[0], c->U[1]};
+    if ((rc = closure_fields<T>(c, open, Uc, false))) return rc;
+    const size_t n2 = c->npts;
+    for (int q = 0; q < 4; ++q)
+        LAUNCHCHK(clo::launch_to_float<T>((const T*)c->CR[q], ps->fld + ((size_t)ps->count * 4 + q) * n2, n2, c->stream));
+    for (int q = 0; q < 2; ++q)
+        LAUNCHCHK(clo::launch_to_float<T>((const T*)c->ureal[q], ps->tgt + ((size_t)ps->count * 2 + q) * n2, n2,
+                                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ps->count += 1;
+    return 0;
+}
+
+mlp::Data pairs_data(const bastr_pairs* ps) {
+    mlp::Data d{};
+    d.fld = ps->fld;
+    d.tgt = ps->tgt;
+    d.n = ps->n;
+    d.shift = ps->shift;
+    d.nsnap = ps->count;
+    return d;
+}
+
+void set_scales(bastr_closure* net, const float* s) {
+    for (int q = 0; q < BASTR_CLOSURE_NSCALE; ++q) net->scales[q] = s[q];
+    for (int q = 0; q < mlp::NFLD; ++q) net->sc.inv_in[q] = 1.0f / s[q];
+    for (int q = 0; q < mlp::NO; ++q) net->sc.out[q] = s[mlp::NFLD + q];
+    net->version += 1;
+}
+
+int net_part(bastr_closure* net, int nw) {
+    if (nw <= net->part_w) return 0;
+    SyncLock lk(g_sync_mu);
+    if (net->part) (void)hipFree(net->part);
+    if (net->part_loss) (void)hipFree(net->part_loss);
+    net->part = nullptr;
+    net->part_loss = nullptr;
+    net->part_w = 0;
+    HIPCHK(hipMalloc(&net->part, sizeof(float) * (size_t)nw * mlp::NPARAM));
+    HIPCHK(hipMalloc(&net->part_loss, sizeof(float) * (size_t)nw));
+    net->part_w = nw;
+    return 0;
+}
+
+// forward + backward over the selected tiles: gradient in net->grad, loss in *dloss
+int loss_grad_launch(bastr_closure* net, const bastr_pairs* ps, const mlp::TileSel& sel, double npts, double* dloss) {
+    const int nw = mlp::train_waves(sel.ntiles);
+    int rc = net_part(net, nw);
+    if (rc) return rc;
+    LAUNCHCHK(mlp::launch_train(net->packed, net->sc, pairs_data(ps), sel, npts, net->part, net->part_loss, net->stream));
+    LAUNCHCHK(mlp::launch_reduce(net->part, net->part_loss, nw, net->grad, dloss, net->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bastr_filter_to(bastr_handle* fine, bastr_handle* coarse) {
+    int rc = check_fine_coarse(fine, coarse, "bastr_filter_to");
+    if (rc) return rc;
+    if (fine == coarse) return fail(BASTR_EINVAL, "bastr_filter_to: the same handle twice");
+    rc = fine->prec == BASTR_PREC_F32 ? fine_spectrum<float>(fine) : fine_spectrum<double>(fine);
+    if (rc) return rc;
+    return fine->prec == BASTR_PREC_F32 ? filter_impl<float>(fine, coarse, nullptr, nullptr, 0)
+                                        : filter_impl<double>(fine, coarse, nullptr, nullptr, 0);
+}
+
+int bastr_pairs_create(int32_t n, int32_t capacity, bastr_pairs** out) {
+    if (!out) return fail(BASTR_EINVAL, "bastr_pairs_create: NULL argument");
+    *out = nullptr;
+    if (!pow2_in(n, 32, 4096)) return fail(BASTR_EINVAL, "bastr_pairs_create: n = %d is not a power of two in [32, 4096]", n);
+    if (capacity < 1) return fail(BASTR_EINVAL, "bastr_pairs_create: capacity must be >= 1");
+    bastr_pairs* ps = new bastr_pairs();
+    ps->n = n;
+    ps->shift = __builtin_ctz((unsigned)n);
+    ps->cap = capacity;
+    SyncLock lk(g_sync_mu);
+    const size_t n2 = (size_t)n * n;
+    if (hipMalloc(&ps->fld, sizeof(float) * n2 * 4 * (size_t)capacity) != hipSuccess ||
+        hipMalloc(&ps->tgt, sizeof(float) * n2 * 2 * (size_t)capacity) != hipSuccess) {
+        bastr_pairs_destroy(ps);
+        return fail(BASTR_ENOMEM, "bastr_pairs_create: device allocation failed");
+    }
+    *out = ps;
+    return BASTR_OK;
+}
+
+int bastr_pairs_add(bastr_pairs* ps, bastr_handle* fine) {
+    if (!ps || !fine) return fail(BASTR_EINVAL, "bastr_pairs_add: NULL argument");
+    if (ps->count >= ps->cap) return fail(BASTR_EINVAL, "bastr_pairs_add: all %d snapshots used", ps->cap);
+    if (!ps->coarse) {
+        bastr_params p{};
+        p.ia = p.ja = ps->n;
+        p.ka = 0;
+        p.reynolds = fine->p.reynolds;
+        p.deltat = fine->p.deltat;
+        p.timemethod = 1;
+        p.forcemethod = 0;
+        p.forcek = 1;
+        p.target_energy = 0;
+        p.lproject = 0;
+        p.kmax = 1;
+        p.precision = fine->prec;
+        int rc = bastr_create(&p, &ps->coarse);
+        if (rc) return rc;
+        bool ok = true;
+        const size_t cbytes = ps->coarse->nhalf * 2 * ps->coarse->esz;
+        ps->FA[0] = dalloc(cbytes, &ok);
+        ps->FA[1] = dalloc(cbytes, &ok);
+        if (!ok || closure_buffers(ps->coarse)) return fail(BASTR_ENOMEM, "bastr_pairs_add: device allocation failed");
+    }
+    int rc = check_fine_coarse(fine, ps->coarse, "bastr_pairs_add");
+    if (rc) return rc;
+    return fine->prec == BASTR_PREC_F32 ? pairs_add_impl<float>(ps, fine) : pairs_add_impl<double>(ps, fine);
+}
+
+int bastr_pairs_count(const bastr_pairs* ps) { return ps ? ps->count : -1; }
+
+int bastr_pairs_get(const bastr_pairs* ps, int32_t i, float* fields, float* target) {
+    if (!ps) return fail(BASTR_EINVAL, "bastr_pairs_get: NULL handle");
+    if (i < 0 || i >= ps->count) return fail(BASTR_EINVAL, "bastr_pairs_get: snapshot %d not in [0, %d)", i, ps->count);
+    SyncLock lk(g_sync_mu);
+    const size_t n2 = (size_t)ps->n * ps->n;
+    if (fields) HIPCHK(hipMemcpy(fields, ps->fld + (size_t)i * 4 * n2, sizeof(float) * 4 * n2, hipMemcpyDeviceToHost));
+    if (target) HIPCHK(hipMemcpy(target, ps->tgt + (size_t)i * 2 * n2, sizeof(float) * 2 * n2, hipMemcpyDeviceToHost));
+    return BASTR_OK;
+}
+
+int bastr_pairs_destroy(bastr_pairs* ps) {
+    if (!ps) return BASTR_OK;
+    {
+        SyncLock lk(g_sync_mu);
+        if (ps->fld) (void)hipFree(ps->fld);
+        if (ps->tgt) (void)hipFree(ps->tgt);
+        for (void* p : ps->FA)
+            if (p) (void)hipFree(p);
+    }
+    if (ps->coarse) bastr_destroy(ps->coarse);
+    delete ps;
+    return BASTR_OK;
+}
+
+int bastr_closure_create(uint64_t seed, bastr_closure** out) {
+    if (!out) return fail(BASTR_EINVAL, "bastr_closure_create: NULL argument");
+    *out = nullptr;
+    // Glorot-uniform weights from a counter-based draw of seed, zero biases
+    std::vector<float> p(mlp::NPARAM, 0.0f);
+    auto draw = [seed](uint64_t i) {
+        uint64_t x = seed * 0x100000001B3ull + i + 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x = x ^ (x >> 31);
+        return (double)(x >> 11) * 0x1.0p-53;  // [0, 1)
+    };
+    auto fill = [&](int off, int fan_out, int fan_in) {
+        const double a = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int i = 0; i < fan_out * fan_in; ++i) p[off + i] = (float)((2.0 * draw(off + i) - 1.0) * a);
+    };
+    fill(mlp::OFF_W1, mlp::NH, mlp::NF);
+    fill(mlp::OFF_W2, mlp::NH, mlp::NH);
+    fill(mlp::OFF_W3, mlp::NO, mlp::NH);
+    bastr_closure* net = new bastr_closure();
+    {
+        SyncLock lk(g_sync_mu);
+        bool ok = hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipMalloc(&net->params, sizeof(float) * mlp::NPARAM) == hipSuccess;
+        ok = ok && hipMalloc(&net->am, sizeof(float) * mlp::NPARAM) == hipSuccess;
+        ok = ok && hipMalloc(&net->av, sizeof(float) * mlp::NPARAM) == hipSuccess;
+        ok = ok && hipMalloc(&net->grad, sizeof(float) * mlp::NPARAM) == hipSuccess;
+        ok = ok && hipMalloc(&net->packed, sizeof(mlp::Packed)) == hipSuccess;
+        ok = ok && hipMalloc(&net->dloss, sizeof(double) * 4096) == hipSuccess;
+        if (!ok) {
+            bastr_closure_destroy(net);
+            return fail(BASTR_ENOMEM, "bastr_closure_create: device allocation failed");
+        }
+    }
+    const float ones[BASTR_CLOSURE_NSCALE] = {1, 1, 1, 1, 1, 1};
+    const int rc = bastr_closure_set_params(net, p.data(), ones);
+    if (rc) {
+        bastr_closure_destroy(net);
+        return rc;
+    }
+    *out = net;
+    return BASTR_OK;
+}
+
+int bastr_closure_destroy(bastr_closure* net) {
+    if (!net) return BASTR_OK;
+    SyncLock lk(g_sync_mu);
+    if (net->stream) (void)hipStreamSynchronize(net->stream);
+    for (void* q : {(void*)net->params, (void*)net->am, (void*)net->av, (void*)net->grad, (void*)net->packed,
+                    (void*)net->part, (void*)net->part_loss, (void*)net->dloss})
+        if (q) (void)hipFree(q);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
+    delete net;
+    return BASTR_OK;
+}
+
+int bastr_closure_set_params(bastr_closure* net, const float* params, const float* scales) {
+    if (!net || !params || !scales) return fail(BASTR_EINVAL, "bastr_closure_set_params: NULL argument");
+    for (int q = 0; q < BASTR_CLOSURE_NSCALE; ++q)
+        if (!(scales[q] > 0.0f) || !std::isfinite(scales[q]))
+            return fail(BASTR_EINVAL, "bastr_closure_set_params: scale %d = %g is not a positive number", q, (double)scales[q]);
+    SyncLock lk(g_sync_mu);
+    HIPCHK(hipMemcpy(net->params, params, sizeof(float) * mlp::NPARAM, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(net->am, 0, sizeof(float) * mlp::NPARAM));
+    HIPCHK(hipMemset(net->av, 0, sizeof(float) * mlp::NPARAM));
+    net->t = 0;
+    set_scales(net, scales);
+    LAUNCHCHK(mlp::launch_pack(net->params, net->packed, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    return BASTR_OK;
+}
+
+int bastr_closure_get_params(const bastr_closure* net, float* params, float* scales) {
+    if (!net) return fail(BASTR_EINVAL, "bastr_closure_get_params: NULL handle");
+    SyncLock lk(g_sync_mu);
+    if (params) HIPCHK(hipMemcpy(params, net->params, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost));
+    if (scales)
+        for (int q = 0; q < BASTR_CLOSURE_NSCALE; ++q) scales[q] = net->scales[q];
+    return BASTR_OK;
+}
+
+int bastr_closure_fit_scales(bastr_closure* net, const bastr_pairs* ps) {
+    if (!net || !ps) return fail(BASTR_EINVAL, "bastr_closure_fit_scales: NULL argument");
+    if (ps->count < 1) return fail(BASTR_EINVAL, "bastr_closure_fit_scales: no pairs");
+    const int nb = mlp::sumsq_blocks();
+    std::vector<double> sums((size_t)nb * 6);
+    {
+        SyncLock lk(g_sync_mu);
+        double* d = nullptr;
+        HIPCHK(hipMalloc(&d, sizeof(double) * sums.size()));
+        const int rl = mlp::launch_sumsq(pairs_data(ps), d, net->stream);
+        const hipError_t e1 = hipGetLastError();
+        const hipError_t e2 = hipStreamSynchronize(net->stream);
+        const hipError_t e3 = hipMemcpy(sums.data(), d, sizeof(double) * sums.size(), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        if (rl || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            return fail(BASTR_EDEVICE, "bastr_closure_fit_scales: reduction failed");
+    }
+    float s[BASTR_CLOSURE_NSCALE];
+    const double npts = (double)ps->count * ps->n * ps->n;
+    double rms[6];
+    for (int q = 0; q < 6; ++q) {
+        double a = 0.0;
+        for (int b = 0; b < nb; ++b) a += sums[(size_t)b * 6 + q];
+        rms[q] = std::sqrt(a / npts);
+    }
+    // fields come in pairs of one unit (u1 u2, theta omega, tau1 tau2); a field
+    // at roundoff level beside its partner (omega of a potential flow, which
+    // stays potential under Burgers) takes the partner's scale, so that its
+    // noise enters the network near zero instead of at unit size
+    for (int q = 0; q < 6; ++q) {
+        const double other = rms[q ^ 1];
+        double v = rms[q] < 1e-6 * other ? other : rms[q];
+        if (!(v > 0.0)) v = 1.0;  // both identically zero: unit scale
+        s[q] = (float)v;
+    }
+    set_scales(net, s);
+    return BASTR_OK;
+}
+
+int bastr_closure_loss_grad(bastr_closure* net, const bastr_pairs* ps, int32_t first, int32_t count, double* loss,
+                            float* grad) {
+    if (!net || !ps) return fail(BASTR_EINVAL, "bastr_closure_loss_grad: NULL argument");
+    if (first < 0 || count < 1 || first + count > ps->count)
+        return fail(BASTR_EINVAL, "bastr_closure_loss_grad: pairs [%d, %d) not in [0, %d)", first, first + count, ps->count);
+    mlp::TileSel sel{};
+    sel.random = 0;
+    sel.s0 = first;
+    sel.ns = count;
+    sel.ntiles = (long)count * ps->n * (ps->n / 32);
+    int rc = loss_grad_launch(net, ps, sel, (double)count * ps->n * ps->n, net->dloss);
+    if (rc) return rc;
+    SyncLock lk(g_sync_mu);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (loss) HIPCHK(hipMemcpy(loss, net->dloss, sizeof(double), hipMemcpyDeviceToHost));
+    if (grad) HIPCHK(hipMemcpy(grad, net->grad, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost));
+    return BASTR_OK;
+}
+
+int bastr_closure_train(bastr_closure* net, const bastr_pairs* ps, int32_t iters, int32_t batch, float lr,
+                        uint64_t seed, double* losses) {
+    if (!net || !ps) return fail(BASTR_EINVAL, "bastr_closure_train: NULL argument");
+    if (ps->count < 1) return fail(BASTR_EINVAL, "bastr_closure_train: no pairs");
+    if (iters < 0 || batch < 1 || !(lr > 0.0f)) return fail(BASTR_EINVAL, "bastr_closure_train: bad iters/batch/lr");
+    const int chunk = 4096;  // losses kept on the device per chunk of steps
+    for (int i0 = 0; i0 < iters; i0 += chunk) {
+        const int ni = std::min(chunk, iters - i0);
+        for (int i = 0; i < ni; ++i) {
+            mlp::TileSel sel{};
+            sel.random = 1;
+            sel.seed = seed * 0x9E3779B97F4A7C15ull + (uint64_t)(net->t + 1);
+            sel.ntiles = batch;
+            int rc = loss_grad_launch(net, ps, sel, (double)batch * 32.0, net->dloss + i);
+            if (rc) return rc;
+            net->t += 1;
+            LAUNCHCHK(mlp::launch_adam(net->params, net->am, net->av, net->grad, lr, net->t, net->stream));
+            LAUNCHCHK(mlp::launch_pack(net->params, net->packed, net->stream));
+        }
+        SyncLock lk(g_sync_mu);
+        HIPCHK(hipStreamSynchronize(net->stream));
+        if (losses) HIPCHK(hipMemcpy(losses + i0, net->dloss, sizeof(double) * ni, hipMemcpyDeviceToHost));
+    }
+    SyncLock lk(g_sync_mu);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    return BASTR_OK;
+}
+
+int bastr_closure_apply(bastr_closure* net, const bastr_pairs* ps, int32_t i, float* force) {
+    if (!net || !ps || !force) return fail(BASTR_EINVAL, "bastr_closure_apply: NULL argument");
+    if (i < 0 || i >= ps->count) return fail(BASTR_EINVAL, "bastr_closure_apply: snapshot %d not in [0, %d)", i, ps->count);
+    const size_t n2 = (size_t)ps->n * ps->n;
+    SyncLock lk(g_sync_mu);
+    float* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(float) * 2 * n2));
+    mlp::Fields<float> F{};
+    for (int q = 0; q < 4; ++q) F.f[q] = ps->fld + ((size_t)i * 4 + q) * n2;
+    F.n = ps->n;
+    F.shift = ps->shift;
+    const int rl = mlp::launch_forward<float>(net->packed, net->sc, F, d, d + n2, net->stream);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(net->stream);
+    const hipError_t e3 = hipMemcpy(force, d, sizeof(float) * 2 * n2, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (rl || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return fail(BASTR_EDEVICE, "bastr_closure_apply: inference failed");
+    return BASTR_OK;
+}
+
+int bastr_closure_attach(bastr_handle* h, bastr_closure* net) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_closure_attach: NULL handle");
+    if (net) {
+        if (h->nmem != 1 || !square_pow2(h, 32, 4096))
+            return fail(BASTR_EINVAL, "bastr_closure_attach: one-field square 2-D handles, n a power of two in [32, 4096]");
+        if (h->p.timemethod != 1) return fail(BASTR_EINVAL, "bastr_closure_attach: RK3 (timemethod 1) only");
+        if (h->overlap) return fail(BASTR_EINVAL, "bastr_closure_attach: not with the two-stream order (BASTR_OVERLAP)");
+        const int rc = closure_buffers(h);
+        if (rc) return rc;
+    }
+    drop_graph(h);  // captured step graphs do not hold the closure's launches (or still do)
+    h->net = net;
+    h->net_version = net ? net->version : 0;
+    return BASTR_OK;
 }
 
 }  // extern "C"

@@ -84,7 +84,7 @@ def cpu_model():
     return name, count or os.cpu_count()
 
 
-NCLASS = 11  # BASTR_K_NCLASS (include/bastr_prof.h)
+NCLASS = 12  # BASTR_K_NCLASS (include/bastr_prof.h)
 
 
 def read_prof(lib, h, ncls=NCLASS):

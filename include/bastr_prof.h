@@ -31,6 +31,7 @@ enum bastr_kclass {
     BASTR_K_VG_DIS,          /* 2-D: the dissipation moment's x pass (VELGRAD is
                                 then stage A's x pass); 3-D: unused (VELGRAD
                                 covers both velocity-gradient kernels)           */
+    BASTR_K_CLOSURE,         /* learned subgrid force of a stage (bastr_closure.h) */
     BASTR_K_NCLASS
 };
 

@@ -213,6 +213,10 @@ struct bo_state {
     double* Es; double* Ed; double* kn; int64_t* Ecount;
     double Esspe, Edspe;
     int law;                         /* spectrum law of bo_init_random_phase */
+    /* learned closure (2-D): FP32 master parameters, scales; NULL = none */
+    float* cl_params;
+    float cl_scales[6];
+    double* cl_fld;                  /* [4][n] u1, u2, theta, omega of a stage */
     int64_t nstep;
     double time;
     int prologue_pending;
@@ -289,6 +293,84 @@ static void projection(bo_state* s, cplx** t) {
     }
 }
 
+/* The closure network of include/bastr_closure.h on one point: FP64 with the
+ * device's BF16 roundings (weights, scaled features, layer-1 activations). */
+static float bf16r(float x) {
+    union { float f; uint32_t u; } v = {x};
+    if ((v.u & 0x7fffffffu) > 0x7f800000u) return x;
+    v.u = ((v.u + 0x7fffu + ((v.u >> 16) & 1u)) >> 16) << 16;
+    return v.f;
+}
+static double gelu_t(double z) { return 0.5 * z * (1.0 + tanh(0.7978845608028654 * (z + 0.044715 * z * z * z))); }
+
+static void closure_point(const bo_state* s, const float* x, double* out) {
+    const float* p = s->cl_params;
+    double h1[64], h2[64];
+    for (int h = 0; h < 64; ++h) {
+        double z = p[2304 + h];
+        for (int f = 0; f < 36; ++f) z += (double)bf16r(p[h * 36 + f]) * (double)x[f];
+        h1[h] = (double)bf16r((float)gelu_t(z));
+    }
+    for (int h = 0; h < 64; ++h) {
+        double z = p[2368 + 4096 + h];
+        for (int k = 0; k < 64; ++k) z += (double)bf16r(p[2368 + h * 64 + k]) * h1[k];
+        h2[h] = gelu_t(z);
+    }
+    for (int c = 0; c < 2; ++c) {
+        double o = p[6656 + c];
+        for (int h = 0; h < 64; ++h) o += (double)bf16r(p[6528 + c * 64 + h]) * h2[h];
+        out[c] = o * (double)s->cl_scales[4 + c];
+    }
+}
+
+/* subgrid force of the stage field: u (spe, physical) and its gradients (ux) */
+static void closure_force(bo_state* s, cplx** t) {
+    const int ia = s->ia, ja = s->ja;
+    const size_t n = s->n;
+    double* F = s->cl_fld;
+    for (size_t m = 0; m < n; ++m) {
+        F[m] = creal(s->spe[0][m]);
+        F[n + m] = creal(s->spe[1][m]);
+        F[2 * n + m] = creal(s->ux[0][0][m]) + creal(s->ux[1][1][m]);
+        F[3 * n + m] = creal(s->ux[1][0][m]) - creal(s->ux[0][1][m]);
+    }
+    float inv[4];
+    for (int c = 0; c < 4; ++c) inv[c] = 1.0f / s->cl_scales[c];
+    OMP_FOR
+    for (long mm = 0; mm < (long)n; ++mm) {
+        const int i = (int)(mm % ia), j = (int)(mm / ia);
+        float x[36];
+        for (int pos = 0; pos < 9; ++pos) {
+            const int dx = pos % 3 - 1, dy = pos / 3 - 1;
+            const size_t q = (size_t)((i + dx + ia) % ia) + (size_t)ia * (size_t)((j + dy + ja) % ja);
+            for (int c = 0; c < 4; ++c) x[4 * pos + c] = bf16r((float)F[c * n + q] * inv[c]);
+        }
+        double o[2];
+        closure_point(s, x, o);
+        t[0][mm] += o[0];
+        t[1][mm] += o[1];
+    }
+}
+
+int bo_set_closure(bo_state* s, const float* params, const float* scales) {
+    if (s->nd != 2 || s->ia != s->ja) return fail(BASTR_EINVAL, "bo_set_closure: square 2-D grids only");
+    if (!params) {
+        free(s->cl_params);
+        free(s->cl_fld);
+        s->cl_params = NULL;
+        s->cl_fld = NULL;
+        return BASTR_OK;
+    }
+    if (!s->cl_params) {
+        s->cl_params = (float*)malloc(sizeof(float) * 6658);
+        s->cl_fld = (double*)malloc(sizeof(double) * 4 * s->n);
+        if (!s->cl_params || !s->cl_fld) return fail(BASTR_ENOMEM, "bo_set_closure: allocation failed");
+    }
+    memcpy(s->cl_params, params, sizeof(float) * 6658);
+    for (int c = 0; c < 6; ++c) s->cl_scales[c] = scales[c];
+    return BASTR_OK;
+}
+
 /* compute_ut2D (solution.F90:816-863) and compute_ut3D (:865-928) */
 static void compute_ut(bo_state* s, cplx** t) {
     const int nd = s->nd;
@@ -322,6 +404,7 @@ static void compute_ut(bo_state* s, cplx** t) {
                           - s->spe[2][m] * creal(s->ux[i][2][m]) + nu * creal(s->uxixi[i][m]);
         }
     }
+    if (s->cl_params) closure_force(s, t);
     for (int i = 0; i < nd; ++i) fftf(s, t[i]);
     for (int i = 0; i < nd; ++i) dealias(s, t[i]);
     if (s->p.lproject) projection(s, t);
@@ -750,6 +833,7 @@ int bo_destroy(bo_state* s) {
     }
     free(s->thetaxixi);
     free(s->Es); free(s->Ed); free(s->kn); free(s->Ecount);
+    free(s->cl_params); free(s->cl_fld);
     if (s->plan.p[0].tw) plannd_free(&s->plan);
     free(s);
     return BASTR_OK;

@@ -28,6 +28,9 @@ int bo_allkmax(const bo_state* s);
 int bo_set_fields(bo_state* s, const double* u1, const double* u2, const double* u3);
 int bo_init_random_phase(bo_state* s, double k0, uint64_t seed, double energy);
 int bo_set_spectrum_law(bo_state* s, int32_t law);
+/* The learned closure of include/bastr_closure.h added to every compute_ut
+ * (2-D, square grids): params NPARAM floats, scales 6 (NULL params: none). */
+int bo_set_closure(bo_state* s, const float* params, const float* scales);
 int bo_get_fields(bo_state* s, double* u1, double* u2, double* u3);
 int bo_set_time(bo_state* s, int64_t nstep, double time);
 int bo_get_time(const bo_state* s, int64_t* nstep, double* time);

@@ -26,6 +26,8 @@ def lib():
         abi.bind(L, "bo")
         L.bo_fft_c2c.restype = C.c_int
         L.bo_fft_c2c.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.bo_set_closure.restype = C.c_int
+        L.bo_set_closure.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.bo_set_threads.argtypes = [C.c_int]
         L.bo_get_threads.restype = C.c_int
         _LIB = L
@@ -34,6 +36,20 @@ def lib():
 
 def solver(params):
     return Solver(lib(), "bo", params)
+
+
+def set_closure(solver, params, scales):
+    """Attach the closure network (params, scales as float32 arrays; None: detach)
+    to an oracle solver: every compute_ut then adds its force."""
+    if params is None:
+        rc = lib().bo_set_closure(solver._h, None, None)
+    else:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        s = np.ascontiguousarray(scales, dtype=np.float32)
+        rc = lib().bo_set_closure(solver._h, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                  s.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc:
+        raise RuntimeError("bo_set_closure failed")
 
 
 def set_threads(n):

@@ -13,10 +13,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "bastr.h")
 
 
-def declared_entry_points():
-    text = open(HEADER).read()
+def declared_entry_points(path=HEADER):
+    text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(bastr_[a-z_]+)\s*\(", text)))
+
+
+def all_headers():
+    d = os.path.join(ROOT, "include")
+    return [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".h")]
 
 
 def test_header_declares_expected_entry_points():
@@ -24,6 +29,8 @@ def test_header_declares_expected_entry_points():
     expected = sorted("bastr_" + n for n, _, _ in abi.SIGNATURES + abi.ENSEMBLE_SIGNATURES
                       + abi.NOALLOC_SIGNATURES)
     assert names == expected
+    closure = declared_entry_points(os.path.join(ROOT, "include", "bastr_closure.h"))
+    assert closure == sorted("bastr_" + n for n, _, _ in abi.CLOSURE_SIGNATURES)
 
 
 def test_hip_library_exports_every_declared_symbol():
@@ -31,8 +38,9 @@ def test_hip_library_exports_every_declared_symbol():
     if not os.path.exists(bastr_amd.LIB_PATH):
         pytest.fail("HIP library not built; run __graft_entry__.build()")
     lib = C.CDLL(bastr_amd.LIB_PATH)
-    for name in declared_entry_points():
-        assert hasattr(lib, name), name
+    for header in all_headers():
+        for name in declared_entry_points(header):
+            assert hasattr(lib, name), (header, name)
     bastr_amd.load()
 
 

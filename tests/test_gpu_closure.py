@@ -1,0 +1,253 @@
+"""The learned subgrid closure on the GPU (include/bastr_closure.h):
+
+- bastr_filter_to and bastr_pairs_add against numpy: the sharp spherical
+  filter (the reference's dealiasing(field, factor), tool.F90:149-173, at the
+  coarse cutoff floor(n/3)), the coarse fields u1, u2, theta, omega, and the
+  subgrid force tau = P_c N(u) - D_c N(u_bar), N(u) = -u_j d_j u_i;
+- the BF16-MFMA inference and training kernels against the numpy oracle
+  (oracle/closure_oracle.py) emulating the same BF16 roundings: outputs, the
+  loss and its gradient;
+- training reduces the loss;
+- a coarse run with a network attached against the C oracle carrying the same
+  network in compute_ut (solution.F90:841-846), and the attach rules;
+- what a trained network gains: a coarse run with it against one without, in
+  E(k) for k < k_c against the filtered fine run on held-out seeds.
+"""
+import numpy as np
+import pytest
+
+from bastr_amd import abi
+from bastr_amd.closure import Closure, Pairs, evaluate, filter_to, make_pairs
+from bastr_amd.fields import hitgen
+from bastr_amd.solver import BastrError, Solver, make_params
+from oracle import closure_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+
+def kgrid(n):
+    k = np.arange(n, dtype=np.float64)
+    k[n // 2 + 1:] -= n
+    return np.meshgrid(k, k, indexing="ij")
+
+
+def sharp(u, n):
+    """numpy: the modes |k| < floor(n/3) of the fine field u, on the n x n grid."""
+    N = u.shape[0]
+    uh = np.fft.fft2(u) / (N * N)
+    k1, k2 = kgrid(N)
+    keep = np.sqrt(k1 ** 2 + k2 ** 2) < n // 3
+    c = np.zeros((n, n), dtype=complex)
+    idx = np.arange(-(n // 3), n // 3 + 1)
+    for a in idx:
+        for b in idx:
+            if a * a + b * b < (n // 3) ** 2 and keep[a % N, b % N]:
+                c[a % n, b % n] = uh[a % N, b % N]
+    return np.real(np.fft.ifft2(c) * n * n)
+
+
+def nonlinear(u1, u2):
+    """N(u) = -u_j d_j u_i on u's own grid (numpy, spectral derivatives)."""
+    k1, k2 = kgrid(u1.shape[0])
+
+    def d(f, k):
+        return np.real(np.fft.ifft2(1j * k * np.fft.fft2(f)))
+
+    return [-(u1 * d(u1, k1) + u2 * d(u1, k2)), -(u1 * d(u2, k1) + u2 * d(u2, k2))]
+
+
+def lowpass(f, n):
+    """the modes |k| < floor(n/3) of f (on its grid), on the n x n grid"""
+    return sharp(f, n)
+
+
+def fine_field(N, steps, seed=3, k0=4):
+    """a fine run of a field with both a dilatational and a solenoidal part (no
+    projection), so that theta and omega are both nonzero"""
+    p = make_params(N, N, forcemethod=2, lproject=False, kmax=128)
+    import bastr_amd
+    g = bastr_amd.solver(p)
+    d = hitgen((N, N), k0=k0, seed=seed)
+    r = hitgen((N, N), k0=k0 + 1, seed=seed + 100)
+    g.set_fields(d[0] - 0.7 * r[1], d[1] + 0.7 * r[0])
+    if steps:
+        g.step(steps)
+    return g
+
+
+def test_filter_to_is_the_sharp_filter(hip):
+    print("case closure filter_to 256 -> 64", flush=True)
+    g = fine_field(256, 3)
+    u = g.get_fields()
+    c = Solver(hip, "bastr", make_params(64, 64))
+    filter_to(g, c)
+    v = c.get_fields()
+    for a, b in zip(v, u):
+        want = sharp(b, 64)
+        assert np.abs(a - want).max() < 1e-13 * np.abs(want).max()
+    assert c.get_time() == g.get_time()
+
+
+def test_pairs_fields_and_subgrid_force(hip):
+    print("case closure pairs 256 -> 64", flush=True)
+    g = fine_field(256, 3, k0=10)  # energy well past the coarse cutoff k = 21
+    u = g.get_fields()
+    ps = Pairs(64, 2)
+    ps.add(g)
+    assert len(ps) == 1
+    fields, tau = ps.get(0)
+    ub = [sharp(x, 64) for x in u]
+    k1, k2 = kgrid(64)
+
+    def d(f, k):
+        return np.real(np.fft.ifft2(1j * k * np.fft.fft2(f)))
+
+    want = [ub[0], ub[1], d(ub[0], k1) + d(ub[1], k2), d(ub[1], k1) - d(ub[0], k2)]
+    for q in range(4):
+        assert np.abs(want[q]).max() > 1e-2 * np.abs(want[q & 2]).max()  # both parts present
+        assert np.abs(fields[q] - want[q]).max() < 1e-6 * np.abs(want[q]).max(), q
+    Nf = nonlinear(*u)
+    Nc = nonlinear(*ub)
+    for q in range(2):
+        want_t = lowpass(Nf[q], 64) - lowpass(Nc[q], 64)
+        err = np.abs(tau[q] - want_t).max() / np.abs(want_t).max()
+        print("tau component", q, "rel err", err, flush=True)
+        assert err < 1e-5
+        assert np.abs(want_t).max() > 1e-3 * np.abs(lowpass(Nf[q], 64)).max()  # the force is not negligible
+
+
+@pytest.fixture(scope="module")
+def small_pairs():
+    import bastr_amd
+    bastr_amd.load()
+    p = make_params(256, 256, forcemethod=2, lproject=True, kmax=128)
+    # k0 = 10: energy well past the coarse cutoff k = 21; snapshots of developed
+    # flow (shocks slave the subgrid scales to the resolved ones, so the force is
+    # a function of the resolved field to learn)
+    return make_pairs(p, 64, seeds=[1, 2], snap_steps=[200, 300], k0=10.0)
+
+
+def test_apply_matches_oracle(hip, small_pairs):
+    print("case closure apply vs oracle", flush=True)
+    net = Closure(seed=7)
+    net.fit_scales(small_pairs)
+    params, scales = net.params()
+    assert np.all(scales > 0)
+    for i in range(len(small_pairs)):
+        fields, _ = small_pairs.get(i)
+        X = co.features(fields, scales)
+        out, _ = co.forward(params, X)
+        got = net.apply(small_pairs, i)
+        for c in range(2):
+            want = out[:, c].reshape(64, 64, order="F") * scales[4 + c]
+            err = np.linalg.norm(got[c] - want) / np.linalg.norm(want)
+            assert err < 1e-4, (i, c, err)
+
+
+def test_loss_grad_matches_oracle(hip, small_pairs):
+    print("case closure loss and gradient vs oracle", flush=True)
+    net = Closure(seed=11)
+    net.fit_scales(small_pairs)
+    params, scales = net.params()
+    Xs, Ts = [], []
+    for i in range(len(small_pairs)):
+        fields, tau = small_pairs.get(i)
+        Xs.append(co.features(fields, scales))
+        Ts.append(co.targets(tau, scales))
+    lo, go = co.loss_grad(params, np.concatenate(Xs), np.concatenate(Ts))
+    lg, gg = net.loss_grad(small_pairs)
+    print("loss", lg, lo, flush=True)
+    assert abs(lg - lo) < 1e-5 * lo
+    for name, a, b in [("W1", co.OFF_W1, co.OFF_B1), ("b1", co.OFF_B1, co.OFF_W2), ("W2", co.OFF_W2, co.OFF_B2),
+                       ("b2", co.OFF_B2, co.OFF_W3), ("W3", co.OFF_W3, co.OFF_B3), ("b3", co.OFF_B3, co.NPARAM)]:
+        err = np.linalg.norm(gg[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        print(name, "grad rel err", err, flush=True)
+        assert err < 2e-3, (name, err)
+    # a sub-range of the pairs
+    l1, _ = net.loss_grad(small_pairs, 1, 2)
+    lo1, _ = co.loss_grad(params, np.concatenate(Xs[1:3]), np.concatenate(Ts[1:3]))
+    assert abs(l1 - lo1) < 1e-5 * lo1
+
+
+def test_training_reduces_the_loss(hip, small_pairs):
+    print("case closure training", flush=True)
+    net = Closure(seed=3)
+    net.fit_scales(small_pairs)
+    l0, _ = net.loss_grad(small_pairs)
+    losses = net.train(small_pairs, iters=400, batch=64, lr=2e-3, seed=5)
+    l1, _ = net.loss_grad(small_pairs)
+    print("loss before", l0, "after", l1, "batch losses", losses[:3], losses[-3:], flush=True)
+    assert np.all(np.isfinite(losses))
+    assert l1 < 0.5 * l0
+
+
+def test_closure_run_matches_oracle(hip, orc, small_pairs):
+    """A coarse 64^2 run with a network attached against the C oracle with the
+    same network in compute_ut; the difference is small against the network's
+    own effect."""
+    print("case closure run vs oracle 64^2", flush=True)
+    net = Closure(seed=9)
+    net.fit_scales(small_pairs)
+    params, scales = net.params()
+    params = params * 3.0  # a force well above roundoff
+    net.set_params(params, scales)
+    p = make_params(64, 64, deltat=4e-4, forcemethod=2, lproject=True, kmax=64)
+    u = hitgen((64, 64), k0=4, seed=12)
+    g = Solver(hip, "bastr", p)
+    g.set_fields(*u)
+    net.attach(g)
+    g0 = Solver(hip, "bastr", p)
+    g0.set_fields(*u)
+    o = orc.solver(p)
+    o.set_fields(*u)
+    orc.set_closure(o, params, scales)
+    rg, dg, sg = g.step(5)
+    r0, _, _ = g0.step(5)
+    ro, do, so = o.step(5)
+    assert rg == ro == r0 == 0
+    a, b, c = g.get_fields(), o.get_fields(), g0.get_fields()
+    diff = np.sqrt(sum(np.sum((x - y) ** 2) for x, y in zip(a, b)))
+    effect = np.sqrt(sum(np.sum((x - y) ** 2) for x, y in zip(b, c)))
+    size = np.sqrt(sum(np.sum(y ** 2) for y in b))
+    print("closure run: |gpu - oracle|", diff, "closure effect", effect, "field", size, flush=True)
+    assert effect > 1e-4 * size
+    assert diff < 1e-4 * effect
+    # detached, the run is the plain run again
+    Closure.detach(g)
+    g.set_fields(*u)
+    g0.set_fields(*u)
+    g.step(3)
+    g0.step(3)
+    assert all(np.array_equal(x, y) for x, y in zip(g.get_fields(), g0.get_fields()))
+
+
+def test_attach_rules(hip):
+    print("case closure attach rules", flush=True)
+    net = Closure(seed=1)
+    for p in [make_params(64, 64, timemethod=2), make_params(64, 32), make_params(16, 16),
+              make_params(32, 32, ka=32)]:
+        s = Solver(hip, "bastr", p)
+        with pytest.raises(BastrError) as e:
+            net.attach(s)
+        assert e.value.code == abi.BASTR_EINVAL
+
+
+def test_trained_closure_improves_the_coarse_spectrum(hip):
+    """Fine runs 1024^2 (dt 1e-4, Re 5000, band forcing k = 5, projection,
+    pure-law fields k0 = 8), coarse 128^2 (k_c = 42, dt 4e-4): a network trained
+    on four seeds brings the coarse E(k), k < k_c, closer to the filtered fine
+    run on two held-out seeds than the coarse run without it."""
+    print("case closure gain 1024 -> 128", flush=True)
+    fp = make_params(1024, 1024, forcemethod=2, lproject=True, kmax=341, deltat=1e-4)
+    cp = make_params(128, 128, forcemethod=2, lproject=True, kmax=42, deltat=4e-4)
+    ps = make_pairs(fp, 128, seeds=[1, 2, 3, 4], snap_steps=[300, 400, 500, 600], k0=8.0)
+    net = Closure(seed=2)
+    net.fit_scales(ps)
+    net.train(ps, iters=2000, batch=256, lr=2e-3, seed=7)
+    errs = {"without": [], "with": []}
+    for seed in (11, 12):
+        for label, nn in (("without", None), ("with", net)):
+            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=400, fine_steps=600, net=nn, k0=8.0)
+            errs[label].append(float(np.linalg.norm(Ec - Ef) / np.linalg.norm(Ef)))
+    print("E(k) rel-L2 against the filtered fine run:", errs, flush=True)
+    assert np.mean(errs["with"]) < 0.6 * np.mean(errs["without"])
