@@ -2909,6 +2909,53 @@ int bastr_closure_apply(bastr_closure* net, const bastr_pairs* ps, int32_t i, fl
     return BASTR_OK;
 }
 
+int bastr_prof_closure(bastr_closure* net, const bastr_pairs* ps, int reps, int batch, double* train_ms,
+                       double* forward_ms) {
+    if (!net || !ps || reps < 1 || batch < 1) return fail(BASTR_EINVAL, "bastr_prof_closure: bad argument");
+    if (ps->count < 1) return fail(BASTR_EINVAL, "bastr_prof_closure: no pairs");
+    mlp::TileSel sel{};
+    sel.random = 1;
+    sel.seed = 12345;
+    sel.ntiles = batch;
+    const int nw = mlp::train_waves(sel.ntiles);
+    int rc = net_part(net, nw);
+    if (rc) return rc;
+    const size_t n2 = (size_t)ps->n * ps->n;
+    SyncLock lk(g_sync_mu);
+    float* d = nullptr;
+    hipEvent_t e[4] = {};
+    HIPCHK(hipMalloc(&d, sizeof(float) * 2 * n2));
+    bool ok = true;
+    for (hipEvent_t& ev : e) ok = ok && hipEventCreate(&ev) == hipSuccess;
+    mlp::Fields<float> F{};
+    for (int q = 0; q < 4; ++q) F.f[q] = ps->fld + (size_t)q * n2;
+    F.n = ps->n;
+    F.shift = ps->shift;
+    const mlp::Data dd = pairs_data(ps);
+    const double npts = (double)batch * 32.0;
+    // one untimed launch of each, then the timed ones
+    ok = ok && mlp::launch_train(net->packed, net->sc, dd, sel, npts, net->part, net->part_loss, net->stream) == 0;
+    ok = ok && mlp::launch_forward<float>(net->packed, net->sc, F, d, d + n2, net->stream) == 0;
+    ok = ok && hipEventRecord(e[0], net->stream) == hipSuccess;
+    for (int r = 0; r < reps && ok; ++r)
+        ok = mlp::launch_train(net->packed, net->sc, dd, sel, npts, net->part, net->part_loss, net->stream) == 0;
+    ok = ok && hipEventRecord(e[1], net->stream) == hipSuccess;
+    ok = ok && hipEventRecord(e[2], net->stream) == hipSuccess;
+    for (int r = 0; r < reps && ok; ++r)
+        ok = mlp::launch_forward<float>(net->packed, net->sc, F, d, d + n2, net->stream) == 0;
+    ok = ok && hipEventRecord(e[3], net->stream) == hipSuccess;
+    ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(net->stream) == hipSuccess;
+    float t1 = 0, t2 = 0;
+    ok = ok && hipEventElapsedTime(&t1, e[0], e[1]) == hipSuccess && hipEventElapsedTime(&t2, e[2], e[3]) == hipSuccess;
+    for (hipEvent_t ev : e)
+        if (ev) (void)hipEventDestroy(ev);
+    (void)hipFree(d);
+    if (!ok) return fail(BASTR_EDEVICE, "bastr_prof_closure: timing failed");
+    if (train_ms) *train_ms = t1 / reps;
+    if (forward_ms) *forward_ms = t2 / reps;
+    return BASTR_OK;
+}
+
 int bastr_closure_attach(bastr_handle* h, bastr_closure* net) {
     if (!h) return fail(BASTR_EINVAL, "bastr_closure_attach: NULL handle");
     if (net) {

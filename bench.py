@@ -357,6 +357,162 @@ def bench_ensemble(args):
     print(json.dumps(out))
 
 
+# --config 2d-closure: the learned subgrid closure (include/bastr_closure.h)
+def closure_prof_api(lib):
+    lib.bastr_prof_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+    lib.bastr_prof_closure.restype = C.c_int
+
+
+def bench_closure(args):
+    """Coarse 2-D 256^2 solve (dt = 4e-4) with the per-point MLP closure,
+    trained on pairs from 2048^2 fine runs of this solver (FP64, Re = 5000,
+    dt = 1e-4, band forcing k = 5 E_t = 0.075, projection; pure-law fields k0 = 4)
+    at 20 seeds (snapshots after 1000, 1150, 1300, 1450, 1600 iterations,
+    filtered at k_c = 85; Adam at learning rates 2e-3, 7e-4, 2e-4 for half,
+    a quarter and a quarter of the steps), judged on 5 held-out seeds: the
+    coarse run starts from the filtered fine field after 1000 fine iterations
+    and runs 150 iterations (600 fine ones); E(k) for k < k_c against the
+    filtered fine run, with and without the network: rel-L2 over the shells,
+    and the mean and largest per-shell relative error.  value = the closure run's grid-pt.steps/s; the fine run's
+    rate beside it, and both in simulated time per second."""
+    import numpy as np
+
+    import bastr_amd
+    from bastr_amd.closure import Closure, evaluate, filter_to, fine_run, make_pairs
+    from bastr_amd.solver import make_params
+
+    lib = bastr_amd.load()
+    prof_api(lib)
+    closure_prof_api(lib)
+    N, n = 2048, 256
+    fp = make_params(N, N, reynolds=5000.0, deltat=1e-4, forcemethod=2, forcek=5, target_energy=0.075,
+                     lproject=True, kmax=128)
+    cp = make_params(n, n, reynolds=5000.0, deltat=4e-4, forcemethod=2, forcek=5, target_energy=0.075,
+                     lproject=True, kmax=n // 3)
+    train_seeds, test_seeds = list(range(1, 21)), list(range(21, 26))
+    snaps = [1000, 1150, 1300, 1450, 1600]
+    t0 = time.perf_counter()
+    ps = make_pairs(fp, n, train_seeds, snaps, k0=4.0)
+    t_pairs = time.perf_counter() - t0
+    print(f"  [2d-closure] {len(ps)} training pairs in {t_pairs:.1f} s", file=sys.stderr, flush=True)
+    net = Closure(seed=1)
+    net.fit_scales(ps)
+    iters = args.steps if args.steps is not None else 8000
+    t1 = time.perf_counter()
+    losses = np.concatenate([net.train(ps, iters=iters // 2, batch=256, lr=2e-3, seed=3),
+                             net.train(ps, iters=iters // 4, batch=256, lr=7e-4, seed=4),
+                             net.train(ps, iters=iters - iters // 2 - iters // 4, batch=256, lr=2e-4, seed=5)])
+    t_train = time.perf_counter() - t1
+    print(f"  [2d-closure] {iters} training steps in {t_train:.2f} s, batch loss {losses[0]:.4f} -> "
+          f"{float(np.mean(losses[-50:])):.4f}", file=sys.stderr, flush=True)
+    # E(k), k < k_c, on the held-out seeds
+    res = {"without": [], "with": []}
+    for seed in test_seeds:
+        for label, nn in (("without", None), ("with", net)):
+            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=1000, fine_steps=1600, net=nn, k0=4.0)
+            rel = np.abs(Ec - Ef) / Ef
+            res[label].append({"seed": seed, "ek_rel_l2": float(np.linalg.norm(Ec - Ef) / np.linalg.norm(Ef)),
+                               "ek_err_mean_per_shell": float(rel.mean()),
+                               "ek_err_max_per_shell": float(rel.max())})
+    summ = {k: {q: float(np.mean([r[q] for r in v]))
+                for q in ("ek_rel_l2", "ek_err_mean_per_shell", "ek_err_max_per_shell")}
+            for k, v in res.items()}
+    print(f"  [2d-closure] held-out E(k): {summ}", file=sys.stderr, flush=True)
+    # throughput: the closure run (graph replay) and the fine run
+    f = fine_run(fp, 21, 4.0)
+    f.step(1000, want_spectra=False)
+    c = bastr_amd.solver(cp)
+    filter_to(f, c)
+    net.attach(c)
+    steps = 150
+    c.step(steps, want_spectra=False)  # captures the graph of the call length
+    filter_to(f, c)
+    tc = time.perf_counter()
+    rc_c = c.step(steps, want_spectra=True)[0]
+    t_coarse = (time.perf_counter() - tc) / steps
+    f.step(steps, want_spectra=False)
+    tf = time.perf_counter()
+    rc_f = f.step(steps, want_spectra=True)[0]
+    t_fine = (time.perf_counter() - tf) / steps
+    c.close()
+    f.close()
+    # the network's kernels against the BF16 MFMA peak
+    tr_ms, fw_ms = C.c_double(), C.c_double()
+    if lib.bastr_prof_closure(net._h, ps._h, 20, 2048, C.byref(tr_ms), C.byref(fw_ms)) != 0:
+        raise RuntimeError("bastr_prof_closure failed")
+    fwd_flops, train_pts = 13056.0, 2048 * 32
+    train_tf = 3 * fwd_flops * train_pts / (tr_ms.value * 1e-3) / 1e12
+    fwd_tf = fwd_flops * n * n / (fw_ms.value * 1e-3) / 1e12
+    peak = 2500.0  # dense BF16 MFMA, TFLOP/s
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's coarse run with the same network in compute_ut
+        from oracle import oracle
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1), 16)
+        oracle.set_threads(threads)
+        g0 = fine_run(fp, 21, 4.0)
+        g0.step(1000, want_spectra=False)
+        cc = bastr_amd.solver(cp)
+        filter_to(g0, cc)
+        u = cc.get_fields()
+        g0.close()
+        cc.close()
+        o = oracle.solver(cp)
+        params, scales = net.params()
+        oracle.set_closure(o, params, scales)
+        o.set_fields(*u)
+        o.get_fields()
+        its = 150
+        tc = time.perf_counter()
+        o.step(its)
+        dt_cpu = time.perf_counter() - tc
+        o.close()
+        model, ncpu = cpu_model()
+        cpu = {"value": n * n * its / dt_cpu, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
+               "cpu_model": model, "host_logical_cpus": ncpu,
+               "sample": f"{its} iterations of the 256^2 closure run ({dt_cpu:.1f} s), FP64 C restatement with "
+                         f"the network's forward pass in compute_ut, OpenMP"}
+    out = {
+        "metric": METRIC,
+        "value": n * n / t_coarse,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": steps,
+        "ms_per_step": t_coarse * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: pure-law random-phase fields E(k) = k^4 exp(-2 (k/4)^2), u_rms=1; training seeds 1-20, "
+                "held-out seeds 21-25",
+        "config": {"workload": "learned closure: coarse 2-D 256^2 Burgers, dt=4e-4, Re=5000, band forcing k=5 "
+                               "E_t=0.075, lproject=t, with a 36-64-64-2 GELU MLP (BF16 MFMA) trained on filtered "
+                               "(k_c=85) 2048^2 runs of this solver",
+                   "grid": [n, n], "fine_grid": [N, N], "precision": "fp64 solve, bf16 network"},
+        "roofline": {"bound": "mfma", "achieved": round(train_tf, 2), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(train_tf / peak, 5), "traffic": None, "kernel": "mlp::train_kernel",
+                     "flops_per_launch": 3 * fwd_flops * train_pts, "mean_launch_us": round(tr_ms.value * 1e3, 2),
+                     "inference": {"kernel": "mlp::forward_kernel", "achieved": round(fwd_tf, 2),
+                                   "frac": round(fwd_tf / peak, 5), "flops_per_launch": fwd_flops * n * n,
+                                   "mean_launch_us": round(fw_ms.value * 1e3, 2)}},
+        "cpu_baseline": cpu,
+        "closure": {
+            "pairs": len(ps), "pairs_seconds": round(t_pairs, 2), "train_steps": iters,
+            "train_seconds": round(t_train, 3), "batch_loss_first": float(losses[0]),
+            "batch_loss_last50": float(np.mean(losses[-50:])),
+            "held_out": summ, "held_out_per_seed": res,
+            "coarse_ms_per_step": t_coarse * 1e3, "fine_ms_per_step": t_fine * 1e3,
+            "fine_grid_pt_steps_per_s": N * N / t_fine,
+            "simulated_time_per_s": {"closure_run": cp.deltat / t_coarse, "fine_run": fp.deltat / t_fine},
+            "simulated_time_ratio": (cp.deltat / t_coarse) / (fp.deltat / t_fine),
+            "status": {"coarse": rc_c, "fine": rc_f},
+        },
+    }
+    print(json.dumps(out))
+
+
 def cfg_params(name, precision=None):
     from bastr_amd.solver import make_params
     cfg = CONFIGS[name]
@@ -466,7 +622,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", default="2d-2048", choices=sorted(CONFIGS) + ["2d-ensemble96"])
+    ap.add_argument("--config", default="2d-2048", choices=sorted(CONFIGS) + ["2d-ensemble96", "2d-closure"])
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -493,6 +649,8 @@ def main():
         sys.exit(2)
     if args.config == "2d-ensemble96":
         return bench_ensemble(args)
+    if args.config == "2d-closure":
+        return bench_closure(args)
 
     import numpy as np
 

@@ -49,6 +49,17 @@ const char* bastr_prof_name(int kclass);
  * (each byte read once and written once). */
 int bastr_prof_copy_gbps(bastr_handle* h, uint64_t bytes, int reps, double* gbps);
 
+/* The closure network's kernels (bastr_closure.h) timed with HIP events on its
+ * stream: `reps` launches of the training kernel on `batch` random 32-point
+ * tiles of the pairs (forward and backward, no parameter update) and `reps`
+ * launches of the inference kernel over pair 0; mean milliseconds per launch.
+ * Algorithmic work: 13056 flops per point forward (2 (36*64 + 64*64 + 64*2)),
+ * three times that per point trained. */
+struct bastr_closure;
+struct bastr_pairs;
+int bastr_prof_closure(struct bastr_closure* net, const struct bastr_pairs* ps, int reps, int batch,
+                       double* train_ms, double* forward_ms);
+
 #ifdef __cplusplus
 }
 #endif
