@@ -32,6 +32,30 @@ def solver(params):
     return Solver(load(), "bastr", params)
 
 
+def autotune(n, precision=abi.PREC_F64):
+    """Time both radix plans of the length-n column passes and keep the faster
+    (bastr_autotune): returns (points per thread, [ms of the 16-point plan,
+    ms of the 8-point plan])."""
+    L = load()
+    pts = C.c_int32()
+    ms = (C.c_double * 2)()
+    rc = L.bastr_autotune(int(n), int(precision), C.byref(pts), ms)
+    if rc:
+        raise BastrError(rc, L.bastr_last_error().decode())
+    return pts.value, [ms[0], ms[1]]
+
+
+def tune_set(n, points, precision=abi.PREC_F64):
+    L = load()
+    rc = L.bastr_tune_set(int(n), int(precision), int(points))
+    if rc:
+        raise BastrError(rc, L.bastr_last_error().decode())
+
+
+def tune_get(n, precision=abi.PREC_F64):
+    return load().bastr_tune_get(int(n), int(precision))
+
+
 def ensemble(params, nmembers, forced=None):
     """A HIP-backed ensemble of nmembers independent 2-D runs on one grid;
     forced: None or nmembers flags (0 = free decay, 1 = forced as params say)."""

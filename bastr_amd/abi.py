@@ -89,10 +89,13 @@ ENSEMBLE_SIGNATURES = [
 ]
 
 
-# The step without device allocation (HIP library only; no counterpart in the
-# reference or the oracle).
+# The step without device allocation and the column radix plans (HIP library
+# only; no counterpart in the reference or the oracle).
 NOALLOC_SIGNATURES = [
     ("step_noalloc", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
+    ("autotune", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), _D]),
+    ("tune_set", C.c_int, [C.c_int32, C.c_int32, C.c_int32]),
+    ("tune_get", C.c_int, [C.c_int32, C.c_int32]),
 ]
 
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -210,6 +211,11 @@ struct bastr_handle {
     // graph and replayed while the step count stays the same
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;
+    // identity for graphs captured over several handles (bastr_step_group):
+    // gen is unique per handle in the process, epoch counts the events that
+    // invalidate captured launches (new buffers, closure scales, attach)
+    uint64_t gen = 0, epoch = 0;
+    uint64_t plan_epoch = 0;  // column radix plans the captured graph was made with
     bool use_graph = true;
     // post-processing arena (bastr_pp_*), allocated on first use
     void* pp[30] = {};
@@ -1137,6 +1143,7 @@ void drop_graph(bastr_handle* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
     h->gsteps = 0;
+    h->epoch += 1;  // group graphs holding this handle's launches are stale too
 }
 
 // longest step call replayed as one graph (about 25 nodes per iteration)
@@ -1200,8 +1207,10 @@ int collect_ensemble(bastr_handle* h, int nsteps, bastr_diag* diags, double* spe
     return any_nan ? BASTR_STOP_NAN : BASTR_OK;
 }
 
+// The per-call launches before a step call's iterations: the pending entry
+// dealias, diagnostic slots, the NaN gate and the fill values
 template <typename T>
-int step_launch(bastr_handle* h, int nsteps) {
+int step_prepare(bastr_handle* h, int nsteps) {
     int rc = 0;
     if (h->prologue_pending) {
         if ((rc = (h->nd == 3 ? run_prologue3<T>(h) : run_prologue<T>(h)))) return rc;
@@ -1214,16 +1223,108 @@ int step_launch(bastr_handle* h, int nsteps) {
         hipLaunchKernelGGL(fill_diag_kernel, dim3((nfill + 255) / 256), dim3(256), 0, h->stream, h->ddiag, nfill);
         HIPCHK(hipGetLastError());
     }
+    if (h->net && h->net_version != h->net->version) {
+        drop_graph(h);  // the captured closure launches hold the network's scales
+        h->net_version = h->net->version;
+    }
+    if (h->plan_epoch != k3d::colx_plan_epoch()) {
+        drop_graph(h);  // the captured column passes are those of the old radix plan
+        h->plan_epoch = k3d::colx_plan_epoch();
+    }
+    return 0;
+}
+
+// Time the two radix plans of the single-term column passes of length n
+// (precision prec) on the current device: per plan, a representative pass
+// (the forward and inverse transforms of 256 columns with two jobs) launched
+// `reps` times between HIP events after a warm-up launch; ms[0] is the
+// 16-point plan, ms[1] the 8-point one (NaN where that plan does not exist).
+template <typename T>
+int time_plans(int n, double* ms) {
+    const int ncols = 256, reps = 10;
+    const size_t elems = (size_t)ncols * n;
+    SyncLock lk(g_sync_mu);
+    void *in = nullptr, *out = nullptr, *out1 = nullptr, *tw = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void* p : {in, out, out1, tw})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    bool ok = hipMalloc(&in, sizeof(cx<T>) * elems) == hipSuccess && hipMalloc(&out, sizeof(cx<T>) * elems) == hipSuccess &&
+              hipMalloc(&out1, sizeof(cx<T>) * elems) == hipSuccess;
+    ok = ok && hipMemset(in, 0, sizeof(cx<T>) * elems) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    if (ok) {
+        std::vector<cx<T>> twh;
+        make_twiddles<T>(n, twh);
+        ok = hipMalloc(&tw, sizeof(cx<T>) * n) == hipSuccess &&
+             hipMemcpy(tw, twh.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    int* st = nullptr;
+    ok = ok && hipMalloc(&st, sizeof(int)) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(64), 0, s, st, 1, INT_MAX);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    const int prec = sizeof(T) == 4 ? 1 : 0;
+    const int keep = k3d::colx_points(n, prec);
+    for (int v = 0; v < 2 && ok; ++v) {
+        ms[v] = std::nan("");
+        if (v == 1 && !k3d::colx_has8(n, prec)) continue;
+        k3d::set_colx_points(n, prec, v == 0 ? 16 : 8);
+        k3d::ColXArgs<T> c{};
+        c.gate = Gate{st, 0, 0};
+        c.in[0] = C<T>(in);
+        c.in[1] = C<T>(in);
+        c.out[0] = C<T>(out);
+        c.out[1] = C<T>(out1);
+        c.job[0] = job1(0, k3d::M_ONE, 0);
+        c.job[1] = job1(1, k3d::M_IK, 1);
+        c.njob = 2;
+        c.pitch = ncols;
+        c.ncols = ncols;
+        c.nplanes = 1;
+        c.scale = 1.0;
+        c.tw = C<T>(tw);
+        c.mstride = 0;
+        c.nmem = 1;
+        float t0 = 0.f;
+        for (int r = 0; r <= reps && ok; ++r) {
+            if (r == 1) ok = hipEventRecord(e0, s) == hipSuccess;
+            ok = ok && k3d::launch_colx<T>(n, true, c, s) == 0 && k3d::launch_colx<T>(n, false, c, s) == 0;
+        }
+        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&t0, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
+        ms[v] = (double)t0 / reps;
+    }
+    k3d::set_colx_points(n, prec, keep);
+    if (st) (void)hipFree(st);
+    cleanup();
+    if (!ok) rc = fail(BASTR_EDEVICE, "bastr_autotune: timing of the column plans failed");
+    return rc;
+}
+
+bool graph_ok(const bastr_handle* h, int nsteps) {
+    return h->use_graph && h->prof_mask == 0 && nsteps <= kMaxGraphSteps;
+}
+
+template <typename T>
+int step_launch(bastr_handle* h, int nsteps) {
+    int rc = step_prepare<T>(h, nsteps);
+    if (rc) return rc;
     // The ~25 launches of each iteration are replayed as one graph; the capture
     // is kept for the next call with the same step count (every argument is
     // fixed per handle: buffers, the iteration index within the call, the
     // device-side NaN gate).  Profiled calls launch directly: their timing
     // events stay out of the capture.
-    if (h->net && h->net_version != h->net->version) {
-        drop_graph(h);  // the captured closure launches hold the network's scales
-        h->net_version = h->net->version;
-    }
-    if (h->use_graph && h->prof_mask == 0 && nsteps <= kMaxGraphSteps) {
+    if (graph_ok(h, nsteps)) {
         if (!h->gexec || h->gsteps != nsteps) {
             SyncLock lk(g_sync_mu);
             drop_graph(h);
@@ -1289,6 +1390,131 @@ template <typename T>
 int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     const int rc = step_launch<T>(h, nsteps);
     return rc ? rc : step_collect(h, nsteps, diags, spectra);
+}
+
+// ---------------------------------------------------------------------------
+// One graph over several handles (bastr_step_group): the iterations of every
+// handle, each on its own stream forked from and joined to the first handle's,
+// captured once and replayed as a single graph launch.  Entries are keyed by
+// the handles' (gen, epoch) and the step count: a handle's new buffers, a
+// changed closure or its destruction never match an old entry.
+// ---------------------------------------------------------------------------
+struct GroupGraph {
+    std::vector<std::pair<uint64_t, uint64_t>> key;
+    int nsteps = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;  // [0] fork, [1..nh-1] joins (capture); [nh..2nh-1] prepared, [2nh] done
+    uint64_t last_use = 0;
+};
+std::vector<GroupGraph> g_groups;
+std::mutex g_groups_mu;
+uint64_t g_group_clock = 0;
+constexpr size_t kMaxGroupGraphs = 4;
+
+void group_entry_free(GroupGraph& e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    for (hipEvent_t x : e.ev)
+        if (x) (void)hipEventDestroy(x);
+    e.exec = nullptr;
+    e.ev.clear();
+}
+
+// drop every group graph that holds handle `gen`
+void group_forget(uint64_t gen) {
+    std::lock_guard<std::mutex> lk(g_groups_mu);
+    for (size_t i = 0; i < g_groups.size();) {
+        bool has = false;
+        for (auto& k : g_groups[i].key) has = has || k.first == gen;
+        if (has) {
+            group_entry_free(g_groups[i]);
+            g_groups.erase(g_groups.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+template <typename T>
+int group_iterations(bastr_handle* h, int nsteps) {
+    int rc = 0;
+    for (int it = 0; it < nsteps && !rc; ++it) rc = h->nd == 3 ? run_iteration3<T>(h, it) : run_iteration<T>(h, it);
+    return rc;
+}
+
+int group_body(bastr_handle* h, int nsteps) {
+    return h->prec == BASTR_PREC_F32 ? group_iterations<float>(h, nsteps) : group_iterations<double>(h, nsteps);
+}
+
+// Launch the iterations of every handle as one graph (after each handle's
+// step_prepare); the caller then collects each handle on its own stream.
+int group_launch(bastr_handle* const* hs, int nh, int nsteps) {
+    std::vector<std::pair<uint64_t, uint64_t>> key(nh);
+    for (int i = 0; i < nh; ++i) key[i] = {hs[i]->gen, hs[i]->epoch};
+    std::lock_guard<std::mutex> lk(g_groups_mu);
+    GroupGraph* e = nullptr;
+    for (auto& x : g_groups)
+        if (x.nsteps == nsteps && x.key == key) e = &x;
+    if (!e) {
+        if (g_groups.size() >= kMaxGroupGraphs) {  // evict the least recently used
+            size_t lru = 0;
+            for (size_t i = 1; i < g_groups.size(); ++i)
+                if (g_groups[i].last_use < g_groups[lru].last_use) lru = i;
+            group_entry_free(g_groups[lru]);
+            g_groups.erase(g_groups.begin() + (long)lru);
+        }
+        GroupGraph n;
+        n.key = key;
+        n.nsteps = nsteps;
+        n.ev.assign(2 * nh + 1, nullptr);
+        for (auto& x : n.ev)
+            if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) {
+                group_entry_free(n);
+                return fail(BASTR_EDEVICE, "bastr_step_group: event creation failed");
+            }
+        SyncLock sl(g_sync_mu);
+        hipStream_t s0 = hs[0]->stream;
+        hipGraph_t g = nullptr;
+        int rc = 0;
+        hipError_t ec = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+        if (ec == hipSuccess) {
+            // every other handle's stream joins the capture through the fork event
+            ec = hipEventRecord(n.ev[0], s0);
+            for (int i = 1; i < nh && ec == hipSuccess; ++i) ec = hipStreamWaitEvent(hs[i]->stream, n.ev[0], 0);
+            for (int i = 0; i < nh && ec == hipSuccess && !rc; ++i) rc = group_body(hs[i], nsteps);
+            for (int i = 1; i < nh && ec == hipSuccess && !rc; ++i) {
+                ec = hipEventRecord(n.ev[i], hs[i]->stream);
+                if (ec == hipSuccess) ec = hipStreamWaitEvent(s0, n.ev[i], 0);
+            }
+            const hipError_t ee = hipStreamEndCapture(s0, &g);
+            if (ec == hipSuccess) ec = ee;
+        }
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            group_entry_free(n);
+            return rc ? rc : fail(BASTR_EDEVICE, "bastr_step_group: graph capture failed: %s", hipGetErrorString(ec));
+        }
+        ec = hipGraphInstantiate(&n.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess) {
+            n.exec = nullptr;
+            group_entry_free(n);
+            return fail(BASTR_EDEVICE, "bastr_step_group: graph instantiation failed: %s", hipGetErrorString(ec));
+        }
+        g_groups.push_back(n);
+        e = &g_groups.back();
+    }
+    e->last_use = ++g_group_clock;
+    hipStream_t s0 = hs[0]->stream;
+    // each handle's prepared state (entry dealias, gates, slots) before the graph
+    for (int i = 1; i < nh; ++i) {
+        HIPCHK(hipEventRecord(e->ev[nh + i], hs[i]->stream));
+        HIPCHK(hipStreamWaitEvent(s0, e->ev[nh + i], 0));
+    }
+    HIPCHK(hipGraphLaunch(e->exec, s0));
+    // each handle's result copies after the graph
+    HIPCHK(hipEventRecord(e->ev[2 * nh], s0));
+    for (int i = 1; i < nh; ++i) HIPCHK(hipStreamWaitEvent(hs[i]->stream, e->ev[2 * nh], 0));
+    return 0;
 }
 
 }  // namespace
@@ -1431,6 +1657,8 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     if (hipGetDevice(&dev) != hipSuccess) return fail(BASTR_EDEVICE, "bastr_create: hipGetDevice failed");
     if (int rs = scratch_init(dev)) return rs;
     bastr_handle* h = new bastr_handle();
+    static std::atomic<uint64_t> next_gen{1};
+    h->gen = next_gen.fetch_add(1);
     h->dev = dev;
     h->p = *p;
     h->nd = nd;
@@ -1472,6 +1700,22 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     }
     h->npts = (size_t)h->n0 * h->n1 * h->n2;
     h->nhalf = (size_t)h->hp * h->n1 * h->n2;
+    {
+        // "1": time both radix plans of each column length this handle uses, once per
+        // length and precision in the process (bastr_autotune), before its first step
+        static std::mutex tuned_mu;
+        static std::vector<int> tuned;
+        const char* tv = std::getenv("BASTR_TUNE");
+        if (tv && std::strcmp(tv, "1") == 0) {
+            for (int len : {h->n1, nd == 3 ? h->n2 : 0}) {
+                if (!len) continue;
+                std::lock_guard<std::mutex> tl(tuned_mu);
+                const int key = len * 2 + h->prec;
+                if (std::find(tuned.begin(), tuned.end(), key) != tuned.end()) continue;
+                if (bastr_autotune(len, h->prec, nullptr, nullptr) == 0) tuned.push_back(key);
+            }
+        }
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(BASTR_EDEVICE, "bastr_create: hipStreamCreate failed");
@@ -1593,6 +1837,7 @@ int bastr_nmembers(const bastr_handle* h) { return h ? h->nmem : -1; }
 
 int bastr_destroy(bastr_handle* h) {
     if (!h) return BASTR_OK;
+    group_forget(h->gen);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->side) (void)hipStreamSynchronize(h->side);
     for (int c = 0; c < 2; ++c)
@@ -1954,14 +2199,33 @@ int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_
             if (hs[j] == hs[i]) return fail(BASTR_EINVAL, "bastr_step_group: handle %d given twice", i);
     }
     if (nsteps == 0) return BASTR_OK;
-    // every handle's launches go to its own stream first, so they overlap on
-    // the device; then each is waited for and its results collected
-    for (int i = 0; i < nh; ++i) {
-        bastr_handle* h = hs[i];
-        const int rc = h->prec == BASTR_PREC_F32 ? step_launch<float>(h, nsteps) : step_launch<double>(h, nsteps);
+    bool one_graph = nh > 1;
+    for (int i = 0; i < nh; ++i) one_graph = one_graph && graph_ok(hs[i], nsteps) && hs[i]->dev == hs[0]->dev;
+    if (one_graph) {
+        // every handle's iterations in one captured graph, replayed by one launch
+        for (int i = 0; i < nh; ++i) {
+            bastr_handle* h = hs[i];
+            const int rc = h->prec == BASTR_PREC_F32 ? step_prepare<float>(h, nsteps) : step_prepare<double>(h, nsteps);
+            if (rc) {
+                for (int j = 0; j < nh; ++j) (void)hipStreamSynchronize(hs[j]->stream);
+                return rc;
+            }
+        }
+        const int rc = group_launch(hs, nh, nsteps);
         if (rc) {
-            for (int j = 0; j < i; ++j) (void)hipStreamSynchronize(hs[j]->stream);
+            for (int j = 0; j < nh; ++j) (void)hipStreamSynchronize(hs[j]->stream);
             return rc;
+        }
+    } else {
+        // every handle's launches go to its own stream first, so they overlap on
+        // the device; then each is waited for and its results collected
+        for (int i = 0; i < nh; ++i) {
+            bastr_handle* h = hs[i];
+            const int rc = h->prec == BASTR_PREC_F32 ? step_launch<float>(h, nsteps) : step_launch<double>(h, nsteps);
+            if (rc) {
+                for (int j = 0; j < i; ++j) (void)hipStreamSynchronize(hs[j]->stream);
+                return rc;
+            }
         }
     }
     int ret = BASTR_OK;
@@ -1975,6 +2239,35 @@ int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_
     }
     return ret;
 }
+
+int bastr_autotune(int32_t n, int32_t precision, int32_t* points, double* ms) {
+    if (!pow2_in(n, 16, 4096)) return fail(BASTR_EINVAL, "bastr_autotune: n = %d is not a power of two in [16, 4096]", n);
+    if (precision != BASTR_PREC_F64 && precision != BASTR_PREC_F32)
+        return fail(BASTR_EINVAL, "bastr_autotune: bad precision %d", precision);
+    double t[2] = {std::nan(""), std::nan("")};
+    const int rc = precision == BASTR_PREC_F32 ? time_plans<float>(n, t) : time_plans<double>(n, t);
+    if (rc) return rc;
+    const int best = (!std::isnan(t[1]) && t[1] < t[0]) ? 8 : 16;
+    k3d::set_colx_points(n, precision, best);
+    if (points) *points = best;
+    if (ms) {
+        ms[0] = t[0];
+        ms[1] = t[1];
+    }
+    return BASTR_OK;
+}
+
+int bastr_tune_set(int32_t n, int32_t precision, int32_t points) {
+    if (!pow2_in(n, 16, 4096) || (precision != BASTR_PREC_F64 && precision != BASTR_PREC_F32))
+        return fail(BASTR_EINVAL, "bastr_tune_set: bad length or precision");
+    if (points != 16 && points != 8) return fail(BASTR_EINVAL, "bastr_tune_set: points must be 16 or 8");
+    if (points == 8 && !k3d::colx_has8(n, precision))
+        return fail(BASTR_EINVAL, "bastr_tune_set: no 8-point plan for n = %d in this precision", n);
+    k3d::set_colx_points(n, precision, points);
+    return BASTR_OK;
+}
+
+int bastr_tune_get(int32_t n, int32_t precision) { return k3d::colx_points(n, precision); }
 
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts) {
     if (!h || !counts) return fail(BASTR_EINVAL, "bastr_shell_counts: NULL argument");

@@ -38,7 +38,8 @@ struct ColCfg {
     static constexpr int CBW = (256 / P) > CBMIN ? (256 / P) : CBMIN;
     static constexpr int CB = CBW > 16 ? 16 : CBW;
     static constexpr int THREADS = CB * P;
-    static_assert(CB >= 1 && THREADS <= 1024 && N >= E, "column config");
+    static constexpr bool FITS = THREADS <= 1024;  // a plan a workgroup can hold
+    static_assert(CB >= 1 && N >= E, "column config");
     static_assert(EMAX == 8 || EMAX == 16, "column config");
     static_assert(CB * (N + N / 16) * (int)sizeof(T) <= 81920, "column LDS");
 };

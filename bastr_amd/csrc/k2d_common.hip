@@ -1,7 +1,9 @@
 // 2-D step: precision-independent kernels and launch-shape queries.
+#include <atomic>
 #include <climits>
 #include <math.h>
 #include "k2d.hpp"
+#include "k3d.hpp"
 
 namespace k2d {
 
@@ -148,3 +150,45 @@ int vg_block_rows(int n, int prec, int nd) {
 }
 
 }  // namespace k2d
+
+namespace k3d {
+
+namespace {
+std::atomic<int> g_colx_points[2][13];  // [prec][log2 n]; 0 = the default (16)
+std::atomic<uint64_t> g_plan_epoch{0};
+
+int lg2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+template <typename T>
+bool has8_of(int n) {
+    switch (n) {
+#define X(NN) case NN: return NN >= 256 && ColCfg<T, NN, 8>::FITS;
+        X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#undef X
+    }
+    return false;
+}
+}  // namespace
+
+int colx_points(int n, int prec) {
+    const int l = lg2(n);
+    if (l < 0 || l > 12 || prec < 0 || prec > 1) return 16;
+    const int v = g_colx_points[prec][l].load(std::memory_order_relaxed);
+    return v == 8 ? 8 : 16;
+}
+
+void set_colx_points(int n, int prec, int points) {
+    const int l = lg2(n);
+    if (l < 0 || l > 12 || prec < 0 || prec > 1) return;
+    if (g_colx_points[prec][l].exchange(points) != points) g_plan_epoch.fetch_add(1);
+}
+
+uint64_t colx_plan_epoch() { return g_plan_epoch.load(); }
+
+bool colx_has8(int n, int prec) { return prec == 1 ? has8_of<float>(n) : has8_of<double>(n); }
+
+}  // namespace k3d

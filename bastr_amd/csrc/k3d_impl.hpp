@@ -49,14 +49,17 @@ __device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko, T kc, T kco) {
     }
 }
 
-// the multi-term variant runs 8 points per thread in FP64 (register pressure)
-template <typename T, int N, int MT>
-using ColxCfg = ColCfg<T, N, (MT == 1 || sizeof(T) == 4) ? 16 : 8>;
+// the multi-term variant runs 8 points per thread in FP64 (register pressure);
+// the single-term one EM points (16 by default, 8 where the autotuner finds
+// radix-8 passes faster, colx_points)
+template <typename T, int N, int MT, int EM = 16>
+using ColxCfg = ColCfg<T, N, MT == 1 ? EM : (sizeof(T) == 4 ? 16 : 8)>;
 
-template <typename T, int N, bool INV, int MT>
-__global__ __launch_bounds__((ColxCfg<T, N, MT>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
+template <typename T, int N, bool INV, int MT, int EM = 16>
+__global__ __launch_bounds__((ColxCfg<T, N, MT, EM>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
 void colx_kernel(ColXArgs<T> a) {
-    using C = ColxCfg<T, N, MT>;
+    using C = ColxCfg<T, N, MT, EM>;
+    static_assert(C::FITS, "column plan exceeds a workgroup");
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
@@ -914,8 +917,12 @@ __global__ __launch_bounds__(256) void scale3_kernel(Scale3Args<T> a) {
 template <typename T, int N>
 static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     using C = ColxCfg<T, N, 1>;
+    using C8 = ColxCfg<T, N, 1, 8>;
     using C4 = ColxCfg<T, N, 4>;
-    if (a.ncols % C::CB || a.ncols % C4::CB || a.njob < 1 || a.njob > 13) return -1;
+    constexpr bool has8 = N >= 256 && C8::FITS;  // a distinct 8-point plan exists
+    const bool use8 = has8 && colx_points(N, sizeof(T) == 4 ? 1 : 0) == 8;
+    const int cb1 = use8 ? C8::CB : C::CB;
+    if (a.ncols % cb1 || a.ncols % C4::CB || a.njob < 1 || a.njob > 13) return -1;
     // single-term and multi-term jobs go to the two kernel variants
     ColXArgs<T> part[2] = {a, a};
     part[0].njob = part[1].njob = 0;
@@ -927,11 +934,17 @@ static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     for (int v = 0; v < 2; ++v) {
         const ColXArgs<T>& b = part[v];
         if (!b.njob) continue;
-        const int cb = v == 0 ? C::CB : C4::CB;
+        const int cb = v == 0 ? cb1 : C4::CB;
         const long nbl = (long)b.njob * (b.ncols / cb) * b.nplanes;
         if (nbl > INT_MAX - 8) return -1;
-        dim3 grid(k2d::xcd_grid_h((int)nbl), 1, k2d::zdim(b.nmem)), block(v == 0 ? C::THREADS : C4::THREADS);
-        if (v == 0) {
+        const int threads = v == 0 ? (use8 ? C8::THREADS : C::THREADS) : C4::THREADS;
+        dim3 grid(k2d::xcd_grid_h((int)nbl), 1, k2d::zdim(b.nmem)), block(threads);
+        if (v == 0 && use8) {
+            if constexpr (has8) {
+                if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1, 8>), grid, block, 0, s, b);
+                else hipLaunchKernelGGL((colx_kernel<T, N, false, 1, 8>), grid, block, 0, s, b);
+            }
+        } else if (v == 0) {
             if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1>), grid, block, 0, s, b);
             else hipLaunchKernelGGL((colx_kernel<T, N, false, 1>), grid, block, 0, s, b);
         } else {

@@ -208,6 +208,26 @@ int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const 
                      double* totals);
 int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* out);
 
+/*
+ * Radix plans of the column (y, z) passes; no counterpart in the reference,
+ * whose transforms are planned by FFTW_MEASURE (fftwlink.F90:284-303).  The
+ * single-term column passes of length n run 16 points per thread (radix-16
+ * passes) unless a plan of 8 points per thread (radix-8 passes) is set for
+ * (n, precision); the table is process-wide and applies to every handle's
+ * later steps (captured step graphs are remade).
+ * bastr_autotune times both plans of length n on the current device (a
+ * representative pass; ms[0] the 16-point plan, ms[1] the 8-point one, NaN
+ * where it does not exist: n < 256, or a plan too wide for one workgroup) and
+ * keeps the faster; *points receives it.  Setting BASTR_TUNE=1 in the
+ * environment makes bastr_create tune each column length of the handle once
+ * per process.  bastr_tune_set fixes a plan, bastr_tune_get returns the one in
+ * use.  The plans differ in rounding only (every result stays within the
+ * parity tolerances); a fixed table gives reproducible bits.
+ */
+int bastr_autotune(int32_t n, int32_t precision, int32_t* points, double* ms);
+int bastr_tune_set(int32_t n, int32_t precision, int32_t points);
+int bastr_tune_get(int32_t n, int32_t precision);
+
 /* Number of modes per shell 1..allkmax (Ecount, solution.F90:976). */
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts);
 

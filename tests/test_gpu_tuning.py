@@ -1,0 +1,116 @@
+"""Radix plans of the column passes (bastr_autotune / bastr_tune_set; the
+reference's analogue is FFTW_MEASURE planning, fftwlink.F90:284-303) and
+graphs over several handles (bastr_step_group).
+
+Both plans of a length are checked against the oracle; switching plans remakes
+the captured step graphs; the autotuner reports its timings and keeps the
+faster plan.  A group graph stays correct when one of its handles is replaced.
+"""
+import numpy as np
+import pytest
+
+import bastr_amd
+from bastr_amd import abi
+from bastr_amd.fields import hitgen
+from bastr_amd.solver import Solver, make_params, step_group
+
+from test_gpu_parity import check_diags, mixed_field, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def default_plans():
+    yield
+    for n in (256, 512, 1024, 2048):
+        for prec in (0, 1):
+            bastr_amd.tune_set(n, 16, prec)
+
+
+@pytest.mark.parametrize("shape,prec", [((256, 256), 0), ((512, 2048), 0), ((512, 512), 1)])
+def test_both_plans_match_the_oracle(hip, orc, shape, prec):
+    print(f"case radix plans {shape} prec={prec}", flush=True)
+    orc.set_threads(16)
+    n1 = shape[1]
+    p = make_params(*shape, forcemethod=2, lproject=True, kmax=128, precision=prec)
+    u = mixed_field(shape, seed=31)
+    res = []
+    for pts in (16, 8):
+        bastr_amd.tune_set(n1, pts, prec)
+        assert bastr_amd.tune_get(n1, prec) == pts
+        g = Solver(hip, "bastr", p)
+        g.set_fields(*u)
+        rg, dg, sg = g.step(4)
+        assert rg == 0
+        res.append((g.get_fields(), dg, sg))
+        g.close()
+    o = orc.solver(make_params(*shape, forcemethod=2, lproject=True, kmax=128))
+    o.set_fields(*u)
+    ro, do, so = o.step(4)
+    tol = 1e-11 if prec == 0 else 1e-5
+    for f, dg, sg in res:
+        assert rel_l2(f, o.get_fields()) < tol
+        if prec == 0:
+            check_diags(dg, do)
+    # the two plans differ in rounding only
+    assert rel_l2(res[0][0], res[1][0]) < (1e-13 if prec == 0 else 1e-5)
+
+
+def test_plan_change_remakes_captured_graphs(hip):
+    print("case radix plan switch under graph replay", flush=True)
+    p = make_params(256, 256, forcemethod=2, lproject=True, kmax=128)
+    u = hitgen((256, 256), k0=4, seed=8)
+    a, b, d = (Solver(hip, "bastr", p) for _ in range(3))
+    for s in (a, b, d):
+        s.set_fields(*u)
+    bastr_amd.tune_set(256, 16)
+    for s in (a, b, d):
+        s.step(4)  # captures each handle's graph with the 16-point plan
+    bastr_amd.tune_set(256, 8)
+    a.step(4)
+    b.step(4)
+    bastr_amd.tune_set(256, 16)
+    d.step(4)
+    fa, fb, fd = a.get_fields(), b.get_fields(), d.get_fields()
+    assert all(np.array_equal(x, y) for x, y in zip(fa, fb))
+    assert not all(np.array_equal(x, y) for x, y in zip(fa, fd))  # the switch took effect
+    assert rel_l2(fa, fd) < 1e-13
+
+
+@pytest.mark.parametrize("n,prec", [(2048, 0), (512, 1)])
+def test_autotune_keeps_the_faster_plan(hip, n, prec):
+    print(f"case autotune {n} prec={prec}", flush=True)
+    pts, ms = bastr_amd.autotune(n, prec)
+    print("autotune", n, prec, pts, ms, flush=True)
+    assert pts in (8, 16) and bastr_amd.tune_get(n, prec) == pts
+    assert np.isfinite(ms[0]) and ms[0] > 0
+    if np.isfinite(ms[1]):
+        assert pts == (8 if ms[1] < ms[0] else 16)
+    else:
+        assert pts == 16
+
+
+def test_group_graph_with_a_replaced_handle(hip):
+    print("case step_group one graph, handle replaced", flush=True)
+
+    def mk(shape, seed):
+        s = Solver(hip, "bastr", make_params(*shape, forcemethod=2, lproject=True, kmax=128))
+        s.set_fields(*hitgen(shape, k0=4, seed=seed))
+        return s
+
+    a, b = mk((128, 128), 1), mk((64, 64), 2)
+    a2, b2 = mk((128, 128), 1), mk((64, 64), 2)
+    rc, _ = step_group([a, b], 5)
+    assert rc == 0
+    a2.step(5)
+    b2.step(5)
+    b.close()
+    b2.close()
+    c, c2 = mk((64, 64), 3), mk((64, 64), 3)  # a new handle, possibly at b's address
+    for _ in range(2):  # capture, then replay
+        rc, out = step_group([a, c], 5)
+        assert rc == 0
+        a2.step(5)
+        c2.step(5)
+    for x, y in ((a, a2), (c, c2)):
+        assert all(np.array_equal(f, g) for f, g in zip(x.get_fields(), y.get_fields()))
