@@ -32,28 +32,34 @@ def solver(params):
     return Solver(load(), "bastr", params)
 
 
-def autotune(n, precision=abi.PREC_F64):
-    """Time both radix plans of the length-n column passes and keep the faster
-    (bastr_autotune): returns (points per thread, [ms of the 16-point plan,
-    ms of the 8-point plan])."""
+def autotune(n, precision=abi.PREC_F64, ncols=None, nplanes=1):
+    """Time the plans of the length-n column passes (ncols columns of nplanes
+    planes; default: the kept columns of an n x n 2-D step) and keep the fastest
+    (bastr_autotune): returns ((points per thread, width multiplier),
+    [ms of the plans (16, 1), (8, 1), (16, 2), (8, 2)], NaN where absent)."""
     L = load()
-    pts = C.c_int32()
-    ms = (C.c_double * 2)()
-    rc = L.bastr_autotune(int(n), int(precision), C.byref(pts), ms)
+    if ncols is None:
+        ncols = max(16, (n // 3 + 15) // 16 * 16)
+    pts, wid = C.c_int32(), C.c_int32()
+    ms = (C.c_double * 4)()
+    rc = L.bastr_autotune(int(n), int(precision), int(ncols), int(nplanes), C.byref(pts), C.byref(wid), ms)
     if rc:
         raise BastrError(rc, L.bastr_last_error().decode())
-    return pts.value, [ms[0], ms[1]]
+    return (pts.value, wid.value), list(ms)
 
 
-def tune_set(n, points, precision=abi.PREC_F64):
+def tune_set(n, points, precision=abi.PREC_F64, width=1):
     L = load()
-    rc = L.bastr_tune_set(int(n), int(precision), int(points))
+    rc = L.bastr_tune_set(int(n), int(precision), int(points), int(width))
     if rc:
         raise BastrError(rc, L.bastr_last_error().decode())
 
 
 def tune_get(n, precision=abi.PREC_F64):
-    return load().bastr_tune_get(int(n), int(precision))
+    """(points per thread, width multiplier) of the length-n column plan in use."""
+    pts, wid = C.c_int32(), C.c_int32()
+    load().bastr_tune_get(int(n), int(precision), C.byref(pts), C.byref(wid))
+    return pts.value, wid.value
 
 
 def ensemble(params, nmembers, forced=None):

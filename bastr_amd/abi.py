@@ -93,9 +93,10 @@ ENSEMBLE_SIGNATURES = [
 # only; no counterpart in the reference or the oracle).
 NOALLOC_SIGNATURES = [
     ("step_noalloc", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
-    ("autotune", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), _D]),
-    ("tune_set", C.c_int, [C.c_int32, C.c_int32, C.c_int32]),
-    ("tune_get", C.c_int, [C.c_int32, C.c_int32]),
+    ("autotune", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                           C.POINTER(C.c_int32), _D]),
+    ("tune_set", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    ("tune_get", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 ]
 
 

@@ -1235,14 +1235,20 @@ int step_prepare(bastr_handle* h, int nsteps) {
 }
 
 // Time the two radix plans of the single-term column passes of length n
-// (precision prec) on the current device: per plan, a representative pass
-// (the forward and inverse transforms of 256 columns with two jobs) launched
-// `reps` times between HIP events after a warm-up launch; ms[0] is the
-// 16-point plan, ms[1] the 8-point one (NaN where that plan does not exist).
+// (precision prec) on the current device, in the shape of a real pass: ncols
+// columns of each of nplanes planes, four jobs (as a stage's y-inverse), the
+// inverse and the forward transform, launched `reps` times between HIP events
+// after a warm-up launch; ms[v] for the plans (points, width) = (16, 1),
+// (8, 1), (16, 2), (8, 2), NaN where a plan does not exist.
 template <typename T>
-int time_plans(int n, double* ms) {
-    const int ncols = 256, reps = 10;
-    const size_t elems = (size_t)ncols * n;
+int time_plans(int n, int ncols, int nplanes, double* ms) {
+    const int reps = 5;
+    // at least 768 MiB per buffer, so that the timed passes stream from HBM as a
+    // step's do (the 256 MB Infinity Cache would hold smaller buffers)
+    const size_t plane_bytes = sizeof(cx<T>) * (size_t)ncols * n;
+    nplanes = std::max<int>(nplanes, (int)(((size_t)768 << 20) / plane_bytes + 1));
+    nplanes = std::min<int>(nplanes, (int)(((size_t)1 << 28) / ((size_t)ncols * n)));
+    const size_t elems = (size_t)ncols * n * nplanes;
     SyncLock lk(g_sync_mu);
     void *in = nullptr, *out = nullptr, *out1 = nullptr, *tw = nullptr;
     hipStream_t s = nullptr;
@@ -1274,11 +1280,12 @@ int time_plans(int n, double* ms) {
         ok = hipGetLastError() == hipSuccess;
     }
     const int prec = sizeof(T) == 4 ? 1 : 0;
-    const int keep = k3d::colx_points(n, prec);
-    for (int v = 0; v < 2 && ok; ++v) {
+    const int keep_p = k3d::colx_points(n, prec), keep_w = k3d::colx_width(n, prec);
+    for (int v = 0; v < 4 && ok; ++v) {
         ms[v] = std::nan("");
-        if (v == 1 && !k3d::colx_has8(n, prec)) continue;
-        k3d::set_colx_points(n, prec, v == 0 ? 16 : 8);
+        const int pts = (v & 1) ? 8 : 16, wid = (v & 2) ? 2 : 1;
+        if (!k3d::colx_plan_ok(n, prec, pts, wid) || ncols % k3d::colx_plan_cb(n, prec, pts, wid) != 0) continue;
+        k3d::set_colx_plan(n, prec, pts, wid);
         k3d::ColXArgs<T> c{};
         c.gate = Gate{st, 0, 0};
         c.in[0] = C<T>(in);
@@ -1287,10 +1294,13 @@ int time_plans(int n, double* ms) {
         c.out[1] = C<T>(out1);
         c.job[0] = job1(0, k3d::M_ONE, 0);
         c.job[1] = job1(1, k3d::M_IK, 1);
-        c.njob = 2;
+        c.job[2] = job1(0, k3d::M_IK, 0);
+        c.job[3] = job1(1, k3d::M_ONE, 1);
+        c.njob = 4;
         c.pitch = ncols;
         c.ncols = ncols;
-        c.nplanes = 1;
+        c.plane_stride = (long)ncols * n;
+        c.nplanes = nplanes;
         c.scale = 1.0;
         c.tw = C<T>(tw);
         c.mstride = 0;
@@ -1304,7 +1314,7 @@ int time_plans(int n, double* ms) {
              hipEventElapsedTime(&t0, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
         ms[v] = (double)t0 / reps;
     }
-    k3d::set_colx_points(n, prec, keep);
+    k3d::set_colx_plan(n, prec, keep_p, keep_w);
     if (st) (void)hipFree(st);
     cleanup();
     if (!ok) rc = fail(BASTR_EDEVICE, "bastr_autotune: timing of the column plans failed");
@@ -1707,12 +1717,15 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
         static std::vector<int> tuned;
         const char* tv = std::getenv("BASTR_TUNE");
         if (tv && std::strcmp(tv, "1") == 0) {
+            // the pass shape: the step's kept columns, every plane of a 3-D field up to 64
+            const int ncols = (h->prune ? h->cols : h->hp) / 16 * 16;
+            const int planes = nd == 3 ? std::min(h->n1, 64) : 1;
             for (int len : {h->n1, nd == 3 ? h->n2 : 0}) {
-                if (!len) continue;
+                if (!len || ncols < 16) continue;
                 std::lock_guard<std::mutex> tl(tuned_mu);
                 const int key = len * 2 + h->prec;
                 if (std::find(tuned.begin(), tuned.end(), key) != tuned.end()) continue;
-                if (bastr_autotune(len, h->prec, nullptr, nullptr) == 0) tuned.push_back(key);
+                if (bastr_autotune(len, h->prec, ncols, planes, nullptr, nullptr, nullptr) == 0) tuned.push_back(key);
             }
         }
     }
@@ -2240,34 +2253,47 @@ int bastr_step_group(bastr_handle* const* hs, int32_t nh, int32_t nsteps, bastr_
     return ret;
 }
 
-int bastr_autotune(int32_t n, int32_t precision, int32_t* points, double* ms) {
+int bastr_autotune(int32_t n, int32_t precision, int32_t ncols, int32_t nplanes, int32_t* points, int32_t* width,
+                   double* ms) {
     if (!pow2_in(n, 16, 4096)) return fail(BASTR_EINVAL, "bastr_autotune: n = %d is not a power of two in [16, 4096]", n);
     if (precision != BASTR_PREC_F64 && precision != BASTR_PREC_F32)
         return fail(BASTR_EINVAL, "bastr_autotune: bad precision %d", precision);
-    double t[2] = {std::nan(""), std::nan("")};
-    const int rc = precision == BASTR_PREC_F32 ? time_plans<float>(n, t) : time_plans<double>(n, t);
+    if (ncols < 16 || ncols % 16 || ncols > 4096 || nplanes < 1 || (size_t)ncols * n * nplanes > ((size_t)1 << 28))
+        return fail(BASTR_EINVAL, "bastr_autotune: ncols must be a multiple of 16 in [16, 4096] and "
+                                  "ncols * n * nplanes at most 2^28 (got %d, %d)", ncols, nplanes);
+    double t[4] = {std::nan(""), std::nan(""), std::nan(""), std::nan("")};
+    const int rc = precision == BASTR_PREC_F32 ? time_plans<float>(n, ncols, nplanes, t)
+                                               : time_plans<double>(n, ncols, nplanes, t);
     if (rc) return rc;
-    const int best = (!std::isnan(t[1]) && t[1] < t[0]) ? 8 : 16;
-    k3d::set_colx_points(n, precision, best);
-    if (points) *points = best;
-    if (ms) {
-        ms[0] = t[0];
-        ms[1] = t[1];
-    }
+    // the default plan unless another is at least 3 % faster (box-to-box spread
+    // of these timings is a few per cent; a smaller gain is not a gain)
+    int best = 0;
+    for (int v = 1; v < 4; ++v)
+        if (!std::isnan(t[v]) && t[v] < 0.97 * t[0] && t[v] < t[best]) best = v;
+    const int pts = (best & 1) ? 8 : 16, wid = (best & 2) ? 2 : 1;
+    k3d::set_colx_plan(n, precision, pts, wid);
+    if (points) *points = pts;
+    if (width) *width = wid;
+    if (ms)
+        for (int v = 0; v < 4; ++v) ms[v] = t[v];
     return BASTR_OK;
 }
 
-int bastr_tune_set(int32_t n, int32_t precision, int32_t points) {
+int bastr_tune_set(int32_t n, int32_t precision, int32_t points, int32_t width) {
     if (!pow2_in(n, 16, 4096) || (precision != BASTR_PREC_F64 && precision != BASTR_PREC_F32))
         return fail(BASTR_EINVAL, "bastr_tune_set: bad length or precision");
-    if (points != 16 && points != 8) return fail(BASTR_EINVAL, "bastr_tune_set: points must be 16 or 8");
-    if (points == 8 && !k3d::colx_has8(n, precision))
-        return fail(BASTR_EINVAL, "bastr_tune_set: no 8-point plan for n = %d in this precision", n);
-    k3d::set_colx_points(n, precision, points);
+    if (!k3d::colx_plan_ok(n, precision, points, width))
+        return fail(BASTR_EINVAL, "bastr_tune_set: no plan of %d points per thread at width x%d for n = %d in this "
+                                  "precision", points, width, n);
+    k3d::set_colx_plan(n, precision, points, width);
     return BASTR_OK;
 }
 
-int bastr_tune_get(int32_t n, int32_t precision) { return k3d::colx_points(n, precision); }
+int bastr_tune_get(int32_t n, int32_t precision, int32_t* points, int32_t* width) {
+    if (points) *points = k3d::colx_points(n, precision);
+    if (width) *width = k3d::colx_width(n, precision);
+    return BASTR_OK;
+}
 
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts) {
     if (!h || !counts) return fail(BASTR_EINVAL, "bastr_shell_counts: NULL argument");

@@ -29,19 +29,22 @@ namespace k2d {
 // padding) for the split real/imaginary exchange.  Small workgroups let
 // several of them share a CU, so one block's loads overlap another's
 // transform.
-template <typename T, int N, int EMAX = 16>
+// CBM = 2 doubles the block width (a wider tile, chosen by the autotuner where
+// it fits: at most 1024 threads and 160 KiB of LDS per workgroup).
+template <typename T, int N, int EMAX = 16, int CBM = 1>
 struct ColCfg {
     static constexpr int E = N >= 256 ? EMAX : 8;
     static constexpr int P = N / E;
     static constexpr int CBMIN =
         (sizeof(T) == 8 && N <= 2048 ? 64 : (sizeof(T) == 4 && N <= 1024 ? 128 : 32)) / (int)sizeof(cx<T>);
     static constexpr int CBW = (256 / P) > CBMIN ? (256 / P) : CBMIN;
-    static constexpr int CB = CBW > 16 ? 16 : CBW;
+    static constexpr int CB = (CBW > 16 ? 16 : CBW) * CBM;
     static constexpr int THREADS = CB * P;
-    static constexpr bool FITS = THREADS <= 1024;  // a plan a workgroup can hold
+    static constexpr int LDS_BYTES = CB * (N + N / 16) * (int)sizeof(T);
+    static constexpr bool FITS = THREADS <= 1024 && LDS_BYTES <= 163840;  // a plan a workgroup can hold
     static_assert(CB >= 1 && N >= E, "column config");
     static_assert(EMAX == 8 || EMAX == 16, "column config");
-    static_assert(CB * (N + N / 16) * (int)sizeof(T) <= 81920, "column LDS");
+    static_assert(CBM != 1 || LDS_BYTES <= 81920, "column LDS");
 };
 
 // Row-pass launch shape for (T, N0): L rows per workgroup, 8 points per

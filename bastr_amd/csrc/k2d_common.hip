@@ -154,7 +154,7 @@ int vg_block_rows(int n, int prec, int nd) {
 namespace k3d {
 
 namespace {
-std::atomic<int> g_colx_points[2][13];  // [prec][log2 n]; 0 = the default (16)
+std::atomic<int> g_colx_plan[2][13];  // [prec][log2 n]: points | width << 8; 0 = the default
 std::atomic<uint64_t> g_plan_epoch{0};
 
 int lg2(int n) {
@@ -163,32 +163,66 @@ int lg2(int n) {
     return l;
 }
 
+template <typename T, int N>
+bool ok_of(int points, int width) {
+    if (points == 16 && width == 1) return true;
+    if (points == 8 && N < 256) return false;  // below 256 every plan is the 8-point default
+    if (points == 8) return width == 1 ? ColCfg<T, N, 8, 1>::FITS : ColCfg<T, N, 8, 2>::FITS;
+    return ColCfg<T, N, 16, 2>::FITS;
+}
+
+template <typename T, int N>
+int cb_of_plan(int points, int width) {
+    if (points == 8) return width == 2 ? ColCfg<T, N, 8, 2>::CB : ColCfg<T, N, 8, 1>::CB;
+    return width == 2 ? ColCfg<T, N, 16, 2>::CB : ColCfg<T, N, 16, 1>::CB;
+}
+
 template <typename T>
-bool has8_of(int n) {
+int plan_cb_of(int n, int points, int width) {
     switch (n) {
-#define X(NN) case NN: return NN >= 256 && ColCfg<T, NN, 8>::FITS;
+#define X(NN) case NN: return cb_of_plan<T, NN>(points, width);
+        X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#undef X
+    }
+    return 0;
+}
+
+template <typename T>
+bool plan_ok_of(int n, int points, int width) {
+    switch (n) {
+#define X(NN) case NN: return ok_of<T, NN>(points, width);
         X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 #undef X
     }
     return false;
 }
+
+int plan_of(int n, int prec) {
+    const int l = lg2(n);
+    if (l < 0 || l > 12 || prec < 0 || prec > 1) return 0;
+    return g_colx_plan[prec][l].load(std::memory_order_relaxed);
+}
 }  // namespace
 
-int colx_points(int n, int prec) {
-    const int l = lg2(n);
-    if (l < 0 || l > 12 || prec < 0 || prec > 1) return 16;
-    const int v = g_colx_points[prec][l].load(std::memory_order_relaxed);
-    return v == 8 ? 8 : 16;
-}
+int colx_points(int n, int prec) { return (plan_of(n, prec) & 0xff) == 8 ? 8 : 16; }
+int colx_width(int n, int prec) { return (plan_of(n, prec) >> 8) == 2 ? 2 : 1; }
 
-void set_colx_points(int n, int prec, int points) {
+void set_colx_plan(int n, int prec, int points, int width) {
     const int l = lg2(n);
     if (l < 0 || l > 12 || prec < 0 || prec > 1) return;
-    if (g_colx_points[prec][l].exchange(points) != points) g_plan_epoch.fetch_add(1);
+    const int v = (points == 16 && width == 1) ? 0 : (points | (width << 8));
+    if (g_colx_plan[prec][l].exchange(v) != v) g_plan_epoch.fetch_add(1);
 }
 
 uint64_t colx_plan_epoch() { return g_plan_epoch.load(); }
 
-bool colx_has8(int n, int prec) { return prec == 1 ? has8_of<float>(n) : has8_of<double>(n); }
+int colx_plan_cb(int n, int prec, int points, int width) {
+    return prec == 1 ? plan_cb_of<float>(n, points, width) : plan_cb_of<double>(n, points, width);
+}
+
+bool colx_plan_ok(int n, int prec, int points, int width) {
+    if ((points != 8 && points != 16) || (width != 1 && width != 2)) return false;
+    return prec == 1 ? plan_ok_of<float>(n, points, width) : plan_ok_of<double>(n, points, width);
+}
 
 }  // namespace k3d

@@ -152,15 +152,19 @@ struct Scale3Args {
 
 template <typename T> int launch_colx(int n, bool inverse, const ColXArgs<T>& a, hipStream_t s);
 
-// Radix plan of the single-term column passes of length n (prec 0 FP64, 1
-// FP32): 16 or 8 points per thread (radix-16 or radix-8 passes).  A process-wide
-// table: 16 unless set (by the autotuner, bastr_autotune); colx_plan_epoch
-// counts the changes (captured step graphs hold the kernels of the old plan).
+// Plan of the single-term column passes of length n (prec 0 FP64, 1 FP32):
+// 16 or 8 points per thread (radix-16 or radix-8 passes) on blocks of the
+// default width or twice it (a wider tile, longer row segments).  A
+// process-wide table: 16 points, default width unless set (by the autotuner,
+// bastr_autotune); colx_plan_epoch counts the changes (captured step graphs
+// hold the kernels of the old plan).
 int colx_points(int n, int prec);
-void set_colx_points(int n, int prec, int points);
+int colx_width(int n, int prec);  // 1: the default block width, 2: twice it
+void set_colx_plan(int n, int prec, int points, int width);
 uint64_t colx_plan_epoch();
-// whether length n in this precision has an 8-point plan distinct from the default
-bool colx_has8(int n, int prec);
+// whether the plan (points per thread, width multiplier) exists for length n
+bool colx_plan_ok(int n, int prec, int points, int width);
+int colx_plan_cb(int n, int prec, int points, int width);  // its block width (columns)
 template <typename T> int launch_cf3(int n, int mode, const Cf3Args<T>& a, hipStream_t s);
 template <typename T> int launch_cf2(int n, int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s);
 // whether cf2 keeps the column passes' block width at n rows (else the

@@ -51,14 +51,14 @@ __device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko, T kc, T kco) {
 
 // the multi-term variant runs 8 points per thread in FP64 (register pressure);
 // the single-term one EM points (16 by default, 8 where the autotuner finds
-// radix-8 passes faster, colx_points)
-template <typename T, int N, int MT, int EM = 16>
-using ColxCfg = ColCfg<T, N, MT == 1 ? EM : (sizeof(T) == 4 ? 16 : 8)>;
+// radix-8 passes faster) on blocks CBM times the default width (colx_plan)
+template <typename T, int N, int MT, int EM = 16, int CBM = 1>
+using ColxCfg = ColCfg<T, N, MT == 1 ? EM : (sizeof(T) == 4 ? 16 : 8), MT == 1 ? CBM : 1>;
 
-template <typename T, int N, bool INV, int MT, int EM = 16>
-__global__ __launch_bounds__((ColxCfg<T, N, MT, EM>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
+template <typename T, int N, bool INV, int MT, int EM = 16, int CBM = 1>
+__global__ __launch_bounds__((ColxCfg<T, N, MT, EM, CBM>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
 void colx_kernel(ColXArgs<T> a) {
-    using C = ColxCfg<T, N, MT, EM>;
+    using C = ColxCfg<T, N, MT, EM, CBM>;
     static_assert(C::FITS, "column plan exceeds a workgroup");
     constexpr int E = C::E, P = C::P, CB = C::CB;
     if (a.gate.closed()) return;
@@ -914,14 +914,28 @@ __global__ __launch_bounds__(256) void scale3_kernel(Scale3Args<T> a) {
 // ---------------------------------------------------------------------------
 #define K3D_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512)
 
+// one single-term variant: (EM points per thread, CBM times the block width)
+template <typename T, int N, int EM, int CBM>
+static void colx1_launch(bool inverse, dim3 grid, const ColXArgs<T>& b, hipStream_t s) {
+    using C = ColxCfg<T, N, 1, EM, CBM>;
+    if constexpr (C::FITS && (EM == 16 || N >= 256)) {
+        if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1, EM, CBM>), grid, dim3(C::THREADS), 0, s, b);
+        else hipLaunchKernelGGL((colx_kernel<T, N, false, 1, EM, CBM>), grid, dim3(C::THREADS), 0, s, b);
+    }
+}
+
 template <typename T, int N>
 static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
-    using C = ColxCfg<T, N, 1>;
-    using C8 = ColxCfg<T, N, 1, 8>;
     using C4 = ColxCfg<T, N, 4>;
-    constexpr bool has8 = N >= 256 && C8::FITS;  // a distinct 8-point plan exists
-    const bool use8 = has8 && colx_points(N, sizeof(T) == 4 ? 1 : 0) == 8;
-    const int cb1 = use8 ? C8::CB : C::CB;
+    const int prec = sizeof(T) == 4 ? 1 : 0;
+    int em = colx_points(N, prec), cbm = colx_width(N, prec);
+    if (!colx_plan_ok(N, prec, em, cbm)) em = 16, cbm = 1;
+    auto cb_of = [](int e, int m) {
+        return e == 8 ? (m == 2 ? ColxCfg<T, N, 1, 8, 2>::CB : ColxCfg<T, N, 1, 8, 1>::CB)
+                      : (m == 2 ? ColxCfg<T, N, 1, 16, 2>::CB : ColxCfg<T, N, 1, 16, 1>::CB);
+    };
+    if (cbm == 2 && a.ncols % cb_of(em, 2)) cbm = 1;  // the wide tile needs whole blocks: the default width
+    const int cb1 = cb_of(em, cbm);
     if (a.ncols % cb1 || a.ncols % C4::CB || a.njob < 1 || a.njob > 13) return -1;
     // single-term and multi-term jobs go to the two kernel variants
     ColXArgs<T> part[2] = {a, a};
@@ -937,16 +951,12 @@ static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
         const int cb = v == 0 ? cb1 : C4::CB;
         const long nbl = (long)b.njob * (b.ncols / cb) * b.nplanes;
         if (nbl > INT_MAX - 8) return -1;
-        const int threads = v == 0 ? (use8 ? C8::THREADS : C::THREADS) : C4::THREADS;
-        dim3 grid(k2d::xcd_grid_h((int)nbl), 1, k2d::zdim(b.nmem)), block(threads);
-        if (v == 0 && use8) {
-            if constexpr (has8) {
-                if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1, 8>), grid, block, 0, s, b);
-                else hipLaunchKernelGGL((colx_kernel<T, N, false, 1, 8>), grid, block, 0, s, b);
-            }
-        } else if (v == 0) {
-            if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 1>), grid, block, 0, s, b);
-            else hipLaunchKernelGGL((colx_kernel<T, N, false, 1>), grid, block, 0, s, b);
+        dim3 grid(k2d::xcd_grid_h((int)nbl), 1, k2d::zdim(b.nmem)), block(C4::THREADS);
+        if (v == 0) {
+            if (em == 8 && cbm == 2) colx1_launch<T, N, 8, 2>(inverse, grid, b, s);
+            else if (em == 8) colx1_launch<T, N, 8, 1>(inverse, grid, b, s);
+            else if (cbm == 2) colx1_launch<T, N, 16, 2>(inverse, grid, b, s);
+            else colx1_launch<T, N, 16, 1>(inverse, grid, b, s);
         } else {
             if (inverse) hipLaunchKernelGGL((colx_kernel<T, N, true, 4>), grid, block, 0, s, b);
             else hipLaunchKernelGGL((colx_kernel<T, N, false, 4>), grid, block, 0, s, b);

@@ -202,21 +202,29 @@ def bench_ensemble(args):
 
     lib = bastr_amd.load()
     prof_api(lib)
-    rng = np.random.default_rng(2024)
-    steps = args.steps if args.steps is not None else 1000
-    groups = []
-    seed = 1
-    for n, M in ENSEMBLE:
-        p = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
-                        target_energy=0.075, lproject=True, kmax=n // 2, precision=1)
-        forced = [m % 2 for m in range(M)]
-        e = bastr_amd.ensemble(p, M, forced)
-        e.set_spectrum_law(LAW_PURE)
-        k0s = rng.integers(3, 9, size=M)
-        for m in range(M):
-            e.init_random_phase_member(m, k0=float(k0s[m]), seed=seed + m, energy=1.0)
-        groups.append(dict(n=n, M=M, ens=e, forced=forced, k0=k0s, seed0=seed, params=p))
-        seed += M
+    # with the pure law, the first member (1024^2, k0 = 7, free decay) reaches a
+    # NaN energy at iteration 929: the timed iterations stay below that, so that
+    # every member does every iteration's work
+    steps = args.steps if args.steps is not None else 600
+
+    def build_groups():
+        rng = np.random.default_rng(2024)
+        gs = []
+        seed = 1
+        for n, M in ENSEMBLE:
+            p = make_params(n, n, reynolds=5000.0, deltat=1e-4, timemethod=1, forcemethod=2, forcek=5,
+                            target_energy=0.075, lproject=True, kmax=n // 2, precision=1)
+            forced = [m % 2 for m in range(M)]
+            e = bastr_amd.ensemble(p, M, forced)
+            e.set_spectrum_law(LAW_PURE)
+            k0s = rng.integers(3, 9, size=M)
+            for m in range(M):
+                e.init_random_phase_member(m, k0=float(k0s[m]), seed=seed + m, energy=1.0)
+            gs.append(dict(n=n, M=M, ens=e, forced=forced, k0=k0s, seed0=seed, params=p))
+            seed += M
+        return gs
+
+    groups = build_groups()
     total_pts = sum(n * n * M for n, M in ENSEMBLE)
 
     # each size's handle has its own HIP stream: bastr_step_group launches the
@@ -230,7 +238,7 @@ def bench_ensemble(args):
             raise RuntimeError(f"bastr_step returned {rc} ({g['n']}^2 ensemble)")
         return sp
 
-    def advance(n_it, keep=False):
+    def advance(n_it, keep=False, groups=groups):
         out = []
         done = 0
         while done < n_it:
@@ -274,6 +282,19 @@ def bench_ensemble(args):
     _, n_l, ms_l, by_l = read_prof(lib, dg["ens"]._h)[dom_name]
     mean_s = ms_l / n_l * 1e-3
     achieved = by_l / mean_s / 1e9
+    # the same iterations again with no events: bastr_step_group then replays all
+    # three sizes' iterations as one captured graph per call
+    graph_ms = None
+    if not args.serial:
+        # fresh members (the first set has evolved for the timed iterations)
+        ggroups = build_groups()
+        gsteps = min(steps, args.chunk)
+        advance(gsteps, groups=ggroups)  # captures the group graph of the call length
+        tg = time.perf_counter()
+        advance(steps, groups=ggroups)
+        graph_ms = (time.perf_counter() - tg) / steps * 1e3
+        for g in ggroups:
+            g["ens"].close()
     copy_gbps = C.c_double()
     if lib.bastr_prof_copy_gbps(groups[0]["ens"]._h, 1 << 30, 10, C.byref(copy_gbps)) != 0:
         copy_gbps.value = float("nan")
@@ -353,6 +374,9 @@ def bench_ensemble(args):
                      "frac_of_copy": round(achieved / copy_gbps.value, 4)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "graph_replay": None if graph_ms is None else {
+            "ms_per_step": graph_ms, "value": total_pts / (graph_ms * 1e-3),
+            "launch": "one captured graph over the three sizes per bastr_step_group call"},
     }
     print(json.dumps(out))
 
@@ -627,6 +651,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="2d-2048: time the headline workload only")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default radix plans (no autotuning)")
     ap.add_argument("--serial", action="store_true", help="2d-ensemble96: step the size groups one after another")
     ap.add_argument("--copy-only", type=int, default=None, metavar="MIB",
                     help="only run the library's device copy (bastr_prof_copy_gbps) of MIB MiB, 5 times: the "
@@ -664,6 +689,24 @@ def main():
     dims = "x".join(map(str, grid))
     lib = bastr_amd.load()
     prof_api(lib)
+    # radix plans of the column passes picked by the autotuner for every column
+    # length this run uses (the table then serves all the timed workloads)
+    tune = {}
+    if not args.no_tune:
+        names = [name] + ([leg for leg, _ in LEGS] if name == "2d-2048" and not args.no_legs else [])
+        for nm in names:
+            cg = CONFIGS[nm]
+            n_col = cg["grid"][1]
+            key = f"{n_col} {'fp32' if cg['prec'] == 1 else 'fp64'}"
+            if key not in tune:
+                # the workload's pass shape: its kept columns (cut rounded up to 16), and
+                # 64 of a 3-D field's planes
+                ncols = (cg["grid"][0] // 3 + 15) // 16 * 16
+                (pts, wid), ms = bastr_amd.autotune(n_col, cg["prec"], ncols, 64 if len(cg["grid"]) == 3 else 1)
+                tune[key] = {"points_per_thread": pts, "width": wid,
+                             "ms": {f"{p}x{w}": (None if m != m else round(m, 4))
+                                    for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)}}
+        print(f"  autotune: {tune}", file=sys.stderr, flush=True)
 
     rec, last = measure(lib, bastr_amd, name, steps, args.warmup, args.chunk, copy=True, graph=True)
     finals = {name: last}
@@ -739,6 +782,7 @@ def main():
         "cpu_baseline": cpu,
         "step_roofline": rec["step_roofline"],
         "graph_replay": rec.get("graph_replay"),
+        "autotune": tune or None,
         "accuracy": accuracy,
     }
     if legs:

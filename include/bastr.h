@@ -209,24 +209,33 @@ int bastr_pp_spectra(bastr_handle* h, const double* u1, const double* u2, const 
 int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const double* u3, double* out);
 
 /*
- * Radix plans of the column (y, z) passes; no counterpart in the reference,
- * whose transforms are planned by FFTW_MEASURE (fftwlink.F90:284-303).  The
+ * Plans of the column (y, z) passes; no counterpart in the reference, whose
+ * transforms are planned by FFTW_MEASURE (fftwlink.F90:284-303).  The
  * single-term column passes of length n run 16 points per thread (radix-16
- * passes) unless a plan of 8 points per thread (radix-8 passes) is set for
- * (n, precision); the table is process-wide and applies to every handle's
- * later steps (captured step graphs are remade).
- * bastr_autotune times both plans of length n on the current device (a
- * representative pass; ms[0] the 16-point plan, ms[1] the 8-point one, NaN
- * where it does not exist: n < 256, or a plan too wide for one workgroup) and
- * keeps the faster; *points receives it.  Setting BASTR_TUNE=1 in the
+ * passes) on blocks of the default width unless another plan is set for
+ * (n, precision): 8 points per thread (radix-8 passes) and/or blocks of twice
+ * the width (longer row segments; where the ncols of a pass is not a whole
+ * number of wide blocks it keeps the default width).  The table is
+ * process-wide and applies to every handle's later steps (captured step graphs
+ * are remade).
+ * bastr_autotune times the plans of length n on the current device in the
+ * shape of a real pass (ncols columns, a multiple of 16, of each of nplanes
+ * planes, more planes where that is under 768 MiB per buffer so that the pass
+ * streams from HBM; four jobs, inverse and forward) and keeps the fastest
+ * (the default plan unless another is at least 3 % faster);
+ * ms[0..3]: the plans (points, width) = (16, 1), (8, 1), (16, 2), (8, 2), NaN
+ * where a plan does not exist (8 points below n = 256; a plan needing more than
+ * 1024 threads or 160 KiB of LDS per workgroup).  Setting BASTR_TUNE=1 in the
  * environment makes bastr_create tune each column length of the handle once
- * per process.  bastr_tune_set fixes a plan, bastr_tune_get returns the one in
- * use.  The plans differ in rounding only (every result stays within the
- * parity tolerances); a fixed table gives reproducible bits.
+ * per process, in the handle's own pass shape.  bastr_tune_set fixes a plan,
+ * bastr_tune_get reads the one in use.  The 8-point plans differ in rounding
+ * (within the parity tolerances); the block width changes no arithmetic.  A
+ * fixed table gives reproducible bits.
  */
-int bastr_autotune(int32_t n, int32_t precision, int32_t* points, double* ms);
-int bastr_tune_set(int32_t n, int32_t precision, int32_t points);
-int bastr_tune_get(int32_t n, int32_t precision);
+int bastr_autotune(int32_t n, int32_t precision, int32_t ncols, int32_t nplanes, int32_t* points, int32_t* width,
+                   double* ms);
+int bastr_tune_set(int32_t n, int32_t precision, int32_t points, int32_t width);
+int bastr_tune_get(int32_t n, int32_t precision, int32_t* points, int32_t* width);
 
 /* Number of modes per shell 1..allkmax (Ecount, solution.F90:976). */
 int bastr_shell_counts(const bastr_handle* h, int64_t* counts);

@@ -27,17 +27,25 @@ def default_plans():
             bastr_amd.tune_set(n, 16, prec)
 
 
-@pytest.mark.parametrize("shape,prec", [((256, 256), 0), ((512, 2048), 0), ((512, 512), 1)])
-def test_both_plans_match_the_oracle(hip, orc, shape, prec):
-    print(f"case radix plans {shape} prec={prec}", flush=True)
+PLANS = [(16, 1), (8, 1), (16, 2), (8, 2)]
+
+
+@pytest.mark.parametrize("shape,prec", [((256, 256), 0), ((512, 2048), 0), ((512, 512), 1), ((512, 2048), 1)])
+def test_every_plan_matches_the_oracle(hip, orc, shape, prec):
+    print(f"case column plans {shape} prec={prec}", flush=True)
     orc.set_threads(16)
     n1 = shape[1]
     p = make_params(*shape, forcemethod=2, lproject=True, kmax=128, precision=prec)
     u = mixed_field(shape, seed=31)
     res = []
-    for pts in (16, 8):
-        bastr_amd.tune_set(n1, pts, prec)
-        assert bastr_amd.tune_get(n1, prec) == pts
+    plans = []
+    for pts, wid in PLANS:
+        try:
+            bastr_amd.tune_set(n1, pts, prec, wid)
+        except Exception:
+            continue  # not a plan of this length and precision
+        plans.append((pts, wid))
+        assert bastr_amd.tune_get(n1, prec) == (pts, wid)
         g = Solver(hip, "bastr", p)
         g.set_fields(*u)
         rg, dg, sg = g.step(4)
@@ -52,8 +60,12 @@ def test_both_plans_match_the_oracle(hip, orc, shape, prec):
         assert rel_l2(f, o.get_fields()) < tol
         if prec == 0:
             check_diags(dg, do)
-    # the two plans differ in rounding only
-    assert rel_l2(res[0][0], res[1][0]) < (1e-13 if prec == 0 else 1e-5)
+    assert len(plans) >= 2
+    for (pa, wa), r in zip(plans, res):
+        # the 8-point plans differ in rounding only; the width changes nothing
+        if pa == 16:
+            assert all(np.array_equal(x, y) for x, y in zip(r[0], res[0][0])) if wa == 2 else True
+        assert rel_l2(r[0], res[0][0]) < (1e-13 if prec == 0 else 1e-5)
 
 
 def test_plan_change_remakes_captured_graphs(hip):
@@ -78,16 +90,15 @@ def test_plan_change_remakes_captured_graphs(hip):
 
 
 @pytest.mark.parametrize("n,prec", [(2048, 0), (512, 1)])
-def test_autotune_keeps_the_faster_plan(hip, n, prec):
+def test_autotune_keeps_the_fastest_plan(hip, n, prec):
     print(f"case autotune {n} prec={prec}", flush=True)
-    pts, ms = bastr_amd.autotune(n, prec)
-    print("autotune", n, prec, pts, ms, flush=True)
-    assert pts in (8, 16) and bastr_amd.tune_get(n, prec) == pts
+    plan, ms = bastr_amd.autotune(n, prec)
+    print("autotune", n, prec, plan, ms, flush=True)
+    assert bastr_amd.tune_get(n, prec) == plan
     assert np.isfinite(ms[0]) and ms[0] > 0
-    if np.isfinite(ms[1]):
-        assert pts == (8 if ms[1] < ms[0] else 16)
-    else:
-        assert pts == 16
+    times = {p: t for p, t in zip(PLANS, ms) if np.isfinite(t)}
+    fast = {p: t for p, t in times.items() if t < 0.97 * times[(16, 1)]}
+    assert plan == (min(fast, key=fast.get) if fast else (16, 1))
 
 
 def test_group_graph_with_a_replaced_handle(hip):
