@@ -150,34 +150,6 @@ struct LdsLine<T, true, WAVE> {
     __device__ __forceinline__ T& at(int e) const { return base[lds_slot(e) * stride]; }
 };
 
-// Exchange: values v[r] written at positions widx(r), then read back so that
-// out[q] = line[ridx(q)].  All threads of the workgroup must call it.
-template <typename T, int NV, bool WAVE, class W, class Rd>
-__device__ __forceinline__ void exchange(const LdsLine<T, false, WAVE>& L, cx<T>* v, W widx, Rd ridx) {
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) L.st(widx(r), v[r]);
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) v[r] = L.ld(ridx(r));
-}
-
-template <typename T, int NV, bool WAVE, class W, class Rd>
-__device__ __forceinline__ void exchange(const LdsLine<T, true, WAVE>& L, cx<T>* v, W widx, Rd ridx) {
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].x;
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) v[r].x = L.at(ridx(r));
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) L.at(widx(r)) = v[r].y;
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int r = 0; r < NV; ++r) v[r].y = L.at(ridx(r));
-}
-
 // twiddle v[r] *= w_N^{(b mod Ns) * r * N/(Ns*R)}, r = 1..R-1.  w^r is the
 // running product with w, refreshed from the table at every 4th power, so
 // each factor is at most three roundings away from a table value while only
@@ -202,6 +174,84 @@ __device__ __forceinline__ void twiddle(cx<T>* v, int b, int Ns, const cx<T>* __
     }
 }
 
+// Exchanges with one address register per direction.  An exchange writes
+// elements b + d(r) and reads c + d'(r), with b, c known per thread and the
+// d(r) known at compile time.  With slot(e) = e + (e >> s) (s = 3 complex,
+// 4 split), slot(b + d) = slot(b) + d + (d >> s) whenever
+// (b & (2^s - 1)) + (d & (2^s - 1)) < 2^s: then every access of the exchange
+// is one base address plus an immediate offset.  lin_write/lin_read check that
+// condition at compile time over every thread of the line; where it fails the
+// exchange computes each slot.
+constexpr int lin_off(int d, int s) { return d + (d >> s); }
+constexpr bool lin_fits(int b, int d, int s) { return (b & ((1 << s) - 1)) + (d & ((1 << s) - 1)) < (1 << s); }
+// Stockham write of a pass with Ns = ns, radix R: b = (j/ns)*ns*R + j%ns, d = r*ns
+constexpr bool lin_write(int P, int ns, int R, int s) {
+    for (int j = 0; j < P; ++j)
+        for (int r = 0; r < R; ++r)
+            if (!lin_fits((j / ns) * ns * R + (j & (ns - 1)), r * ns, s)) return false;
+    return true;
+}
+// read of element j + d(q): d = q*P (full passes), or (q/Lr)*P + (q%Lr)*(N/Lr)
+// ahead of a partial last pass of radix Lr
+constexpr bool lin_read(int N, int P, int E, int Lr, int s) {
+    for (int j = 0; j < P; ++j)
+        for (int q = 0; q < E; ++q)
+            if (!lin_fits(j, Lr == 1 ? q * P : (q / Lr) * P + (q % Lr) * (N / Lr), s)) return false;
+    return true;
+}
+
+// exchange of pass-p data: writes b + r*NS, reads j + d(q) as above
+template <typename T, int N, int E, int NS, int LR, bool WAVE>
+__device__ __forceinline__ void exchange_pass(const LdsLine<T, false, WAVE>& L, cx<T>* v, int b, int j) {
+    constexpr int P = N / E, S = 3;
+    constexpr bool LW = lin_write(P, NS, E, S), LRD = lin_read(N, P, E, LR, S);
+    auto rd = [](int q) { return LR == 1 ? q * P : (q / LR) * P + (q % LR) * (N / LR); };
+    lds_sync<WAVE>();
+    if constexpr (LW) {
+        cx<T>* const w = L.base + lds_slot_cx(b) * L.stride;
+#pragma unroll
+        for (int r = 0; r < E; ++r) w[lin_off(r * NS, S) * L.stride] = v[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) L.st(b + r * NS, v[r]);
+    }
+    lds_sync<WAVE>();
+    if constexpr (LRD) {
+        const cx<T>* const q0 = L.base + lds_slot_cx(j) * L.stride;
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = q0[lin_off(rd(q), S) * L.stride];
+    } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = L.ld(j + rd(q));
+    }
+}
+
+template <typename T, int N, int E, int NS, int LR, bool WAVE>
+__device__ __forceinline__ void exchange_pass(const LdsLine<T, true, WAVE>& L, cx<T>* v, int b, int j) {
+    constexpr int P = N / E, S = 4;
+    constexpr bool LW = lin_write(P, NS, E, S), LRD = lin_read(N, P, E, LR, S);
+    auto rd = [](int q) { return LR == 1 ? q * P : (q / LR) * P + (q % LR) * (N / LR); };
+    T* const w = L.base + lds_slot(b) * L.stride;
+    T* const q0 = L.base + lds_slot(j) * L.stride;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const T x = h ? v[r].y : v[r].x;
+            if constexpr (LW) w[lin_off(r * NS, S) * L.stride] = x;
+            else L.at(b + r * NS) = x;
+        }
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const T x = LRD ? q0[lin_off(rd(q), S) * L.stride] : L.at(j + rd(q));
+            if (h) v[q].y = x;
+            else v[q].x = x;
+        }
+    }
+}
+
 template <int N, int E>
 struct FftShape {
     static_assert((N & (N - 1)) == 0 && N >= 4, "N must be a power of two");
@@ -212,6 +262,24 @@ struct FftShape {
     static_assert(NFULL >= 1, "N must be >= E");
 };
 
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// Full radix-E passes p, p+1, ... of a line transform (Ns = E^p), each
+// followed by its exchange unless it is the last pass.
+template <typename T, int N, int E, bool INV, bool SPLIT, bool WAVE, int p>
+__device__ __forceinline__ void fft_passes(cx<T>* v, int j, const LdsLine<T, SPLIT, WAVE>& L,
+                                           const cx<T>* __restrict__ tw) {
+    using S = FftShape<N, E>;
+    constexpr int NS = ipow(E, p);
+    twiddle<T, N, E, INV>(v, j, NS, tw);
+    dft<T, E, INV>(v);
+    if constexpr (p < S::NFULL - 1 || S::LAST > 1) {
+        const int b = (j / NS) * NS * E + (j & (NS - 1));
+        exchange_pass<T, N, E, NS, (p < S::NFULL - 1 ? 1 : S::LAST)>(L, v, b, j);
+    }
+    if constexpr (p + 1 < S::NFULL) fft_passes<T, N, E, INV, SPLIT, WAVE, p + 1>(v, j, L, tw);
+}
+
 // Full line transform.  On entry v[k] = x[j + k*P]; on exit v[k] = X[j + k*P].
 // Unnormalised in both directions.
 template <typename T, int N, int E, bool INV, bool SPLIT, bool WAVE>
@@ -219,26 +287,9 @@ __device__ __forceinline__ void line_fft(cx<T>* v, int j, const LdsLine<T, SPLIT
                                          const cx<T>* __restrict__ tw) {
     using S = FftShape<N, E>;
     constexpr int P = S::P;
-    int Ns = 1;
-#pragma unroll
-    for (int p = 0; p < S::NFULL; ++p) {
-        twiddle<T, N, E, INV>(v, j, Ns, tw);
-        dft<T, E, INV>(v);
-        const bool last = (p == S::NFULL - 1) && (S::LAST == 1);
-        if (!last) {
-            const int base = (j / Ns) * Ns * E + (j & (Ns - 1));
-            const int ns = Ns;
-            if (p < S::NFULL - 1) {
-                exchange<T, E>(L, v, [&](int r) { return base + r * ns; }, [&](int r) { return j + r * P; });
-            } else {
-                constexpr int Lr = S::LAST;
-                exchange<T, E>(L, v, [&](int r) { return base + r * ns; },
-                               [&](int q) { return j + (q / Lr) * P + (q % Lr) * (N / Lr); });
-            }
-        }
-        Ns *= E;
-    }
+    fft_passes<T, N, E, INV, SPLIT, WAVE, 0>(v, j, L, tw);
     if constexpr (S::LAST > 1) {
+        constexpr int Ns = ipow(E, S::NFULL);
         constexpr int Lr = S::LAST;
         constexpr int Q = E / Lr;
 #pragma unroll

@@ -58,8 +58,10 @@ struct RowCfg {
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
-    // waves per SIMD asked of the compiler: FP32 stage passes fit 96 registers (5 waves)
-    static constexpr int WPE = sizeof(T) == 4 ? 5 : 1;
+    // waves per SIMD asked of the compiler: FP32 stage passes fit 96 registers (5
+    // waves); FP64 ones 128 (4 waves, measured: 2048^2 row stage 54.5 -> 49.9 us
+    // from 3), but for 1024-point rows, which would spill there
+    static constexpr int WPE = sizeof(T) == 4 ? 5 : (N == 1024 ? 1 : 4);
 };
 
 // Row-pass shape of the velocity-gradient kernels (8 points per thread, FP32

@@ -55,8 +55,17 @@ __device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko, T kc, T kco) {
 template <typename T, int N, int MT, int EM = 16, int CBM = 1>
 using ColxCfg = ColCfg<T, N, MT == 1 ? EM : (sizeof(T) == 4 ? 16 : 8), MT == 1 ? CBM : 1>;
 
+// waves per SIMD asked of the compiler: a workgroup of W waves per SIMD
+// (THREADS / 256) only ever adds W, so ask for the largest multiple of W up to
+// `want` (asking 3 of a 2-wave workgroup only lowers the register budget)
+constexpr int wpe_for(int threads, int want) {
+    const int w = threads <= 256 ? 1 : threads / 256;
+    return w >= want ? w : want / w * w;
+}
+
 template <typename T, int N, bool INV, int MT, int EM = 16, int CBM = 1>
-__global__ __launch_bounds__((ColxCfg<T, N, MT, EM, CBM>::THREADS)) __attribute__((amdgpu_waves_per_eu(MT == 1 ? 3 : 2)))
+__global__ __launch_bounds__((ColxCfg<T, N, MT, EM, CBM>::THREADS))
+__attribute__((amdgpu_waves_per_eu(wpe_for(ColxCfg<T, N, MT, EM, CBM>::THREADS, MT == 1 ? 3 : 2))))
 void colx_kernel(ColXArgs<T> a) {
     using C = ColxCfg<T, N, MT, EM, CBM>;
     static_assert(C::FITS, "column plan exceeds a workgroup");
@@ -578,7 +587,9 @@ void vg3_kernel(Vg3Args<T> a) {
     };
     cx<T> z[E], w[E];
     T u3[E], b11[E];
-    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, eta = eta0;
+    // eta_min = min sqrt(nu / |div|) = sqrt(nu / max |div|): both functions are
+    // monotone, so one division and one square root per thread give the same value
+    double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, dmax = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k) z[k] = pack_rows<T, N>(W(0), W(1), j + k * P);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
@@ -628,10 +639,11 @@ void vg3_kernel(Vg3Args<T> a) {
         s_ijji += (b11k * b11k + b22 * b22 + b33 * b33) * uu;
         s_u2t += div * uu;
         s_eps += nu * div * div;
-        eta = fmin(eta, sqrt(nu / fabs(div)));
+        dmax = fmax(dmax, fabs(div));
         s_urms += u1 * u1 + u2 * u2 + uu3 * uu3;
         s_dudx += b11k * b11k + b22 * b22 + b33 * b33;
     }
+    double eta = fmin(eta0, sqrt(nu / dmax));
     if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
     double v0[3] = {s_um2, s_ijji, s_u2t};
     double v1[4] = {s_urms, s_dudx, s_eps, eta};
