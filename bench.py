@@ -21,12 +21,13 @@ in chunks of 200 iterations (the spectra cadence of the configuration) and
 returns the per-iteration diagnostics to the host, as a log-writing driver
 would.
 
-Prints one JSON line.  `value` = grid points x steps / wall seconds.
-`roofline` is the dominant kernel class: its compulsory HBM bytes per launch
-over its mean duration, timed with HIP events on the library's stream during
-the timed region (whose step calls therefore launch directly);
-`graph_replay` times the same number of iterations again with no events, the
-step calls then replayed as captured HIP graphs.  `step_roofline` prices the whole step at the reference
+Prints one JSON line.  `value` = grid points x steps / wall seconds of the
+timed run, whose step calls replay their captured HIP graphs (what a caller
+gets).  `roofline` is the dominant kernel class: its compulsory HBM bytes per
+launch over its mean duration, timed with HIP events on the library's stream
+around each of its launches in a run of as many iterations from the same
+field (events make those step calls launch directly; `direct_launch` is that
+run's throughput).  `step_roofline` prices the whole step at the reference
 algorithm's transform count (one compulsory real round trip, 2*s*N^d bytes,
 per whole-field transform).  `cpu_baseline` is the FP64 CPU restatement
 (oracle/) timed on this host's cores over a bounded sample of the same
@@ -268,33 +269,29 @@ def bench_ensemble(args):
             if n_l and (best is None or ms > best[3]):
                 best = (g, name, c, ms)
     dg, dom_name, dom_cls, _ = best
-    # one untimed unprofiled chunk per size, one after another: captures each
-    # handle's iteration graph for the chunk length before the timed region
-    for g in groups:
-        lib.bastr_prof_enable(g["ens"]._h, 0)
-        step_group(g, min(args.chunk, steps))
+    # the profiled run: HIP events around the dominant class's launches (its
+    # handle then launches directly), as many iterations as the timed run
     for g in groups:
         lib.bastr_prof_enable(g["ens"]._h, (1 << dom_cls) if g is dg else 0)
     t0 = time.perf_counter()
     advance(steps)
-    wall = time.perf_counter() - t0
-    print(f"  timed {steps} ensemble steps in {wall:.2f} s", file=sys.stderr, flush=True)
+    wall_direct = time.perf_counter() - t0
     _, n_l, ms_l, by_l = read_prof(lib, dg["ens"]._h)[dom_name]
     mean_s = ms_l / n_l * 1e-3
     achieved = by_l / mean_s / 1e9
-    # the same iterations again with no events: bastr_step_group then replays all
-    # three sizes' iterations as one captured graph per call
-    graph_ms = None
-    if not args.serial:
-        # fresh members (the first set has evolved for the timed iterations)
-        ggroups = build_groups()
-        gsteps = min(steps, args.chunk)
-        advance(gsteps, groups=ggroups)  # captures the group graph of the call length
-        tg = time.perf_counter()
-        advance(steps, groups=ggroups)
-        graph_ms = (time.perf_counter() - tg) / steps * 1e3
-        for g in ggroups:
-            g["ens"].close()
+    # the timed run, as a caller gets it: fresh members (the first set has
+    # evolved), no events, so bastr_step_group replays all three sizes'
+    # iterations as one captured graph per call (--serial: each handle's graph)
+    ggroups = build_groups()
+    gsteps = min(steps, args.chunk)
+    advance(gsteps, groups=ggroups)  # captures the graphs of the call length
+    tg = time.perf_counter()
+    advance(steps, groups=ggroups)
+    wall = time.perf_counter() - tg
+    for g in ggroups:
+        g["ens"].close()
+    print(f"  timed {steps} ensemble steps in {wall:.2f} s (graph replay); profiled run {wall_direct:.2f} s",
+          file=sys.stderr, flush=True)
     copy_gbps = C.c_double()
     if lib.bastr_prof_copy_gbps(groups[0]["ens"]._h, 1 << 30, 10, C.byref(copy_gbps)) != 0:
         copy_gbps.value = float("nan")
@@ -365,8 +362,10 @@ def bench_ensemble(args):
                                "other member, free decay on the rest, lproject=t, spectra kmax=n/2",
                    "members": [list(x) for x in ENSEMBLE], "precision": "fp32",
                    "launch": "one handle per size (every member of a size in each launch), the three on their "
-                             "own streams, " + ("stepped one after another" if args.serial else
-                                               "stepped together by bastr_step_group")},
+                             "own streams, " + ("stepped one after another, each call replaying its handle's "
+                                               "graph" if args.serial else
+                                               "stepped together by bastr_step_group: one captured graph over the "
+                                               "three sizes per call")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
                      "ensemble_size": dg["n"], "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
@@ -374,9 +373,8 @@ def bench_ensemble(args):
                      "frac_of_copy": round(achieved / copy_gbps.value, 4)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
-        "graph_replay": None if graph_ms is None else {
-            "ms_per_step": graph_ms, "value": total_pts / (graph_ms * 1e-3),
-            "launch": "one captured graph over the three sizes per bastr_step_group call"},
+        "direct_launch": {"ms_per_step": wall_direct / steps * 1e3, "value": total_pts * steps / wall_direct,
+                          "launch": "the profiled run: the dominant size's handle launches directly"},
     }
     print(json.dumps(out))
 
@@ -557,15 +555,18 @@ def new_field(bastr_amd, params):
     return s
 
 
-def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False, graph=False):
-    """Time `steps` iterations of workload `name` after `warmup` untimed ones.
-    The timed region brackets the dominant kernel class (the largest share of
-    the warm-up, all classes timed) with HIP events on the library's stream;
-    `graph`: afterwards time as many iterations again with no events, which
-    replays the step call as a captured HIP graph (the path a caller gets).
-    Returns the record and the last iteration's spectrum."""
+def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
+    """Time `steps` iterations of workload `name` after `warmup` untimed ones,
+    as a caller runs them: each bastr_step call (a chunk of iterations) replays
+    its captured HIP graph.  Profiling events would force direct launches, so
+    the dominant kernel class (the largest share of a warm-up with every class
+    timed) is timed in a second run of as many iterations from the same initial
+    field: HIP events on the library's stream around each of its launches.
+    That run's wall time is reported too (direct launches).  Returns the record
+    and the last timed iteration's spectrum."""
     cfg = CONFIGS[name]
     grid, prec = cfg["grid"], cfg["prec"]
+    # the profiled run (direct launches)
     s = new_field(bastr_amd, cfg_params(name))
     lib.bastr_prof_enable(s._h, (1 << NCLASS) - 1)
     run_steps(s, max(warmup, 1), chunk)
@@ -578,9 +579,8 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False, graph=False)
                   file=sys.stderr, flush=True)
     lib.bastr_prof_enable(s._h, 1 << dom_cls)
     t0 = time.perf_counter()
-    last = run_steps(s, steps, chunk)
-    wall = time.perf_counter() - t0
-    print(f"  [{name}] timed {steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    run_steps(s, steps, chunk)
+    wall_direct = time.perf_counter() - t0
     prof = read_prof(lib, s._h)
     _, n_l, ms_l, by_l = prof[dom_name]
     mean_s = ms_l / n_l * 1e-3
@@ -591,17 +591,18 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False, graph=False)
         if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(cg)) == 0:
             copy_gbps = cg.value
     s.close()
-    graph_ms = None
-    if graph:
-        # a second run from the same initial field (the workload's field blows up
-        # after about 1860 iterations at 2048^2), unprofiled: the warm-up's last
-        # call captures the graph of the chunk length the timed calls replay
-        g = new_field(bastr_amd, cfg_params(name))
-        run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
-        t1 = time.perf_counter()
-        run_steps(g, steps, chunk)
-        graph_ms = (time.perf_counter() - t1) / steps * 1e3
-        g.close()
+    # the timed run: unprofiled, so every call replays its graph (the warm-up's
+    # last call captures the graph of the chunk length the timed calls replay;
+    # a fresh field, since the workload's field blows up after about 1860
+    # iterations at 2048^2)
+    g = new_field(bastr_amd, cfg_params(name))
+    run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
+    t1 = time.perf_counter()
+    last = run_steps(g, steps, chunk)
+    wall = time.perf_counter() - t1
+    g.close()
+    print(f"  [{name}] timed {steps} steps in {wall:.3f} s (graph replay); profiled run {wall_direct:.3f} s",
+          file=sys.stderr, flush=True)
     npts = 1
     for n in grid:
         npts *= n
@@ -610,20 +611,22 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False, graph=False)
     rec = {
         "value": npts * steps / wall, "unit": "grid-pt·steps/s", "steps": steps, "warmup": warmup,
         "ms_per_step": wall / steps * 1e3, "dtype": "f32" if prec == 1 else "f64",
+        "launch": "graph replay (each bastr_step call of up to 200 iterations replays its captured HIP graph)",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile(dom_name, prec, grid), "kernel": dom_name,
-                     "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2)},
+                     "bytes_per_launch": by_l, "mean_launch_us": round(mean_s * 1e6, 2),
+                     "timing": f"HIP events around each of the class's {n_l} launches in a {steps}-iteration "
+                               "run of the same workload from the same field (direct launches)"},
         "step_roofline": {"reference_transforms": T, "bytes_per_step": b_step,
                           "achieved": round(b_step * steps / wall / 1e9, 1), "unit": "GB/s",
                           "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
+        "direct_launch": {"ms_per_step": wall_direct / steps * 1e3, "value": npts * steps / wall_direct},
     }
     if copy:
         rec["roofline"]["copy_measured"] = round(copy_gbps, 1)
         rec["roofline"]["frac_of_copy"] = round(achieved / copy_gbps, 4)
         rec["step_roofline"]["frac_of_copy"] = round(b_step * steps / wall / 1e9 / copy_gbps, 4)
-    if graph_ms is not None:
-        rec["graph_replay"] = {"ms_per_step": graph_ms, "value": npts / (graph_ms * 1e-3)}
     return rec, last
 
 
@@ -708,7 +711,7 @@ def main():
                                     for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)}}
         print(f"  autotune: {tune}", file=sys.stderr, flush=True)
 
-    rec, last = measure(lib, bastr_amd, name, steps, args.warmup, args.chunk, copy=True, graph=True)
+    rec, last = measure(lib, bastr_amd, name, steps, args.warmup, args.chunk, copy=True)
     finals = {name: last}
     accuracy = {}
     legs = {}
@@ -781,7 +784,8 @@ def main():
         "roofline": rec["roofline"],
         "cpu_baseline": cpu,
         "step_roofline": rec["step_roofline"],
-        "graph_replay": rec.get("graph_replay"),
+        "launch": rec["launch"],
+        "direct_launch": rec["direct_launch"],
         "autotune": tune or None,
         "accuracy": accuracy,
     }
