@@ -380,6 +380,49 @@ def bench_ensemble(args):
 
 
 # --config 2d-closure: the learned subgrid closure (include/bastr_closure.h)
+CLOSURE_FWD_FLOPS = 13056.0  # 2 (36*64 + 64*64 + 64*2) per point
+MFMA_BF16_PEAK = 2500.0      # dense BF16 MFMA, TFLOP/s
+
+
+def closure_kernels(lib, args):
+    """--closure-kernels: the network's kernels alone at the shapes of the
+    closure workload (training batches of 2048 tiles of 32 points, inference
+    over a 256^2 grid), on pairs filtered from one short 512^2 run (their
+    values do not change the kernels' work): bastr_prof_closure's event
+    timings against the BF16 MFMA peak, then 150 closure-run iterations."""
+    import bastr_amd
+    from bastr_amd.closure import Closure, filter_to, fine_run, make_pairs
+    from bastr_amd.solver import make_params
+
+    n = 256
+    fp = make_params(512, 512, reynolds=5000.0, deltat=1e-4, forcemethod=2, forcek=5, target_energy=0.075,
+                     lproject=True, kmax=128)
+    cp = make_params(n, n, reynolds=5000.0, deltat=4e-4, forcemethod=2, forcek=5, target_energy=0.075,
+                     lproject=True, kmax=n // 3)
+    ps = make_pairs(fp, n, [1], [200, 300], k0=4.0)
+    net = Closure(seed=1)
+    net.fit_scales(ps)
+    net.train(ps, iters=200, batch=256, lr=2e-3, seed=3)
+    tr_ms, fw_ms = C.c_double(), C.c_double()
+    if lib.bastr_prof_closure(net._h, ps._h, 20, 2048, C.byref(tr_ms), C.byref(fw_ms)) != 0:
+        raise RuntimeError("bastr_prof_closure failed")
+    f = fine_run(fp, 2, 4.0)
+    f.step(200, want_spectra=False)
+    c = bastr_amd.solver(cp)
+    filter_to(f, c)
+    net.attach(c)
+    rc = c.step(150, want_spectra=False)[0]
+    c.close()
+    f.close()
+    train_pts = 2048 * 32
+    train_tf = 3 * CLOSURE_FWD_FLOPS * train_pts / (tr_ms.value * 1e-3) / 1e12
+    fwd_tf = CLOSURE_FWD_FLOPS * n * n / (fw_ms.value * 1e-3) / 1e12
+    print(json.dumps({"closure_kernels": {
+        "train_kernel": {"points": train_pts, "mean_launch_us": round(tr_ms.value * 1e3, 2),
+                         "tflops": round(train_tf, 2), "frac_of_bf16_peak": round(train_tf / MFMA_BF16_PEAK, 5)},
+        "forward_kernel": {"points": n * n, "mean_launch_us": round(fw_ms.value * 1e3, 2),
+                           "tflops": round(fwd_tf, 2), "frac_of_bf16_peak": round(fwd_tf / MFMA_BF16_PEAK, 5)},
+        "closure_run_status": rc}}))
 def closure_prof_api(lib):
     lib.bastr_prof_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
@@ -390,10 +433,10 @@ def bench_closure(args):
     """Coarse 2-D 256^2 solve (dt = 4e-4) with the per-point MLP closure,
     trained on pairs from 2048^2 fine runs of this solver (FP64, Re = 5000,
     dt = 1e-4, band forcing k = 5 E_t = 0.075, projection; pure-law fields k0 = 4)
-    at 20 seeds (snapshots after 1000, 1150, 1300, 1450, 1600 iterations,
+    at 20 seeds (snapshots after 800, 950, 1100, 1250, 1400 iterations,
     filtered at k_c = 85; Adam at learning rates 2e-3, 7e-4, 2e-4 for half,
     a quarter and a quarter of the steps), judged on 5 held-out seeds: the
-    coarse run starts from the filtered fine field after 1000 fine iterations
+    coarse run starts from the filtered fine field after 800 fine iterations
     and runs 150 iterations (600 fine ones); E(k) for k < k_c against the
     filtered fine run, with and without the network: rel-L2 over the shells,
     and the mean and largest per-shell relative error.  value = the closure run's grid-pt.steps/s; the fine run's
@@ -407,13 +450,17 @@ def bench_closure(args):
     lib = bastr_amd.load()
     prof_api(lib)
     closure_prof_api(lib)
+    if args.closure_kernels:
+        return closure_kernels(lib, args)
     N, n = 2048, 256
     fp = make_params(N, N, reynolds=5000.0, deltat=1e-4, forcemethod=2, forcek=5, target_energy=0.075,
                      lproject=True, kmax=128)
     cp = make_params(n, n, reynolds=5000.0, deltat=4e-4, forcemethod=2, forcek=5, target_energy=0.075,
                      lproject=True, kmax=n // 3)
     train_seeds, test_seeds = list(range(1, 21)), list(range(21, 26))
-    snaps = [1000, 1150, 1300, 1450, 1600]
+    # snapshots well before the blow-up of these shocked runs (the first seeds'
+    # fields reach NaN energies between 1450 and 1900 iterations)
+    snaps = [800, 950, 1100, 1250, 1400]
     t0 = time.perf_counter()
     ps = make_pairs(fp, n, train_seeds, snaps, k0=4.0)
     t_pairs = time.perf_counter() - t0
@@ -432,7 +479,7 @@ def bench_closure(args):
     res = {"without": [], "with": []}
     for seed in test_seeds:
         for label, nn in (("without", None), ("with", net)):
-            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=1000, fine_steps=1600, net=nn, k0=4.0)
+            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=800, fine_steps=1400, net=nn, k0=4.0)
             rel = np.abs(Ec - Ef) / Ef
             res[label].append({"seed": seed, "ek_rel_l2": float(np.linalg.norm(Ec - Ef) / np.linalg.norm(Ef)),
                                "ek_err_mean_per_shell": float(rel.mean()),
@@ -443,7 +490,7 @@ def bench_closure(args):
     print(f"  [2d-closure] held-out E(k): {summ}", file=sys.stderr, flush=True)
     # throughput: the closure run (graph replay) and the fine run
     f = fine_run(fp, 21, 4.0)
-    f.step(1000, want_spectra=False)
+    f.step(800, want_spectra=False)
     c = bastr_amd.solver(cp)
     filter_to(f, c)
     net.attach(c)
@@ -463,10 +510,10 @@ def bench_closure(args):
     tr_ms, fw_ms = C.c_double(), C.c_double()
     if lib.bastr_prof_closure(net._h, ps._h, 20, 2048, C.byref(tr_ms), C.byref(fw_ms)) != 0:
         raise RuntimeError("bastr_prof_closure failed")
-    fwd_flops, train_pts = 13056.0, 2048 * 32
+    fwd_flops, train_pts = CLOSURE_FWD_FLOPS, 2048 * 32
     train_tf = 3 * fwd_flops * train_pts / (tr_ms.value * 1e-3) / 1e12
     fwd_tf = fwd_flops * n * n / (fw_ms.value * 1e-3) / 1e12
-    peak = 2500.0  # dense BF16 MFMA, TFLOP/s
+    peak = MFMA_BF16_PEAK
     cpu = None
     if not args.no_cpu:
         # the oracle's coarse run with the same network in compute_ut
@@ -474,7 +521,7 @@ def bench_closure(args):
         threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1), 16)
         oracle.set_threads(threads)
         g0 = fine_run(fp, 21, 4.0)
-        g0.step(1000, want_spectra=False)
+        g0.step(800, want_spectra=False)
         cc = bastr_amd.solver(cp)
         filter_to(g0, cc)
         u = cc.get_fields()
@@ -656,6 +703,8 @@ def main():
     ap.add_argument("--no-legs", action="store_true", help="2d-2048: time the headline workload only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default radix plans (no autotuning)")
     ap.add_argument("--serial", action="store_true", help="2d-ensemble96: step the size groups one after another")
+    ap.add_argument("--closure-kernels", action="store_true",
+                    help="2d-closure: time only the network's kernels at the workload's shapes")
     ap.add_argument("--copy-only", type=int, default=None, metavar="MIB",
                     help="only run the library's device copy (bastr_prof_copy_gbps) of MIB MiB, 5 times: the "
                          "known-size kernel the HBM counter summaries are calibrated against")
