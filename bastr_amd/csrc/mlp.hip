@@ -429,19 +429,41 @@ void train_kernel(const Packed* __restrict__ Pk, Scales sc, Data d, TileSel sel,
     }
 }
 
+// grad[j] = sum_w part[w][j]: 32 parameters per workgroup, eight slices of the
+// wavefronts per parameter (slice s sums w = s, s + 8, ... in order, then the
+// slices are added in order: a fixed summation order), so a launch of a few
+// thousand partials is 209 workgroups of short loops instead of one long loop
+// per parameter.  Workgroup 0 also sums the partial losses (in FP64, 256
+// strided runs added in a fixed tree).
+constexpr int RED_PARAMS = 32, RED_SLICES = 8;
+
 __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ part,
                                                      const float* __restrict__ part_loss, int nw,
                                                      float* __restrict__ grad, double* __restrict__ loss) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j < NPARAM) {
-        float s = 0.f;
-        for (int w = 0; w < nw; ++w) s += part[(long)w * NPARAM + j];
-        grad[j] = s;
+    __shared__ float red[RED_SLICES][RED_PARAMS];
+    __shared__ double lred[256];
+    const int t = threadIdx.x, q = t % RED_PARAMS, s = t / RED_PARAMS;
+    const int j = blockIdx.x * RED_PARAMS + q;
+    float acc = 0.f;
+    if (j < NPARAM)
+        for (int w = s; w < nw; w += RED_SLICES) acc += part[(long)w * NPARAM + j];
+    red[s][q] = acc;
+    __syncthreads();
+    if (s == 0 && j < NPARAM) {
+        float g = red[0][q];
+        for (int k = 1; k < RED_SLICES; ++k) g += red[k][q];
+        grad[j] = g;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += (double)part_loss[w];
-        *loss = s;
+    if (blockIdx.x == 0) {
+        double ls = 0.0;
+        for (int w = t; w < nw; w += 256) ls += (double)part_loss[w];
+        lred[t] = ls;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) lred[t] += lred[t + o];
+            __syncthreads();
+        }
+        if (t == 0) *loss = lred[0];
     }
 }
 
@@ -520,7 +542,8 @@ int launch_train(const Packed* net, Scales sc, Data d, TileSel sel, double npts_
 }
 
 int launch_reduce(const float* part, const float* part_loss, int nw, float* grad, double* loss, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, s, part, part_loss, nw, grad, loss);
+    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + RED_PARAMS - 1) / RED_PARAMS), dim3(256), 0, s, part, part_loss, nw,
+                       grad, loss);
     return 0;
 }
 

@@ -613,7 +613,16 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
     and the last timed iteration's spectrum."""
     cfg = CONFIGS[name]
     grid, prec = cfg["grid"], cfg["prec"]
-    # the profiled run (direct launches)
+    # the timed run: unprofiled, so every call replays its graph (the warm-up's
+    # last call captures the graph of the chunk length the timed calls replay)
+    g = new_field(bastr_amd, cfg_params(name))
+    run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
+    t1 = time.perf_counter()
+    last = run_steps(g, steps, chunk)
+    wall = time.perf_counter() - t1
+    g.close()
+    # the profiled run (direct launches) from the same initial field (a fresh
+    # field: the workload's field blows up after about 1860 iterations at 2048^2)
     s = new_field(bastr_amd, cfg_params(name))
     lib.bastr_prof_enable(s._h, (1 << NCLASS) - 1)
     run_steps(s, max(warmup, 1), chunk)
@@ -638,16 +647,6 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
         if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(cg)) == 0:
             copy_gbps = cg.value
     s.close()
-    # the timed run: unprofiled, so every call replays its graph (the warm-up's
-    # last call captures the graph of the chunk length the timed calls replay;
-    # a fresh field, since the workload's field blows up after about 1860
-    # iterations at 2048^2)
-    g = new_field(bastr_amd, cfg_params(name))
-    run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
-    t1 = time.perf_counter()
-    last = run_steps(g, steps, chunk)
-    wall = time.perf_counter() - t1
-    g.close()
     print(f"  [{name}] timed {steps} steps in {wall:.3f} s (graph replay); profiled run {wall_direct:.3f} s",
           file=sys.stderr, flush=True)
     npts = 1
