@@ -138,14 +138,12 @@ struct bastr_handle {
     size_t npts = 0, nhalf = 0;
     size_t esz = 8;  // sizeof(T)
     hipStream_t stream = nullptr;
-    // 2-D overlap: the spectra, the dissipation moment's x pass and the
-    // diagnostics reduction run on `side` beside the main chain (forked and
-    // joined with the events fork[0..2] each iteration); stage A's y-forward
-    // then writes WA, since the dissipation pass still reads W
+    // 2-D overlap: the spectra and the diagnostics reduction run on `side`
+    // beside the main chain (forked and joined with the events fork[0..2]
+    // each iteration)
     bool overlap = false;
     hipStream_t side = nullptr;
     hipEvent_t fork[3] = {};
-    void* WA[2] = {};
     void* U[3] = {};
     void* UB[3] = {};
     void* KA[3] = {};
@@ -635,14 +633,9 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
         if (k3d::cf2_full_rows<T>(h->n1)) {
             rc = run_cf2<T>(h, gstep, modes[stage], first);
         } else {
-            // the fused kernel would need narrower column blocks here: y-forward, then the
-            // update (the first stage writes WA while the dissipation pass may still read W)
+            // the fused kernel would need narrower column blocks here: y-forward, then the update
             void* nx[2] = {h->NX[0], h->NX[1]};
             void* w[2] = {h->W[0], h->W[1]};
-            if (first && h->overlap) {
-                w[0] = h->WA[0];
-                w[1] = h->WA[1];
-            }
             rc = run_yfwd2<T>(h, gstep, nx, w, 2, 1.0 / ((double)h->n0 * (double)h->n1), h->hp, h->tw1, true);
             if (!rc) rc = run_update2<T>(h, gstep, modes[stage], first, true, w);
         }
@@ -654,11 +647,10 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
 // One iteration of mainloop2D's body (solution.F90:54-97).
 // With h->overlap the diagnostics that only feed the log run on h->side:
 //   main: fork0 | col_inv_vg, velgrad (stage A's x pass) | fork1 | stage A | wait fork2 | stages B, C, forcing
-//   side: wait fork0 | spectra | wait fork1 | dissipation x pass, reduction | fork2
-// The dissipation pass reads W, which stage A leaves alone (its y-forward
-// writes WA); nothing writes u_hat before the join; a NaN found by the
-// reduction stops stages B and C and the forcing (stage A writes only its
-// scratch KA, UB), as in the one-stream order.
+//   side: wait fork0 | spectra | wait fork1 | reduction | fork2
+// Nothing writes u_hat before the join; a NaN found by the reduction stops
+// stages B and C and the forcing (stage A writes only its scratch KA, UB), as
+// in the one-stream order.
 template <typename T>
 int run_iteration(bastr_handle* h, int it) {
     const Gate gdiag{h->status, it, 0};
@@ -706,15 +698,11 @@ int run_iteration(bastr_handle* h, int it) {
     va.tw = C<T>(h->tw0);
     {
         ProfScope ps(h, BASTR_K_VELGRAD);
-        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg, 1));
+        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, h->stream, &h->nblk_vg));
     }
     if (ov) {
         HIPCHK(hipEventRecord(h->fork[1], h->stream));
         HIPCHK(hipStreamWaitEvent(h->side, h->fork[1], 0));
-    }
-    {
-        ProfScope ps(h, BASTR_K_VG_DIS, ss);
-        LAUNCHCHK(launch_velgrad<T>(h->n0, va, h->n1, ss, &h->nblk_vg, 2));
     }
     FinArgs fa{};
     fa.gate = gdiag;
@@ -1760,8 +1748,6 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
             h->ureal[c] = dalloc(h->npts * h->esz * M, &ok);
         }
         for (int q = 0; q < 7; ++q) h->W[q] = dalloc(cbytes * M, &ok);
-        if (h->overlap)
-            for (int c = 0; c < 2; ++c) h->WA[c] = dalloc(cbytes * M, &ok);
     } else {
         for (int c = 0; c < 3; ++c) {
             h->U[c] = dalloc(cbytes, &ok);
@@ -1853,8 +1839,6 @@ int bastr_destroy(bastr_handle* h) {
     group_forget(h->gen);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->side) (void)hipStreamSynchronize(h->side);
-    for (int c = 0; c < 2; ++c)
-        if (h->WA[c]) (void)hipFree(h->WA[c]);
     for (hipEvent_t e : h->fork)
         if (e) (void)hipEventDestroy(e);
     if (h->side) (void)hipStreamDestroy(h->side);
@@ -2435,8 +2419,7 @@ int bastr_prof_read(const bastr_handle* h, int c, int64_t* launches, double* tot
     if (h->nd == 2) {
         units[BASTR_K_SPECTRA] = 2 * fc * fy;
         units[BASTR_K_COL_INV_VG] = fc * (2 * fy + 7);   // y-inverse: u_hat -> 7 fields
-        units[BASTR_K_VELGRAD] = 6 * fc;                 // stage A's x pass: r4 w2
-        units[BASTR_K_VG_DIS] = 6 * fc;                  // dissipation x pass: r6
+        units[BASTR_K_VELGRAD] = 9 * fc;                 // stage A's x pass and every moment: r7 w2
         units[BASTR_K_COL_INV] = fc * (2 * fy + 4);      // y-inverse: 2 -> 4
         units[BASTR_K_ROW_STAGE] = 6 * fc;               // r4 w2
         units[BASTR_K_COL_FWD_A] = 2 * fc + 6 * fc * fy; // y-forward of 2, update r U w KA UB

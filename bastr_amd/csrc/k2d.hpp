@@ -221,9 +221,8 @@ struct ScaleArgs {
 // launchers (k2d.hip); n is the transform length along the pass
 template <typename T> int launch_update(int mode, const ColFwdArgs<T>& a, hipStream_t s);
 template <typename T> int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s);
-// which: 0 both velocity-gradient kernels, 1 stage A's (row_stage_vg_kernel), 2 the dissipation one
-template <typename T> int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk,
-                                         int which = 0);
+// velgrad_calculate2D's moments and stage A's x pass (row_stage_vg_kernel)
+template <typename T> int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk);
 template <typename T> int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s);
 int launch_finalize(const FinArgs& a, hipStream_t s);
 template <typename T> int launch_force_band(const ForceArgs<T>& a, hipStream_t s);

@@ -11,7 +11,8 @@
 //                               projection (:856-858), Kutta RK3 combination (:183-200)
 //   spectra_kernel              spectra_compute2D (:930-1016)
 //   colx (velgrad jobs) +
-//   row_stage_vg_kernel         velgrad_calculate2D (:1110-1186) fused with stage A's x pass
+//   row_stage_vg_kernel         velgrad_calculate2D (:1110-1186), all its moments, fused
+//                               with stage A's x pass
 //   force_band_kernel           forcing2D methods 1, 2, 5 (:303-540)
 #pragma once
 #include <climits>
@@ -182,6 +183,14 @@ template <typename T, int N, typename R>
 __device__ __forceinline__ cx<T> pack_rows(const R& A, const R& B, int m) {
     const int h = m <= N / 2 ? m : N - m;
     return pack_vals<T, N>(A[h], B[h], m);
+}
+
+// a wave-uniform pointer the compiler cannot see through: loads from it are
+// not merged with earlier loads of the same addresses
+template <typename P>
+__device__ __forceinline__ P launder(P p) {
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
 // the row index of a line, made wave-uniform (SGPR) when the line spans whole wavefronts
@@ -430,12 +439,16 @@ __device__ __forceinline__ void vg_reduce(double* vals, double (*red)[8], int q0
 // Stage A of RK3 fused with velgrad_calculate2D (solution.F90:1110-1186):
 // the derivative fields of the nonlinear term are the velgrad ones, so one
 // pass over A1 A2 B1 B2 produces both the x-forward nonlinear term and the
-// first-derivative moments' partial sums (part[q * gridDim.x + block], all q
-// but the dissipation term 3, which vg_dis_kernel adds from C1 C2 D).
-// Asks for 3 waves per SIMD (168 registers; measured at 2048^2 FP64 66.6 us
-// against 76.5 us at 2 waves).
+// moments' partial sums (part[q * gridDim.x + block]).  The dissipation
+// moment, sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta), takes u and
+// theta = d1 u1 + d2 u2 from the gradient transforms and two more x
+// transforms: (Lap u1, Lap u2) from -k1^2 A_i + C_i, and Lap(theta) from
+// -i k1^3 A1 + i k1 C1 - k1^2 B2 + D (C_i: the y part of Lap u_i, D: of
+// d2 Lap u2).
+// Occupancy asked of the compiler: 2 waves per SIMD in FP64 (204 registers;
+// at 3 it spills 96 bytes per lane: 2048^2 116 us against 131 us), 3 in FP32.
 template <typename T, int N>
-__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 3)))
 void row_stage_vg_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
@@ -483,6 +496,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     // eta_min = min sqrt(nu / |div|) = sqrt(nu / max |div|): both functions are
     // monotone, so one division and one square root per thread give the same value
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, dmax = 0;
+    T th[E];  // theta at the thread's points, for the dissipation moment
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         n[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
@@ -491,6 +505,7 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         const double u1 = z[k].x, u2 = z[k].y;
         const double b11 = w[k].x, b21 = w[k].y, b12 = y[k].x, b22 = y[k].y;
         const double div = b11 + b22;
+        th[k] = (T)div;
         const double uu = u1 * u1 + u2 * u2;
         s_um2 += div * div * uu;
         s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
@@ -500,85 +515,66 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         s_urms += u1 * u1 + u2 * u2;
         s_dudx += b11 * b11 + b22 * b22;
     }
+    // the first-derivative moments' partial sums now (their accumulators are
+    // not held across the transforms below)
+    {
+        double eta = fmin(eta0, sqrt(nu / dmax));
+        if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+        double v0[3] = {s_um2, s_ijji, s_u2t};
+        double v1[4] = {s_urms, s_dudx, s_eps, eta};
+        vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
+        vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     // nonlinear term: x-forward and split into the two half rows
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid, lim);
 
-    double eta = fmin(eta0, sqrt(nu / dmax));
-    if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
-    double v0[3] = {s_um2, s_ijji, s_u2t};
-    double v1[4] = {s_urms, s_dudx, s_eps, eta};
-    vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
-    vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
-}
-
-// The dissipation moment of velgrad_calculate2D (solution.F90:1110-1186),
-//   sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta),
-// from A1 A2 (u), B2 (d2 u2), C1 C2 (y part of Lap u), D (y part of d2 Lap u2):
-// theta and Lap(theta) are real and share one transform.
-template <typename T, int N>
-__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
-void vg_dis_kernel(VgArgs<T> a) {
-    using C = VgCfg<T, N>;
-    constexpr int E = C::E, P = C::P, LR = C::L;
-    if (a.gate.closed()) return;
-    at_member(a);
-    __shared__ cx<T> lds[LR * lds_line_cx(N)];
-    __shared__ double red[C::THREADS / 64][8];
-    const int tid = threadIdx.x;
-    const int l = tid / P, j = tid % P;
-    const int row0 = blockIdx.x * LR + l;
-    const bool valid = row0 < a.g.n1;
-    const int row = valid ? row0 : a.g.n1 - 1;
-    const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
-    const size_t hp = (size_t)a.g.hp;
-    constexpr bool BUF = P >= 64;
-    const size_t ro = (size_t)line_row<P>(row) * hp;
-    const int lim = a.g.kxlim;  // entries from kxlim on are zero
-    const HalfRow<T, BUF> A1(a.w[0], ro, lim), A2(a.w[1], ro, lim), B2(a.w[3], ro, lim);
+    // the dissipation moment: (Lap u1, Lap u2), then Lap(theta) (the second
+    // transform's imaginary input is zero)
+    // A1, A2, B2 are read again (from the L2) through laundered base pointers:
+    // otherwise the compiler merges these loads with the first ones and holds
+    // their values in registers across every transform in between
     const HalfRow<T, BUF> C1(a.w[4], ro, lim), C2(a.w[5], ro, lim), D(a.w[6], ro, lim);
-
-    // one load phase: each input half row is read once and feeds all three
-    // packed transforms (u1, u2), (theta, Lap theta), (Lap u1, Lap u2)
-    cx<T> z[E], t[E], w[E];
+    const HalfRow<T, BUF> A1d(launder(a.w[0]), ro, lim), A2d(launder(a.w[1]), ro, lim), B2d(launder(a.w[3]), ro, lim);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int kk = 0; kk < E; ++kk) {
-        // points k and E-1-k in turn: the half-row entries of point E-1-k are
-        // the mirrors (h = N - m) of point k's on the partner lane, so the
-        // second read of each entry follows the first and hits the cache
-        const int k = (kk & 1) ? E - 1 - (kk >> 1) : (kk >> 1);
-        __builtin_amdgcn_sched_barrier(0);
+    for (int k = 0; k < E; ++k) {
+        if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
-        const T k1 = (T)h;
-        const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> a1 = A1[h], a2 = A2[h], b2 = B2[h], c1 = C1[h], c2 = C2[h], d = D[h];
-        z[k] = pack_vals<T, N>(a1, a2, m);
-        // theta = i k1 A1 + B2, Lap(theta) = -i k1^3 A1 + i k1 C1 - k1^2 B2 + D
-        const cx<T> th = cadd(cscale(cmuli(a1), k1o), b2);
-        const cx<T> lt = cadd(cadd(cscale(cmuli(a1), -(k1o * k1o * k1o)), cscale(cmuli(c1), k1o)),
-                              cadd(cscale(b2, -(k1 * k1)), d));
-        t[k] = pack_vals<T, N>(th, lt, m);
-        // Lap(u_i) = x-inverse of (-k1^2 A_i + C_i)
-        w[k] = pack_vals<T, N>(cadd(cscale(a1, -(k1 * k1)), c1), cadd(cscale(a2, -(k1 * k1)), c2), m);
+        const T q = -((T)h * (T)h);
+        w[k] = pack_vals<T, N>(cadd(cscale(A1d[h], q), C1[h]), cadd(cscale(A2d[h], q), C2[h]), m);
     }
     __builtin_amdgcn_sched_barrier(0);
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);  // theta, Lap(theta)
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
     double s_dis = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * (double)t[k].x * (double)w[k].x +
-                 2.0 * (double)z[k].y * (double)t[k].x * (double)w[k].y;
+        s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1 = (T)h;
+        const T k1o = (h == N / 2) ? (T)0 : k1;
+        const cx<T> lt = cadd(cadd(cscale(cmuli(A1d[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
+                              cadd(cscale(B2d[h], -(k1 * k1)), D[h]));
+        w[k] = pack_vals<T, N>(lt, cx<T>{(T)0, (T)0}, m);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(theta)
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double uu = (double)z[k].x * z[k].x + (double)z[k].y * z[k].y;
-        s_dis += uu * (double)t[k].y;
+        s_dis += uu * (double)w[k].x;
     }
+
     if (!valid) s_dis = 0;
-    double v[1] = {s_dis};
-    vg_reduce<C::THREADS, 1>(v, red, 3, a.part);
+    double vd[1] = {s_dis};
+    vg_reduce<C::THREADS, 1>(vd, red, 3, a.part);
 }
 
 // shell spectra (spectra_compute2D, solution.F90:930-1016); deterministic:
@@ -839,15 +835,12 @@ static int row_n(int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
 }
 
 template <typename T, int N>
-static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk, int which) {
+static int vg_n(const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
     using C = VgCfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (nblk && *nblk != g) return -1;  // partial buffer sized for a different grid
-    if (!a.hout[0] || !a.hout[1] || which < 0 || which > 2) return -1;
-    if (which != 2)
-        hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
-    if (which != 1)
-        hipLaunchKernelGGL((vg_dis_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
+    if (!a.hout[0] || !a.hout[1]) return -1;
+    hipLaunchKernelGGL((row_stage_vg_kernel<T, N>), dim3(g, 1, zdim(a.g.nmem)), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
@@ -874,9 +867,9 @@ int launch_row(int n, int mode, const RowArgs<T>& a, int nrows, hipStream_t s) {
     return -1;
 }
 template <typename T>
-int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk, int which) {
+int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nblk) {
     switch (n) {
-#define X(NN) case NN: return vg_n<T, NN>(a, nrows, s, nblk, which);
+#define X(NN) case NN: return vg_n<T, NN>(a, nrows, s, nblk);
         K2D_SIZES(X)
 #undef X
     }
@@ -911,7 +904,7 @@ int launch_scale(const ScaleArgs<T>& a, hipStream_t s) {
 #define K2D_INSTANTIATE(T)                                                                     \
     template int launch_update<T>(int, const ColFwdArgs<T>&, hipStream_t);                      \
     template int launch_row<T>(int, int, const RowArgs<T>&, int, hipStream_t);                 \
-    template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*, int);        \
+    template int launch_velgrad<T>(int, const VgArgs<T>&, int, hipStream_t, int*);             \
     template int launch_spectra<T>(const SpecArgs<T>&, int, hipStream_t);                      \
     template int launch_force_band<T>(const ForceArgs<T>&, hipStream_t);                      \
     template int launch_energy<T>(const EnergyArgs<T>&, int, hipStream_t);                     \

@@ -11,7 +11,7 @@
 //                               dealiasing3D (tool.F90:196-217), projection3D
 //                               (:245-270), RK33D (:233-270) / CN3D (:272-301)
 //   spec3_kernel                spectra_compute3D (:1018-1108)
-//   vg3_kernel                  velgrad_calculate3D (:1188-1286)
+//   vg3_kernel                  velgrad_calculate3D (:1188-1286), all its moments
 //   force3_kernel               forcing3D methods 1, 2, 5 (:542-814)
 #pragma once
 #include <climits>
@@ -555,8 +555,13 @@ using Vg3Cfg = k2d::VgCfg<T, N, 8>;
 // (b23, b32) for the symmetric terms, then (b22, b33) completes theta with
 // b11 from the (u3, b11) transform.  Only u, b11 and later theta are held
 // across transforms, so 8 points per thread fit in registers and a 512-point
-// line is one wavefront.
-template <typename T, int N>
+// line is one wavefront.  DIS (FP64): the dissipation moment follows from u,
+// theta and two more transforms (Lap u and Lap theta), so every moment of the
+// iteration comes from this one pass (at 512^3 3.82 ms against 2.25 + 1.77
+// ms for two passes); in FP32 the two-pass form is faster (2.52 against
+// 2.74 ms: the merged kernel's registers allow 3 waves per SIMD, the two
+// passes 4 and 5) and vg3_dis_kernel adds the dissipation moment.
+template <typename T, int N, bool DIS>
 __global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
 void vg3_kernel(Vg3Args<T> a) {
     using C = Vg3Cfg<T, N>;
@@ -627,14 +632,14 @@ void vg3_kernel(Vg3Args<T> a) {
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(W(4), W(8), j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u2, d3 u3
-    double dv[E];
+    T th[DIS ? E : 1];  // theta at the thread's points, for the dissipation moment
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
         const double b11k = b11[k], b22 = w[k].x, b33 = w[k].y;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
         const double div = b11k + b22 + b33;
-        dv[k] = div;
+        if constexpr (DIS) th[k] = (T)div;
         s_um2 += div * div * uu;
         s_ijji += (b11k * b11k + b22 * b22 + b33 * b33) * uu;
         s_u2t += div * uu;
@@ -643,12 +648,63 @@ void vg3_kernel(Vg3Args<T> a) {
         s_urms += u1 * u1 + u2 * u2 + uu3 * uu3;
         s_dudx += b11k * b11k + b22 * b22 + b33 * b33;
     }
-    double eta = fmin(eta0, sqrt(nu / dmax));
-    if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
-    double v0[3] = {s_um2, s_ijji, s_u2t};
-    double v1[4] = {s_urms, s_dudx, s_eps, eta};
-    k2d::vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
-    k2d::vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
+    {
+        double eta = fmin(eta0, sqrt(nu / dmax));
+        if (!valid) { s_um2 = s_ijji = s_u2t = s_urms = s_dudx = s_eps = 0; eta = eta0; }
+        double v0[3] = {s_um2, s_ijji, s_u2t};
+        double v1[4] = {s_urms, s_dudx, s_eps, eta};
+        k2d::vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
+        k2d::vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
+    }
+    if constexpr (DIS) {
+        // the dissipation moment, sum 2 theta u_i Lap(u_i) + |u|^2 Lap(theta), from
+        // u and theta above and two more x transforms: (Lap u1, Lap u2) and
+        // (Lap u3, Lap theta), Lap(u_c) = -k1^2 Y_c + L_c, Lap(theta) =
+        // -k1^2 theta + i k1 L_1 + D with theta = i k1 Y_1 + Y2_2 + Y3_3 (the
+        // rows already read come again from the L2, through laundered pointers
+        // so the compiler does not keep the first loads' values live)
+        __builtin_amdgcn_sched_barrier(0);
+        auto Wd = [&](int q) { return Row(k2d::launder(a.w[q]), ro, lim); };
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
+            const int m = j + k * P;
+            const int h = m <= N / 2 ? m : N - m;
+            const T q = -((T)h * (T)h);
+            w[k] = pack_vals<T, N>(cadd(cscale(Wd(0)[h], q), W(9)[h]), cadd(cscale(Wd(1)[h], q), W(10)[h]), m);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
+        double s_dis = 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
+                     2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
+            const int m = j + k * P;
+            const int h = m <= N / 2 ? m : N - m;
+            const T k1 = (T)h;
+            const T k1o = (h == N / 2) ? (T)0 : k1;
+            const T q = -(k1 * k1);
+            const cx<T> l1 = W(9)[h];
+            const cx<T> thh = cadd(cscale(cmuli(Wd(0)[h]), k1o), cadd(Wd(4)[h], Wd(8)[h]));
+            const cx<T> lth = cadd(cadd(cscale(thh, q), cscale(cmuli(l1), k1o)), W(12)[h]);
+            w[k] = pack_vals<T, N>(cadd(cscale(Wd(2)[h], q), W(11)[h]), lth, m);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u3), Lap(theta)
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
+            const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
+            s_dis += 2.0 * uu3 * (double)th[k] * (double)w[k].x + uu * (double)w[k].y;
+        }
+        if (!valid) s_dis = 0;
+        double vd[1] = {s_dis};
+        k2d::vg_reduce<C::THREADS, 1>(vd, red, 3, a.part);
+    }
 }
 
 // The dissipation moment of velgrad_calculate3D (solution.F90:1188-1286),
@@ -995,8 +1051,12 @@ static int vg3_n(const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
     using C = Vg3Cfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (g != nblk) return -1;
-    hipLaunchKernelGGL((vg3_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
-    hipLaunchKernelGGL((vg3_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    if constexpr (sizeof(T) == 8) {
+        hipLaunchKernelGGL((vg3_kernel<T, N, true>), dim3(g), dim3(C::THREADS), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((vg3_kernel<T, N, false>), dim3(g), dim3(C::THREADS), 0, s, a);
+        hipLaunchKernelGGL((vg3_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
+    }
     return 0;
 }
 

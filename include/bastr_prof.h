@@ -28,9 +28,9 @@ enum bastr_kclass {
     BASTR_K_COL_FWD_B,
     BASTR_K_COL_FWD_C,
     BASTR_K_FORCE,           /* band forcing                                    */
-    BASTR_K_VG_DIS,          /* 2-D: the dissipation moment's x pass (VELGRAD is
-                                then stage A's x pass); 3-D: unused (VELGRAD
-                                covers both velocity-gradient kernels)           */
+    BASTR_K_VG_DIS,          /* not launched: VELGRAD's pass forms every
+                                velocity-gradient moment, the dissipation one
+                                included (the number is kept)                    */
     BASTR_K_CLOSURE,         /* learned subgrid force of a stage (bastr_closure.h) */
     BASTR_K_NCLASS
 };
