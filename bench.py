@@ -670,6 +670,14 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
                           "frac": round(b_step * steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
         "direct_launch": {"ms_per_step": wall_direct / steps * 1e3, "value": npts * steps / wall_direct},
     }
+    # every class of the warm-up (all classes timed with events): per launch
+    # and per iteration, against its compulsory bytes
+    nw = max(warmup, 1)
+    rec["classes"] = {
+        cname: {"us_per_launch": round(ms / n * 1e3, 2), "launches_per_iter": round(n / nw, 3),
+                "us_per_iter": round(ms / nw * 1e3, 2), "bytes_per_launch": by,
+                "frac": round(by / (ms / n * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        for cname, (c, n, ms, by) in sorted(warm.items(), key=lambda kv: -kv[1][2]) if n}
     if copy:
         rec["roofline"]["copy_measured"] = round(copy_gbps, 1)
         rec["roofline"]["frac_of_copy"] = round(achieved / copy_gbps, 4)
@@ -835,6 +843,7 @@ def main():
         "step_roofline": rec["step_roofline"],
         "launch": rec["launch"],
         "direct_launch": rec["direct_launch"],
+        "classes": rec["classes"],
         "autotune": tune or None,
         "accuracy": accuracy,
     }
