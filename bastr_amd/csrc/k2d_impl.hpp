@@ -445,10 +445,11 @@ __device__ __forceinline__ void vg_reduce(double* vals, double (*red)[8], int q0
 // transforms: (Lap u1, Lap u2) from -k1^2 A_i + C_i, and Lap(theta) from
 // -i k1^3 A1 + i k1 C1 - k1^2 B2 + D (C_i: the y part of Lap u_i, D: of
 // d2 Lap u2).
-// Occupancy asked of the compiler: 2 waves per SIMD in FP64 (204 registers;
-// at 3 it spills 96 bytes per lane: 2048^2 116 us against 131 us), 3 in FP32.
+// Occupancy asked of the compiler: 3 waves per SIMD (168 registers in FP64,
+// 128 in FP32; FP64 1024-point rows 2, which spill at 3).
 template <typename T, int N>
-__global__ __launch_bounds__((VgCfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 3)))
+__global__ __launch_bounds__((VgCfg<T, N>::THREADS))
+__attribute__((amdgpu_waves_per_eu((sizeof(T) == 8 && N == 1024) ? 2 : 3)))
 void row_stage_vg_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
@@ -456,12 +457,16 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     at_member(a);
     __shared__ cx<T> lds[LR * lds_line_cx(N)];
     __shared__ double red[C::THREADS / 64][8];
+    // theta at each thread's points, held here (each thread reads back only
+    // what it wrote) from the moments to the dissipation moment
+    __shared__ T thl[LR * N];
     const int tid = threadIdx.x;
     const int l = tid / P, j = tid % P;
     const int row0 = blockIdx.x * LR + l;
     const bool valid = row0 < a.g.n1;
     const int row = valid ? row0 : a.g.n1 - 1;
     const LdsLine<T, false> Ls{lds + l * lds_line_cx(N), 1};
+    T* const th = thl + l * N;
     const size_t hp = (size_t)a.g.hp;
     constexpr bool BUF = P >= 64;
     const size_t ro = (size_t)line_row<P>(row) * hp;
@@ -496,16 +501,20 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     // eta_min = min sqrt(nu / |div|) = sqrt(nu / max |div|): both functions are
     // monotone, so one division and one square root per thread give the same value
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, dmax = 0;
-    T th[E];  // theta at the thread's points, for the dissipation moment
+    // the nonlinear term goes to the line's LDS as it is formed (each thread
+    // reads back only its own entries before the forward transform), so it
+    // is not held in registers beside z, w and y
+    lds_sync<false>();
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        n[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
-        n[k].x = n[k].x - z[k].y * y[k].x;
-        n[k].y = n[k].y - z[k].y * y[k].y;
+        cx<T> nk = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
+        nk.x = nk.x - z[k].y * y[k].x;
+        nk.y = nk.y - z[k].y * y[k].y;
+        Ls.st(j + k * P, nk);
         const double u1 = z[k].x, u2 = z[k].y;
         const double b11 = w[k].x, b21 = w[k].y, b12 = y[k].x, b22 = y[k].y;
         const double div = b11 + b22;
-        th[k] = (T)div;
+        th[j + k * P] = (T)div;
         const double uu = u1 * u1 + u2 * u2;
         s_um2 += div * div * uu;
         s_ijji += (b11 * b11 + b22 * b22 + 2.0 * b12 * b21) * uu;
@@ -527,6 +536,8 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     }
     __builtin_amdgcn_sched_barrier(0);
     // nonlinear term: x-forward and split into the two half rows
+#pragma unroll
+    for (int k = 0; k < E; ++k) n[k] = Ls.ld(j + k * P);
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid, lim);
 
@@ -551,8 +562,8 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     double s_dis = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
-                 2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
+        s_dis += 2.0 * (double)z[k].x * (double)th[j + k * P] * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)th[j + k * P] * (double)w[k].y;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
