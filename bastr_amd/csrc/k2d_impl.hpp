@@ -133,6 +133,10 @@ __device__ __forceinline__ cx<T> pack_at(const cx<T>* __restrict__ A, const cx<T
 constexpr int kBufWord3 = 0x00020000;          // gfx9 raw buffer, no format conversion
 constexpr unsigned kBufRange = 0x80000000u;    // whole-array resources (every array < 2 GiB)
 
+// off: the lane's byte offset (the whole offset: measured, a wave-uniform part
+// passed as the instructions' SGPR offset gave results that differed from run
+// to run while other streams' kernels ran beside them).  An off at or past
+// the range (kBufRange) reads zero.
 template <typename T>
 __device__ __forceinline__ cx<T> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     cx<T> v;
@@ -146,6 +150,22 @@ __device__ __forceinline__ cx<T> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off
         __builtin_memcpy(&v, &w, 8);
     }
     return v;
+}
+
+// Store through a buffer resource: an off at or past the range is dropped.
+template <typename T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, cx<T> v) {
+    if constexpr (sizeof(cx<T>) == 16) {
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        u4 w;
+        __builtin_memcpy(&w, &v, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, 0);
+    } else {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        u2 w;
+        __builtin_memcpy(&w, &v, 8);
+        __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, 0);
+    }
 }
 
 // Half row ro (element offset of its first entry) of the array at base, its

@@ -35,18 +35,38 @@ using k2d::xcd_block;
 // inputs they share come from its L2.  MT = 1 serves the single-term jobs
 // (most of them) with a lean load phase; MT = 4 the sums.
 // ---------------------------------------------------------------------------
-// multiplier of `mult` at signed wavenumber k as a complex factor (re, im)
+// The multipliers as (c0 + c2 k^2) g, times i where `usei`: g is the pass's
+// wavenumber ko (zero at its Nyquist index) where `useko`, else a per-column
+// constant.  Every product is of integer-valued operands, exact or rounded
+// once, so each factor equals the direct formula's:
+//   M_ONE 1; M_IK i ko; M_MK2 -k^2; M_MIK3 -i ko^3 = i (-k^2) ko;
+//   M_LAP -(kc^2 + k^2); M_LAP_IKC -i (kc^2 + k^2) kco; M_LAP_IK -i (kc^2 + k^2) ko.
 template <typename T>
-__device__ __forceinline__ cx<T> mult_factor(int mult, T k, T ko, T kc, T kco) {
+struct MultCoef {
+    T c0, c2, g;
+    bool useko, usei;
+};
+template <typename T>
+__device__ __forceinline__ MultCoef<T> mult_coef(int mult, T kc, T kco) {
+    const T kc2 = -(kc * kc);
     switch (mult) {
-        case M_IK: return {(T)0, ko};
-        case M_MK2: return {-(k * k), (T)0};
-        case M_MIK3: return {(T)0, -(ko * ko * ko)};
-        case M_LAP: return {-(kc * kc + k * k), (T)0};
-        case M_LAP_IKC: return {(T)0, -((kc * kc + k * k) * kco)};
-        case M_LAP_IK: return {(T)0, -((kc * kc + k * k) * ko)};
-        default: return {(T)1, (T)0};
+        case M_IK: return {(T)1, (T)0, (T)1, true, true};
+        case M_MK2: return {(T)0, (T)-1, (T)1, false, false};
+        case M_MIK3: return {(T)0, (T)-1, (T)1, true, true};
+        case M_LAP: return {kc2, (T)-1, (T)1, false, false};
+        case M_LAP_IKC: return {kc2, (T)-1, kco, false, true};
+        case M_LAP_IK: return {kc2, (T)-1, (T)1, true, true};
+        default: return {(T)1, (T)0, (T)1, false, false};
     }
+}
+// v times the multiplier at position m of a length-N pass
+template <typename T, int N>
+__device__ __forceinline__ cx<T> apply_mult(cx<T> v, const MultCoef<T>& c, int m) {
+    const T k = (T)ksigned(m, N);
+    const T ko = (m == N / 2) ? (T)0 : k;
+    const T f = (c.c0 + c.c2 * (k * k)) * (c.useko ? ko : c.g);
+    const cx<T> w = cscale(v, f);
+    return c.usei ? cmuli(w) : w;
 }
 
 // the multi-term variant runs 8 points per thread in FP64 (register pressure);
@@ -63,6 +83,11 @@ constexpr int wpe_for(int threads, int want) {
     return w >= want ? w : want / w * w;
 }
 
+// Loads and stores go through a buffer resource at the column block's first
+// entry (wave-uniform), with 32-bit offsets ((m pitch + l) entries; colx_n
+// checks that N pitch entries fit 2^31 bytes): entries the pass skips
+// (|k| >= kcut) take an offset past the range, which reads zero or drops the
+// store, so neither phase branches per point.
 template <typename T, int N, bool INV, int MT, int EM = 16, int CBM = 1>
 __global__ __launch_bounds__((ColxCfg<T, N, MT, EM, CBM>::THREADS))
 __attribute__((amdgpu_waves_per_eu(wpe_for(ColxCfg<T, N, MT, EM, CBM>::THREADS, MT == 1 ? 3 : 2))))
@@ -70,6 +95,7 @@ void colx_kernel(ColXArgs<T> a) {
     using C = ColxCfg<T, N, MT, EM, CBM>;
     static_assert(C::FITS, "column plan exceeds a workgroup");
     constexpr int E = C::E, P = C::P, CB = C::CB;
+    constexpr unsigned ES = sizeof(cx<T>);
     if (a.gate.closed()) return;
     __shared__ T lds[CB * lds_line(N)];
     const int nbpp = a.ncols / CB;
@@ -80,38 +106,38 @@ void colx_kernel(ColXArgs<T> a) {
     const int plane = plane0 >= a.pskip_at ? plane0 + a.pskip : plane0;
     const int tid = threadIdx.x;
     const int l = tid % CB, j = tid / CB;
-    const size_t base = (size_t)plane * a.plane_stride + (size_t)cb * CB + l;
-    const size_t pitch = (size_t)a.pitch;
+    // the block's first entry (wave-uniform) and this lane's offset from it
+    const size_t blk0 = (size_t)blockIdx.z * (size_t)a.mstride + (size_t)plane * a.plane_stride + (size_t)cb * CB;
+    const unsigned pitch = (unsigned)a.pitch;
+    const unsigned lo = ((unsigned)j * pitch + (unsigned)l) * ES;
     const LdsLine<T, true> L{lds + l, CB};
     const ColJob& jb = a.job[q];
     // the column's own wavenumber (LAP multipliers only)
     T kc = 0, kco = 0;
     if (a.kc_div > 0) {
-        const int ic = (int)(base / (size_t)a.kc_div);
-        kc = (T)ksigned(ic, a.kc_n);
-        kco = (2 * ic == a.kc_n) ? (T)0 : kc;
+        const unsigned ic = (unsigned)((size_t)plane * a.plane_stride + (size_t)cb * CB + l) / (unsigned)a.kc_div;
+        kc = (T)ksigned((int)ic, a.kc_n);
+        kco = (2 * (int)ic == a.kc_n) ? (T)0 : kc;
     }
     // entries |k| >= kcut along the pass: zero in an inverse pass's inputs,
     // not stored by a forward pass
     const int kcut = a.kcut > 0 ? a.kcut : N;
     auto live = [&](int m) { return (m <= N / 2 ? m : N - m) < kcut; };
+    // point k: the lane offset lo plus k P pitch entries, or past the range
+    // where the pass skips the point
+    auto at = [&](int k) { return lo + (unsigned)(k * P) * pitch * ES; };
+    auto off = [&](int k) { return live(j + k * P) ? at(k) : k2d::kBufRange; };
     cx<T> v[E];
-    const size_t mo = (size_t)blockIdx.z * (size_t)a.mstride;  // ensemble member offset
     if (MT == 1) {
-        const cx<T>* __restrict__ src = a.in[jb.in[0]] + mo;
+        const auto r = __builtin_amdgcn_make_buffer_rsrc((void*)(a.in[jb.in[0]] + blk0), (short)0,
+                                                         (int)k2d::kBufRange, k2d::kBufWord3);
         const int mult = jb.mult[0];
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int m = j + k * P;
-            v[k] = (!INV || live(m)) ? src[base + (size_t)m * pitch] : cx<T>{(T)0, (T)0};
-        }
-        if (mult != M_ONE) {
+        for (int k = 0; k < E; ++k) v[k] = k2d::buf_load<T>(r, INV ? off(k) : at(k));
+        if (INV && mult != M_ONE) {  // (forward jobs take no multiplier: colx_n)
+            const MultCoef<T> c = mult_coef<T>(mult, kc, kco);
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int m = j + k * P;
-                const T kk = (T)ksigned(m, N);
-                v[k] = cmul(v[k], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco));
-            }
+            for (int k = 0; k < E; ++k) v[k] = apply_mult<T, N>(v[k], c, j + k * P);
         }
     } else {
 #pragma unroll
@@ -120,26 +146,24 @@ void colx_kernel(ColXArgs<T> a) {
             // one term's loads at a time: keeps the next term's loads from
             // being hoisted over this one's accumulation (register pressure)
             __builtin_amdgcn_sched_barrier(0);
-            const cx<T>* __restrict__ src = a.in[jb.in[t]] + mo;
-            const int mult = jb.mult[t];
+            const auto r = __builtin_amdgcn_make_buffer_rsrc((void*)(a.in[jb.in[t]] + blk0), (short)0,
+                                                             (int)k2d::kBufRange, k2d::kBufWord3);
+            const MultCoef<T> c = mult_coef<T>(jb.mult[t], kc, kco);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
-                const int m = j + k * P;
-                const T kk = (T)ksigned(m, N);
-                if (!INV || live(m))
-                    v[k] = cadd(v[k], cmul(src[base + (size_t)m * pitch], mult_factor(mult, kk, (m == N / 2) ? (T)0 : kk, kc, kco)));
+                const cx<T> x = k2d::buf_load<T>(r, INV ? off(k) : at(k));
+                v[k] = cadd(v[k], jb.mult[t] == M_ONE ? x : apply_mult<T, N>(x, c, j + k * P));
             }
         }
         __builtin_amdgcn_sched_barrier(0);
     }
     line_fft<T, N, E, INV, true>(v, j, L, a.tw);
-    cx<T>* __restrict__ dst = a.out[jb.out] + mo;
+    const auto w = __builtin_amdgcn_make_buffer_rsrc((void*)(a.out[jb.out] + blk0), (short)0, (int)k2d::kBufRange,
+                                                     k2d::kBufWord3);
     const T sc = (T)a.scale;
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int m = j + k * P;
-        if (INV || live(m)) dst[base + (size_t)m * pitch] = (sc == (T)1) ? v[k] : cscale(v[k], sc);
-    }
+    for (int k = 0; k < E; ++k)
+        k2d::buf_store<T>(w, INV ? at(k) : off(k), (sc == (T)1) ? v[k] : cscale(v[k], sc));
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,11 +1029,14 @@ static int colx_n(bool inverse, const ColXArgs<T>& a, hipStream_t s) {
     if (cbm == 2 && a.ncols % cb_of(em, 2)) cbm = 1;  // the wide tile needs whole blocks: the default width
     const int cb1 = cb_of(em, cbm);
     if (a.ncols % cb1 || a.ncols % C4::CB || a.njob < 1 || a.njob > 13) return -1;
+    // a block's entries sit within 32-bit byte offsets of its first (colx_kernel)
+    if ((long)N * a.pitch * (long)sizeof(cx<T>) >= (long)k2d::kBufRange) return -1;
     // single-term and multi-term jobs go to the two kernel variants
     ColXArgs<T> part[2] = {a, a};
     part[0].njob = part[1].njob = 0;
     for (int q = 0; q < a.njob; ++q) {
         if (a.job[q].nterm < 1 || a.job[q].nterm > 4) return -1;
+        if (!inverse && a.job[q].nterm == 1 && a.job[q].mult[0] != M_ONE) return -1;
         ColXArgs<T>& d = part[a.job[q].nterm > 1 ? 1 : 0];
         d.job[d.njob++] = a.job[q];
     }

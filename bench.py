@@ -22,8 +22,9 @@ returns the per-iteration diagnostics to the host, as a log-writing driver
 would.
 
 Prints one JSON line.  `value` = grid points x steps / wall seconds of the
-timed run, whose step calls replay their captured HIP graphs (what a caller
-gets).  `roofline` is the dominant kernel class: its compulsory HBM bytes per
+timed run, whose step calls run as a caller's do: replaying their captured HIP
+graphs on grids of at most 2^21 points, with direct launches above (the
+library's default, faster there).  `roofline` is the dominant kernel class: its compulsory HBM bytes per
 launch over its mean duration, timed with HIP events on the library's stream
 around each of its launches in a run of as many iterations from the same
 field (events make those step calls launch directly; `direct_launch` is that
@@ -70,6 +71,8 @@ def prof_api(lib):
     lib.bastr_prof_name.restype = C.c_char_p
     lib.bastr_prof_copy_gbps.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
     lib.bastr_prof_copy_gbps.restype = C.c_int
+    lib.bastr_prof_graph.argtypes = [C.c_void_p]
+    lib.bastr_prof_graph.restype = C.c_int
 
 
 def cpu_model():
@@ -606,7 +609,8 @@ def new_field(bastr_amd, params):
 def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
     """Time `steps` iterations of workload `name` after `warmup` untimed ones,
     as a caller runs them: each bastr_step call (a chunk of iterations) replays
-    its captured HIP graph.  Profiling events would force direct launches, so
+    its captured HIP graph on a small grid, or launches directly on a large one.
+    Profiling events force direct launches, so
     the dominant kernel class (the largest share of a warm-up with every class
     timed) is timed in a second run of as many iterations from the same initial
     field: HIP events on the library's stream around each of its launches.
@@ -614,9 +618,11 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
     and the last timed iteration's spectrum."""
     cfg = CONFIGS[name]
     grid, prec = cfg["grid"], cfg["prec"]
-    # the timed run: unprofiled, so every call replays its graph (the warm-up's
-    # last call captures the graph of the chunk length the timed calls replay)
+    # the timed run: unprofiled, so every call runs as a caller's does (where the
+    # library replays graphs, the warm-up's last call captures the graph of the
+    # chunk length the timed calls replay)
     g = new_field(bastr_amd, cfg_params(name))
+    replay = lib.bastr_prof_graph(g._h) == 1
     run_steps(g, max(warmup, min(chunk, steps)), min(chunk, steps))
     t1 = time.perf_counter()
     last = run_steps(g, steps, chunk)
@@ -648,7 +654,8 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
         if lib.bastr_prof_copy_gbps(s._h, 1 << 30, 10, C.byref(cg)) == 0:
             copy_gbps = cg.value
     s.close()
-    print(f"  [{name}] timed {steps} steps in {wall:.3f} s (graph replay); profiled run {wall_direct:.3f} s",
+    print(f"  [{name}] timed {steps} steps in {wall:.3f} s ({'graph replay' if replay else 'direct launches'}); "
+          f"profiled run {wall_direct:.3f} s",
           file=sys.stderr, flush=True)
     npts = 1
     for n in grid:
@@ -658,7 +665,9 @@ def measure(lib, bastr_amd, name, steps, warmup, chunk, copy=False):
     rec = {
         "value": npts * steps / wall, "unit": "grid-pt·steps/s", "steps": steps, "warmup": warmup,
         "ms_per_step": wall / steps * 1e3, "dtype": "f32" if prec == 1 else "f64",
-        "launch": "graph replay (each bastr_step call of up to 200 iterations replays its captured HIP graph)",
+        "launch": ("graph replay (each bastr_step call of up to 200 iterations replays its captured HIP graph)"
+                   if replay else "direct launches (the library's default above 2^21 grid points: "
+                   "faster there than replaying a captured graph)"),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile(dom_name, prec, grid), "kernel": dom_name,

@@ -43,6 +43,11 @@ int bastr_prof_enable(bastr_handle* h, uint32_t mask);
 int bastr_prof_read(const bastr_handle* h, int kclass, int64_t* launches, double* total_ms,
                     double* bytes_per_launch);
 const char* bastr_prof_name(int kclass);
+/* 1 if the handle's step calls replay a captured graph (unprofiled calls of
+ * up to 256 iterations), 0 if they launch every kernel directly: the default
+ * replays grids of at most 2^21 points over all members; BASTR_GRAPH=0 or 1
+ * in the environment at bastr_create forces either. */
+int bastr_prof_graph(const bastr_handle* h);
 /* Device-to-device copy bandwidth on the handle's stream: `reps` copies of
  * `bytes` between two scratch buffers of that size (allocated and freed by
  * the call), timed with HIP events; *gbps = 2 * bytes * reps / time / 1e9

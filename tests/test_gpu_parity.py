@@ -235,10 +235,13 @@ def test_graph_replay_equals_direct_launches(hip, monkeypatch, shape, prec):
     ka = shape[2] if len(shape) == 3 else 0
     p = make_params(shape[0], shape[1], ka, forcemethod=2, lproject=True, precision=prec)
     u = hitgen(shape, k0=4, seed=21)
+    hip.bastr_prof_graph.argtypes = [C.c_void_p]
+    hip.bastr_prof_graph.restype = C.c_int
     monkeypatch.setenv("BASTR_GRAPH", "0")
     a = Solver(hip, "bastr", p)
     monkeypatch.delenv("BASTR_GRAPH")
-    b = Solver(hip, "bastr", p)
+    b = Solver(hip, "bastr", p)  # a small grid: graph replay by default
+    assert hip.bastr_prof_graph(a._h) == 0 and hip.bastr_prof_graph(b._h) == 1
     a.set_fields(*u)
     b.set_fields(*u)
     for n, prof in ((3, 0), (3, 0), (2, 0x3FF), (3, 0), (2, 0)):
