@@ -120,6 +120,7 @@ CLOSURE_SIGNATURES = [
     ("closure_train", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _D]),
     ("closure_apply", C.c_int, [_P, _P, C.c_int32, _F]),
     ("closure_attach", C.c_int, [_P, _P]),
+    ("closure_per_step", C.c_int, [_P, C.c_int32]),
 ]
 
 

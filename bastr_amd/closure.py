@@ -125,6 +125,14 @@ class Closure:
         _check(self._lib, self._lib.bastr_closure_attach(solver._h, self._h))
 
     @staticmethod
+    def per_step(solver, on=True):
+        """Cadence of the attached force: True evaluates the network once per
+        time step (on the step's input field) and adds that force at the three
+        RK3 stages; False (the default) evaluates it at every stage."""
+        lib = load()
+        _check(lib, lib.bastr_closure_per_step(solver._h, 1 if on else 0))
+
+    @staticmethod
     def detach(solver):
         lib = load()
         _check(lib, lib.bastr_closure_attach(solver._h, None))
@@ -174,11 +182,12 @@ def shell_energy(spec_row):
     return spec_row[1] + spec_row[2]
 
 
-def evaluate(fine_params, coarse_params, seed, t0_steps, fine_steps, net=None, k0=4.0, law=1):
+def evaluate(fine_params, coarse_params, seed, t0_steps, fine_steps, net=None, k0=4.0, law=1, per_step=False):
     """E(k) of a coarse run against the fine run of the same seed.  The coarse
     run starts from the fine field filtered after t0_steps fine iterations and
     advances (fine_steps - t0_steps) * dt_fine / dt_coarse iterations, with the
-    network attached when given.  Returns (E_fine[1..K], E_coarse[1..K]) after
+    network attached when given (per_step: its force evaluated once per time
+    step, Closure.per_step).  Returns (E_fine[1..K], E_coarse[1..K]) after
     fine_steps fine iterations, K = the coarse cutoff - 1 (shells inside it)."""
     from . import solver as _solver
     ratio = coarse_params.deltat / fine_params.deltat
@@ -191,6 +200,7 @@ def evaluate(fine_params, coarse_params, seed, t0_steps, fine_steps, net=None, k
     filter_to(f, c)
     if net is not None:
         net.attach(c)
+        Closure.per_step(c, per_step)
     rc2 = f.step(fine_steps - t0_steps, want_spectra=False)[0]
     rc3 = c.step(int(round(ncoarse)), want_spectra=False)[0]
     # the diagnostics of the next iteration are those of the field reached

@@ -186,6 +186,7 @@ struct bastr_handle {
     void* CR[4] = {};
     void* CF[2] = {};
     void* CX[2] = {};
+    bool clo_per_step = false;  // the closure force of stage A held for stages B and C
     int* mforce = nullptr;            // [nmem] device: member is band-forced (null: all are)
     std::vector<char> mset;           // member's field has been set
     std::vector<int64_t> mnstep;      // per-member nstep / time (ensembles)
@@ -623,8 +624,14 @@ int run_stage(bastr_handle* h, int stage, const Gate& gstep, const void* const* 
     }
     if (h->net) {  // subgrid force of the stage field (RK3 only: Uin is the stage input)
         ProfScope ps(h, BASTR_K_CLOSURE);
-        const int rc = run_closure<T>(h, gstep, Uin);
-        if (rc) return rc;
+        if (h->clo_per_step && stage > 0) {
+            // once per step: stage A's x-transformed force, still in CX
+            LAUNCHCHK(clo::launch_add<T>(gstep, C<T>(h->NX[0]), C<T>(h->NX[1]), C<T>(h->CX[0]), C<T>(h->CX[1]),
+                                         h->nhalf, h->stream));
+        } else {
+            const int rc = run_closure<T>(h, gstep, Uin);
+            if (rc) return rc;
+        }
     }
     {
         static const int modes[5] = {UP_A, UP_B, UP_C, UP_CN_MID, UP_CN_LAST};
@@ -3284,6 +3291,14 @@ int bastr_closure_attach(bastr_handle* h, bastr_closure* net) {
     drop_graph(h);  // captured step graphs do not hold the closure's launches (or still do)
     h->net = net;
     h->net_version = net ? net->version : 0;
+    return BASTR_OK;
+}
+
+int bastr_closure_per_step(bastr_handle* h, int32_t per_step) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_closure_per_step: NULL handle");
+    if (per_step != 0 && per_step != 1) return fail(BASTR_EINVAL, "bastr_closure_per_step: per_step must be 0 or 1");
+    if (h->clo_per_step != (per_step == 1)) drop_graph(h);  // the captured stages launch the other cadence
+    h->clo_per_step = per_step == 1;
     return BASTR_OK;
 }
 

@@ -443,8 +443,10 @@ def bench_closure(args):
     coarse run starts from the filtered fine field after 800 fine iterations
     and runs 150 iterations (600 fine ones); E(k) for k < k_c against the
     filtered fine run, with and without the network: rel-L2 over the shells,
-    and the mean and largest per-shell relative error.  value = the closure run's grid-pt.steps/s; the fine run's
-    rate beside it, and both in simulated time per second."""
+    and the mean and largest per-shell relative error, with the force evaluated at
+    every RK3 stage and once per time step.  value = the closure run's
+    grid-pt.steps/s with the once-per-step force (the per-stage rate beside it);
+    the fine run's rate beside them, and both in simulated time per second."""
     import numpy as np
 
     import bastr_amd
@@ -480,10 +482,10 @@ def bench_closure(args):
     print(f"  [2d-closure] {iters} training steps in {t_train:.2f} s, batch loss {losses[0]:.4f} -> "
           f"{float(np.mean(losses[-50:])):.4f}", file=sys.stderr, flush=True)
     # E(k), k < k_c, on the held-out seeds
-    res = {"without": [], "with": []}
+    res = {"without": [], "with": [], "with_per_step": []}
     for seed in test_seeds:
-        for label, nn in (("without", None), ("with", net)):
-            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=800, fine_steps=1400, net=nn, k0=4.0)
+        for label, nn, per in (("without", None, False), ("with", net, False), ("with_per_step", net, True)):
+            Ef, Ec = evaluate(fp, cp, seed=seed, t0_steps=800, fine_steps=1400, net=nn, k0=4.0, per_step=per)
             rel = np.abs(Ec - Ef) / Ef
             res[label].append({"seed": seed, "ek_rel_l2": float(np.linalg.norm(Ec - Ef) / np.linalg.norm(Ef)),
                                "ek_err_mean_per_shell": float(rel.mean()),
@@ -492,23 +494,27 @@ def bench_closure(args):
                 for q in ("ek_rel_l2", "ek_err_mean_per_shell", "ek_err_max_per_shell")}
             for k, v in res.items()}
     print(f"  [2d-closure] held-out E(k): {summ}", file=sys.stderr, flush=True)
-    # throughput: the closure run (graph replay) and the fine run
+    # throughput: the closure run (graph replay) in both cadences and the fine run
     f = fine_run(fp, 21, 4.0)
     f.step(800, want_spectra=False)
-    c = bastr_amd.solver(cp)
-    filter_to(f, c)
-    net.attach(c)
     steps = 150
-    c.step(steps, want_spectra=False)  # captures the graph of the call length
-    filter_to(f, c)
-    tc = time.perf_counter()
-    rc_c = c.step(steps, want_spectra=True)[0]
-    t_coarse = (time.perf_counter() - tc) / steps
+    t_cad, rc_cad = {}, {}
+    for per in (True, False):
+        c = bastr_amd.solver(cp)
+        filter_to(f, c)
+        net.attach(c)
+        Closure.per_step(c, per)
+        c.step(steps, want_spectra=False)  # captures the graph of the call length
+        filter_to(f, c)
+        tc = time.perf_counter()
+        rc_cad[per] = c.step(steps, want_spectra=True)[0]
+        t_cad[per] = (time.perf_counter() - tc) / steps
+        c.close()
+    t_coarse, rc_c = t_cad[True], rc_cad[True]
     f.step(steps, want_spectra=False)
     tf = time.perf_counter()
     rc_f = f.step(steps, want_spectra=True)[0]
     t_fine = (time.perf_counter() - tf) / steps
-    c.close()
     f.close()
     # the network's kernels against the BF16 MFMA peak
     tr_ms, fw_ms = C.c_double(), C.c_double()
@@ -534,6 +540,7 @@ def bench_closure(args):
         o = oracle.solver(cp)
         params, scales = net.params()
         oracle.set_closure(o, params, scales)
+        oracle.set_closure_per_step(o, True)
         o.set_fields(*u)
         o.get_fields()
         its = 150
@@ -545,7 +552,7 @@ def bench_closure(args):
         cpu = {"value": n * n * its / dt_cpu, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
                "cpu_model": model, "host_logical_cpus": ncpu,
                "sample": f"{its} iterations of the 256^2 closure run ({dt_cpu:.1f} s), FP64 C restatement with "
-                         f"the network's forward pass in compute_ut, OpenMP"}
+                         f"the network's forward pass in compute_ut once per time step, OpenMP"}
     out = {
         "metric": METRIC,
         "value": n * n / t_coarse,
@@ -576,10 +583,15 @@ def bench_closure(args):
             "train_seconds": round(t_train, 3), "batch_loss_first": float(losses[0]),
             "batch_loss_last50": float(np.mean(losses[-50:])),
             "held_out": summ, "held_out_per_seed": res,
+            "cadence": "value, coarse_ms_per_step and simulated_time_ratio: the force evaluated once per "
+                       "time step (bastr_closure_per_step); per_stage: evaluated at each RK3 stage",
             "coarse_ms_per_step": t_coarse * 1e3, "fine_ms_per_step": t_fine * 1e3,
             "fine_grid_pt_steps_per_s": N * N / t_fine,
             "simulated_time_per_s": {"closure_run": cp.deltat / t_coarse, "fine_run": fp.deltat / t_fine},
             "simulated_time_ratio": (cp.deltat / t_coarse) / (fp.deltat / t_fine),
+            "per_stage": {"coarse_ms_per_step": t_cad[False] * 1e3,
+                          "simulated_time_ratio": (cp.deltat / t_cad[False]) / (fp.deltat / t_fine),
+                          "status": rc_cad[False]},
             "status": {"coarse": rc_c, "fine": rc_f},
         },
     }

@@ -14,7 +14,8 @@
  * (36 -> 64 -> 64 -> 2, GELU, BF16 weights and MFMA operands, FP32
  * accumulation) models tau from the 3x3 stencil of the coarse u1, u2,
  * theta = d1 u1 + d2 u2 and omega = d1 u2 - d2 u1.  A handle with a network
- * attached adds its output to N(u_bar) at every RK3 stage.
+ * attached adds its output to N(u_bar) at every RK3 stage (evaluated on each
+ * stage's field, or once per step: bastr_closure_per_step).
  *
  * Conventions as in bastr.h (status codes, bastr_last_error, Fortran-order
  * fields; calls are synchronous).  Grids: square, a power of two in
@@ -101,6 +102,13 @@ int bastr_closure_apply(bastr_closure* net, const bastr_pairs* ps, int32_t i, fl
  * steps.
  */
 int bastr_closure_attach(bastr_handle* h, bastr_closure* net);
+/*
+ * Cadence of the attached force.  per_step = 0 (the default): evaluated on
+ * each RK3 stage's field.  per_step = 1: evaluated once per time step, on the
+ * step's input field (stage A), and the same force added at the three stages
+ * (the closure term held over the step: a third of the network evaluations).
+ */
+int bastr_closure_per_step(bastr_handle* h, int32_t per_step);
 
 #ifdef __cplusplus
 }

@@ -217,6 +217,9 @@ struct bo_state {
     float* cl_params;
     float cl_scales[6];
     double* cl_fld;                  /* [4][n] u1, u2, theta, omega of a stage */
+    double* cl_out;                  /* [2][n] the force of the step's stage A (per-step cadence) */
+    int cl_per_step;                 /* 1: the force of stage A is added at the three stages */
+    int cl_stage;                    /* RK3 stage of the current compute_ut (0, 1, 2) */
     int64_t nstep;
     double time;
     int prologue_pending;
@@ -323,10 +326,20 @@ static void closure_point(const bo_state* s, const float* x, double* out) {
     }
 }
 
-/* subgrid force of the stage field: u (spe, physical) and its gradients (ux) */
+/* subgrid force of the stage field: u (spe, physical) and its gradients (ux);
+ * with the per-step cadence, stages B and C add stage A's force again */
 static void closure_force(bo_state* s, cplx** t) {
     const int ia = s->ia, ja = s->ja;
     const size_t n = s->n;
+    double* const O = s->cl_out;
+    if (s->cl_per_step && s->cl_stage > 0) {
+        OMP_FOR
+        for (long mm = 0; mm < (long)n; ++mm) {
+            t[0][mm] += O[mm];
+            t[1][mm] += O[n + mm];
+        }
+        return;
+    }
     double* F = s->cl_fld;
     for (size_t m = 0; m < n; ++m) {
         F[m] = creal(s->spe[0][m]);
@@ -347,6 +360,8 @@ static void closure_force(bo_state* s, cplx** t) {
         }
         double o[2];
         closure_point(s, x, o);
+        O[mm] = o[0];
+        O[n + mm] = o[1];
         t[0][mm] += o[0];
         t[1][mm] += o[1];
     }
@@ -357,17 +372,26 @@ int bo_set_closure(bo_state* s, const float* params, const float* scales) {
     if (!params) {
         free(s->cl_params);
         free(s->cl_fld);
+        free(s->cl_out);
         s->cl_params = NULL;
         s->cl_fld = NULL;
+        s->cl_out = NULL;
         return BASTR_OK;
     }
     if (!s->cl_params) {
         s->cl_params = (float*)malloc(sizeof(float) * 6658);
         s->cl_fld = (double*)malloc(sizeof(double) * 4 * s->n);
-        if (!s->cl_params || !s->cl_fld) return fail(BASTR_ENOMEM, "bo_set_closure: allocation failed");
+        s->cl_out = (double*)malloc(sizeof(double) * 2 * s->n);
+        if (!s->cl_params || !s->cl_fld || !s->cl_out) return fail(BASTR_ENOMEM, "bo_set_closure: allocation failed");
     }
     memcpy(s->cl_params, params, sizeof(float) * 6658);
     for (int c = 0; c < 6; ++c) s->cl_scales[c] = scales[c];
+    return BASTR_OK;
+}
+
+int bo_set_closure_per_step(bo_state* s, int32_t per_step) {
+    if (per_step != 0 && per_step != 1) return fail(BASTR_EINVAL, "bo_set_closure_per_step: 0 or 1");
+    s->cl_per_step = per_step;
     return BASTR_OK;
 }
 
@@ -425,6 +449,7 @@ static void load_spe(bo_state* s) {
 static void rk3(bo_state* s) {
     const int nd = s->nd;
     const double dt = s->dt;
+    s->cl_stage = 0;
     compute_ut(s, s->tA);
     for (int c = 0; c < nd; ++c) {
         const double* u = s->u[c];
@@ -434,6 +459,7 @@ static void rk3(bo_state* s) {
         for (long m = 0; m < (long)s->n; ++m) sp[m] = u[m] + dt * creal(a[m]) / 2.0;
         fftf(s, sp);
     }
+    s->cl_stage = 1;
     compute_ut(s, s->tB);
     for (int c = 0; c < nd; ++c) {
         const double* u = s->u[c];
@@ -444,6 +470,7 @@ static void rk3(bo_state* s) {
         for (long m = 0; m < (long)s->n; ++m) sp[m] = u[m] - dt * creal(a[m]) + 2.0 * dt * creal(b[m]);
         fftf(s, sp);
     }
+    s->cl_stage = 2;
     compute_ut(s, s->tC);
     for (int c = 0; c < nd; ++c) {
         double* u = s->u[c];
@@ -833,7 +860,7 @@ int bo_destroy(bo_state* s) {
     }
     free(s->thetaxixi);
     free(s->Es); free(s->Ed); free(s->kn); free(s->Ecount);
-    free(s->cl_params); free(s->cl_fld);
+    free(s->cl_params); free(s->cl_fld); free(s->cl_out);
     if (s->plan.p[0].tw) plannd_free(&s->plan);
     free(s);
     return BASTR_OK;

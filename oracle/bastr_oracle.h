@@ -31,6 +31,9 @@ int bo_set_spectrum_law(bo_state* s, int32_t law);
 /* The learned closure of include/bastr_closure.h added to every compute_ut
  * (2-D, square grids): params NPARAM floats, scales 6 (NULL params: none). */
 int bo_set_closure(bo_state* s, const float* params, const float* scales);
+/* 1: the closure force is evaluated at RK3 stage A only and added again at
+ * stages B and C (bastr_closure_per_step); 0 (default): at every stage. */
+int bo_set_closure_per_step(bo_state* s, int32_t per_step);
 int bo_get_fields(bo_state* s, double* u1, double* u2, double* u3);
 int bo_set_time(bo_state* s, int64_t nstep, double time);
 int bo_get_time(const bo_state* s, int64_t* nstep, double* time);

@@ -28,6 +28,8 @@ def lib():
         L.bo_fft_c2c.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int]
         L.bo_set_closure.restype = C.c_int
         L.bo_set_closure.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.bo_set_closure_per_step.restype = C.c_int
+        L.bo_set_closure_per_step.argtypes = [C.c_void_p, C.c_int32]
         L.bo_set_threads.argtypes = [C.c_int]
         L.bo_get_threads.restype = C.c_int
         _LIB = L
@@ -50,6 +52,13 @@ def set_closure(solver, params, scales):
                                   s.ctypes.data_as(C.POINTER(C.c_float)))
     if rc:
         raise RuntimeError("bo_set_closure failed")
+
+
+def set_closure_per_step(solver, on=True):
+    """Cadence of the oracle's closure force: once per time step (stage A's
+    force added at the three stages) or, on=False, at every stage."""
+    if lib().bo_set_closure_per_step(solver._h, 1 if on else 0):
+        raise RuntimeError("bo_set_closure_per_step failed")
 
 
 def set_threads(n):

@@ -3,8 +3,9 @@ hook of oracle/bastr_oracle.c) against what the mathematics fixes: BF16
 round-to-nearest-even known answers, the tanh form of GELU against the exact
 z Phi(z), the backward pass against central finite differences of the
 forward pass (FP64), the stencil orientation of the features on a field
-with a known form, and the C oracle's force (seen through a step of size
-1e-8) against the numpy forward pass."""
+with a known form, the C oracle's force (seen through a step of size
+1e-8) against the numpy forward pass, and its once-per-step cadence (one
+step of a field too small for the nonlinear term is u + dt D(tau(u)))."""
 import math
 
 import numpy as np
@@ -128,3 +129,44 @@ def test_c_oracle_closure_force_matches_numpy_forward(orc, n):
     orc.set_closure(c, np.zeros(co.NPARAM, np.float32), scales)
     c.step(1)
     assert all(np.array_equal(x, y) for x, y in zip(c.get_fields(), a.get_fields()))
+
+
+def test_c_oracle_per_step_cadence_holds_stage_a_force(orc):
+    """The per-step cadence (bo_set_closure_per_step) adds the force of the
+    step's input field at all three RK3 stages, whose weights sum to one.  On a
+    field of size 1e-6 with nu ~ 0 the nonlinear and viscous terms vanish to
+    1e-9 of the force, so one step is u + dt D(tau(u)), D the tendency's
+    dealiasing and tau the numpy forward pass on u: the held force is stage A's
+    exactly.  Evaluated at every stage instead, the force sees the stage fields
+    u + O(dt tau), which differ from u by far more than u itself."""
+    from bastr_amd.fields import hitgen
+    n = 32
+    rng = np.random.default_rng(7)
+    params = (rng.standard_normal(co.NPARAM) * 0.2).astype(np.float32)
+    scales = np.array([0.8, 0.7, 5.0, 6.0, 0.3, 0.4], dtype=np.float32)
+    u = [x * 1e-6 for x in hitgen((n, n), k0=3, seed=6)]
+    dt = 1e-3
+    p = make_params(n, n, deltat=dt, reynolds=1e12)
+    rate = {}
+    for per_step in (True, False):
+        s = orc.solver(p)
+        s.set_fields(*u)
+        u0 = s.get_fields()  # entry-dealiased
+        orc.set_closure(s, params, scales)
+        orc.set_closure_per_step(s, per_step)
+        s.step(1)
+        rate[per_step] = [(x - y) / dt for x, y in zip(s.get_fields(), u0)]
+    X = co.features(spectral_fields(*u0), scales)
+    out, _ = co.forward(params, X)
+    want = [out[:, c].reshape(n, n, order="F") * scales[4 + c] for c in range(2)]
+    k = np.arange(n, dtype=np.float64)
+    k[n // 2 + 1:] -= n
+    k1, k2 = np.meshgrid(k, k, indexing="ij")
+    keep = np.sqrt(k1 ** 2 + k2 ** 2) < n // 3
+    want = [np.real(np.fft.ifft2(np.fft.fft2(w) * keep)) for w in want]
+    scale = max(np.abs(w).max() for w in want)
+    held = max(np.abs(f - w).max() for f, w in zip(rate[True], want))
+    staged = max(np.abs(f - w).max() for f, w in zip(rate[False], want))
+    print("held", held, "staged", staged, "scale", scale)
+    assert scale > 0.05 and held < 1e-6 * scale, (held, scale)
+    assert staged > 2e-4 * scale, (staged, scale)

@@ -181,11 +181,12 @@ def test_training_reduces_the_loss(hip, small_pairs):
     assert l1 < 0.5 * l0
 
 
-def test_closure_run_matches_oracle(hip, orc, small_pairs):
+@pytest.mark.parametrize("per_step", [False, True])
+def test_closure_run_matches_oracle(hip, orc, small_pairs, per_step):
     """A coarse 64^2 run with a network attached against the C oracle with the
-    same network in compute_ut; the difference is small against the network's
-    own effect."""
-    print("case closure run vs oracle 64^2", flush=True)
+    same network in compute_ut, the force evaluated at every RK3 stage or once
+    per step; the difference is small against the network's own effect."""
+    print(f"case closure run vs oracle 64^2 per_step={per_step}", flush=True)
     net = Closure(seed=9)
     net.fit_scales(small_pairs)
     params, scales = net.params()
@@ -196,11 +197,13 @@ def test_closure_run_matches_oracle(hip, orc, small_pairs):
     g = Solver(hip, "bastr", p)
     g.set_fields(*u)
     net.attach(g)
+    Closure.per_step(g, per_step)
     g0 = Solver(hip, "bastr", p)
     g0.set_fields(*u)
     o = orc.solver(p)
     o.set_fields(*u)
     orc.set_closure(o, params, scales)
+    orc.set_closure_per_step(o, per_step)
     rg, dg, sg = g.step(5)
     r0, _, _ = g0.step(5)
     ro, do, so = o.step(5)
