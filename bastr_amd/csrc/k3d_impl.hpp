@@ -323,16 +323,26 @@ void cf2_kernel(k2d::ColFwdArgs<T> a) {
     const int l = tid % CB, j = tid / CB;
     const int kx = blk * CB + l;
     const LdsLine<T, true> L{lds + l, CB};
-    const size_t pitch = (size_t)a.pitch;
     const T sc = (T)a.g.inv_n;
     const int kcut = a.kcut > 0 ? a.kcut : N;
+    // every array through a buffer resource at the block's first column, with
+    // 32-bit lane offsets (cf2_n checks that N pitch entries fit 2^31 bytes)
+    const int c0 = blk * CB;
+    const unsigned pitch = (unsigned)a.pitch;
+    auto rs = [&](const cx<T>* p) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(p + c0), (short)0, (int)k2d::kBufRange, k2d::kBufWord3);
+    };
+    auto at = [&](int k) { return ((unsigned)(j + k * P) * pitch + (unsigned)l) * (unsigned)sizeof(cx<T>); };
+    const __amdgpu_buffer_rsrc_t rNX[2] = {rs(a.nx[0]), rs(a.nx[1])}, rU[2] = {rs(a.U[0]), rs(a.U[1])},
+                                 rUB[2] = {rs(a.UB[0]), rs(a.UB[1])}, rKA[2] = {rs(a.KA[0]), rs(a.KA[1])};
+    // both components' loads issued before either transform
     cx<T> v[2][E];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[c][k] = a.nx[c][(size_t)(j + k * P) * pitch + kx];
-        line_fft<T, N, E, false, true>(v[c], j, L, a.tw);
-    }
+        for (int k = 0; k < E; ++k) v[c][k] = k2d::buf_load<T>(rNX[c], at(k));
+#pragma unroll
+    for (int c = 0; c < 2; ++c) line_fft<T, N, E, false, true>(v[c], j, L, a.tw);
     const T dt = (T)a.g.dt;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -341,22 +351,22 @@ void cf2_kernel(k2d::ColFwdArgs<T> a) {
         const int ky = ksigned(m, N);
         if ((ky < 0 ? -ky : ky) >= kcut) continue;
         const int kk2 = kx * kx + ky * ky;
-        const size_t idx = (size_t)m * pitch + kx;
+        const unsigned o = at(k);
         cx<T> t0 = cscale(v[0][k], sc), t1 = cscale(v[1][k], sc);
-        const cx<T> U0 = a.U[0][idx], U1 = a.U[1][idx];
+        const cx<T> U0 = k2d::buf_load<T>(rU[0], o), U1 = k2d::buf_load<T>(rU[1], o);
         cx<T> s0, s1, ka0, ka1;
         if (MODE == k2d::UP_A) {
             s0 = U0; s1 = U1;
         } else if (MODE == k2d::UP_CN_MID || MODE == k2d::UP_CN_LAST) {
             if (a.cn_first) { s0 = U0; s1 = U1; }
-            else { s0 = a.UB[0][idx]; s1 = a.UB[1][idx]; }
+            else { s0 = k2d::buf_load<T>(rUB[0], o); s1 = k2d::buf_load<T>(rUB[1], o); }
         } else if (MODE == k2d::UP_B) {
-            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
+            ka0 = k2d::buf_load<T>(rKA[0], o); ka1 = k2d::buf_load<T>(rKA[1], o);
             s0 = cadd(U0, cscale(cscale(ka0, dt), (T)0.5));
             s1 = cadd(U1, cscale(cscale(ka1, dt), (T)0.5));
         } else {
-            ka0 = a.KA[0][idx]; ka1 = a.KA[1][idx];
-            s0 = a.UB[0][idx]; s1 = a.UB[1][idx];
+            ka0 = k2d::buf_load<T>(rKA[0], o); ka1 = k2d::buf_load<T>(rKA[1], o);
+            s0 = k2d::buf_load<T>(rUB[0], o); s1 = k2d::buf_load<T>(rUB[1], o);
         }
         const T vis = (T)(-a.g.nu * (double)kk2);
         t0 = cadd(t0, cscale(s0, vis));
@@ -382,25 +392,26 @@ void cf2_kernel(k2d::ColFwdArgs<T> a) {
         }
         if (MODE == k2d::UP_CN_MID) {
             const cx<T> n0 = cadd(U0, cscale(t0, dt)), n1 = cadd(U1, cscale(t1, dt));
-            a.UB[0][idx] = cscale(cadd(n0, U0), (T)0.5);
-            a.UB[1][idx] = cscale(cadd(n1, U1), (T)0.5);
+            k2d::buf_store<T>(rUB[0], o, cscale(cadd(n0, U0), (T)0.5));
+            k2d::buf_store<T>(rUB[1], o, cscale(cadd(n1, U1), (T)0.5));
         } else if (MODE == k2d::UP_CN_LAST) {
-            a.U[0][idx] = cadd(U0, cscale(t0, dt));
-            a.U[1][idx] = cadd(U1, cscale(t1, dt));
+            k2d::buf_store<T>(rU[0], o, cadd(U0, cscale(t0, dt)));
+            k2d::buf_store<T>(rU[1], o, cadd(U1, cscale(t1, dt)));
         } else if (MODE == k2d::UP_A) {
-            a.KA[0][idx] = t0; a.KA[1][idx] = t1;
-            a.UB[0][idx] = cadd(U0, cscale(cscale(t0, dt), (T)0.5));
-            a.UB[1][idx] = cadd(U1, cscale(cscale(t1, dt), (T)0.5));
+            k2d::buf_store<T>(rKA[0], o, t0);
+            k2d::buf_store<T>(rKA[1], o, t1);
+            k2d::buf_store<T>(rUB[0], o, cadd(U0, cscale(cscale(t0, dt), (T)0.5)));
+            k2d::buf_store<T>(rUB[1], o, cadd(U1, cscale(cscale(t1, dt), (T)0.5)));
         } else if (MODE == k2d::UP_B) {
             const T dt2 = (T)2 * dt;
-            a.UB[0][idx] = cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2));
-            a.UB[1][idx] = cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2));
-            a.KA[0][idx] = cadd(ka0, cscale(t0, (T)4));
-            a.KA[1][idx] = cadd(ka1, cscale(t1, (T)4));
+            k2d::buf_store<T>(rUB[0], o, cadd(csub(U0, cscale(ka0, dt)), cscale(t0, dt2)));
+            k2d::buf_store<T>(rUB[1], o, cadd(csub(U1, cscale(ka1, dt)), cscale(t1, dt2)));
+            k2d::buf_store<T>(rKA[0], o, cadd(ka0, cscale(t0, (T)4)));
+            k2d::buf_store<T>(rKA[1], o, cadd(ka1, cscale(t1, (T)4)));
         } else {
             const T r6 = (T)(1.0 / 6.0);
-            a.U[0][idx] = cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6));
-            a.U[1][idx] = cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6));
+            k2d::buf_store<T>(rU[0], o, cadd(U0, cscale(cscale(cadd(ka0, t0), dt), r6)));
+            k2d::buf_store<T>(rU[1], o, cadd(U1, cscale(cscale(cadd(ka1, t1), dt), r6)));
         }
     }
 }
@@ -1119,6 +1130,8 @@ template <typename T, int N>
 static int cf2_n(int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s) {
     using C = Fwd2Cfg<T, N>;
     if (a.nkx < C::CB || a.nkx % C::CB || a.nkx > a.pitch || a.g.n1 != N) return -1;
+    // a block's entries sit within 32-bit byte offsets of its first (cf2_kernel)
+    if ((long)N * a.pitch * (long)sizeof(cx<T>) >= (long)k2d::kBufRange) return -1;
     dim3 grid(k2d::xcd_grid_h(a.nkx / C::CB), 1, k2d::zdim(a.g.nmem)), block(C::THREADS);
     switch (mode) {
 #define M(MM) case MM: hipLaunchKernelGGL((cf2_kernel<T, N, MM>), grid, block, 0, s, a); break;
