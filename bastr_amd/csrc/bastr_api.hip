@@ -375,7 +375,9 @@ k3d::ColJob job1(int in, int mult, int out) {
 // 2-D y-inverse of the half spectra src[0..1] into W:
 //   mode 0: W0 = u1, W1 = u2
 //   mode 1: + W2 = i k2 u1, W3 = i k2 u2                 (compute_ut2D :825-830)
-//   mode 2: + W4 = -k2^2 u1, W5 = -k2^2 u2, W6 = -i k2^3 u2   (velgrad_calculate2D)
+//   mode 2: + W4 = Lap u1, W5 = Lap u2 = -(k1^2 + k2^2) u_i, W6 = d2 Lap u2
+//           = -i k2 (k1^2 + k2^2) u2, the y part of Lap theta = d1 Lap u1 +
+//           d2 Lap u2 (velgrad_calculate2D :1125-1140); k1 is the column's kx
 template <typename T>
 int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode, bool prune) {
     k3d::ColXArgs<T> c{};
@@ -391,9 +393,11 @@ int run_yinv2(bastr_handle* h, const Gate& g, const void* const* src, int mode, 
         c.job[nj++] = job1(1, k3d::M_IK, 3);
     }
     if (mode == 2) {
-        c.job[nj++] = job1(0, k3d::M_MK2, 4);
-        c.job[nj++] = job1(1, k3d::M_MK2, 5);
-        c.job[nj++] = job1(1, k3d::M_MIK3, 6);
+        c.job[nj++] = job1(0, k3d::M_LAP, 4);
+        c.job[nj++] = job1(1, k3d::M_LAP, 5);
+        c.job[nj++] = job1(1, k3d::M_LAP_IK, 6);
+        c.kc_div = 1;  // the column offset within a row is kx
+        c.kc_n = h->n0;
     }
     c.njob = nj;
     c.pitch = h->hp;

@@ -149,7 +149,7 @@ template <typename T>
 struct VgArgs {
     Geo<T> g;
     Gate gate;
-    const cx<T>* w[7];  // A1 A2 B1 B2 C1 C2 D
+    const cx<T>* w[7];  // u1 u2 (i k2 u1) (i k2 u2) (Lap u1) (Lap u2) (d2 Lap u2), y-inverses
     double* part;       // [8][gridDim.x]
     cx<T>* hout[2];     // non-null: also produce stage A's x-forward nonlinear term
     const cx<T>* tw;

@@ -462,9 +462,8 @@ __device__ __forceinline__ void vg_reduce(double* vals, double (*red)[8], int q0
 // moments' partial sums (part[q * gridDim.x + block]).  The dissipation
 // moment, sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta), takes u and
 // theta = d1 u1 + d2 u2 from the gradient transforms and two more x
-// transforms: (Lap u1, Lap u2) from -k1^2 A_i + C_i, and Lap(theta) from
-// -i k1^3 A1 + i k1 C1 - k1^2 B2 + D (C_i: the y part of Lap u_i, D: of
-// d2 Lap u2).
+// transforms: (Lap u1, Lap u2), whose y-inverses the column pass formed with
+// the full multiplier -(k1^2 + k2^2), and Lap(theta) = i k1 Lap u1 + d2 Lap u2.
 // Occupancy asked of the compiler: 3 waves per SIMD (168 registers in FP64,
 // 128 in FP32; FP64 1024-point rows 2, which spill at 3).
 template <typename T, int N>
@@ -561,21 +560,18 @@ void row_stage_vg_kernel(VgArgs<T> a) {
     line_fft<T, N, E, false, false>(n, j, Ls, a.tw);
     unpack_store<T, N, E>(n, j, Ls, a.hout[0] + ro, a.hout[1] + ro, valid, lim);
 
-    // the dissipation moment: (Lap u1, Lap u2), then Lap(theta) (the second
-    // transform's imaginary input is zero)
-    // A1, A2, B2 are read again (from the L2) through laundered base pointers:
-    // otherwise the compiler merges these loads with the first ones and holds
-    // their values in registers across every transform in between
-    const HalfRow<T, BUF> C1(a.w[4], ro, lim), C2(a.w[5], ro, lim), D(a.w[6], ro, lim);
-    const HalfRow<T, BUF> A1d(launder(a.w[0]), ro, lim), A2d(launder(a.w[1]), ro, lim), B2d(launder(a.w[3]), ro, lim);
+    // the dissipation moment: (Lap u1, Lap u2), then Lap(theta) = i k1 Lap u1 +
+    // d2 Lap u2 (the second transform's imaginary input is zero), from the
+    // y-inverses the column pass formed.  Lap u1 is read again (from the L2)
+    // through a laundered base pointer: otherwise the compiler merges these
+    // loads with the first ones and holds their values across the transform.
+    const HalfRow<T, BUF> L1(a.w[4], ro, lim), L2(a.w[5], ro, lim), D2(a.w[6], ro, lim);
+    const HalfRow<T, BUF> L1d(launder(a.w[4]), ro, lim);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T q = -((T)h * (T)h);
-        w[k] = pack_vals<T, N>(cadd(cscale(A1d[h], q), C1[h]), cadd(cscale(A2d[h], q), C2[h]), m);
+        w[k] = pack_rows<T, N>(L1, L2, j + k * P);
     }
     __builtin_amdgcn_sched_barrier(0);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
@@ -589,11 +585,8 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
-        const T k1 = (T)h;
-        const T k1o = (h == N / 2) ? (T)0 : k1;
-        const cx<T> lt = cadd(cadd(cscale(cmuli(A1d[h]), -(k1o * k1o * k1o)), cscale(cmuli(C1[h]), k1o)),
-                              cadd(cscale(B2d[h], -(k1 * k1)), D[h]));
-        w[k] = pack_vals<T, N>(lt, cx<T>{(T)0, (T)0}, m);
+        const T k1o = (h == N / 2) ? (T)0 : (T)h;
+        w[k] = pack_vals<T, N>(cadd(cscale(cmuli(L1d[h]), k1o), D2[h]), cx<T>{(T)0, (T)0}, m);
     }
     __builtin_amdgcn_sched_barrier(0);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(theta)
