@@ -107,106 +107,7 @@ __global__ void finalize_kernel(FinArgs a) {
     }
 }
 
-// Small grids: the reduction and the diagnostics in one workgroup (one launch
-// instead of two).  Wavefront w reduces quantities w, w + 16, ..., QB at a
-// time with all their loads issued together: lane l folds partials l, l + 64,
-// ... in order, then a fixed butterfly over the lanes; the sums stay in LDS
-// for the diagnostics.
-constexpr int kFin1MaxPartials = 256;  // largest per-quantity partial count of this path
-__global__ __launch_bounds__(1024) void finalize1_kernel(FinArgs a) {
-    if (a.gate.closed()) return;
-    extern __shared__ double red[];  // [2*allkmax + 12]
-    const int K = a.allkmax;
-    {  // ensemble member blockIdx.z
-        const size_t m = blockIdx.z;
-        a.part_bins += m * (size_t)a.nblk_spec * (K + 1) * 2;
-        a.part_tot += m * (size_t)a.nblk_spec * 4;
-        a.part_vg += m * (size_t)a.nblk_vg * 8;
-        a.diag += m * (size_t)a.dstride;
-        a.spec += m * (size_t)a.sstride;
-        a.status += m;
-    }
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int nq = 2 * K + 12;
-    constexpr int QB = 4, RB = kFin1MaxPartials / 64;
-    for (int q0 = wave; q0 < nq; q0 += 16 * QB) {
-        double v[QB][RB];
-        bool mn[QB];
-        int cnt[QB];
-#pragma unroll
-        for (int u = 0; u < QB; ++u) {
-            const int q = q0 + 16 * u;
-            const double* src = a.part_tot;
-            int n = 0;
-            bool is_min = false;
-            if (q < 2 * K) {
-                const int b = q % K + 1, c = q / K;
-                src = a.part_bins + ((size_t)b * 2 + c) * a.nblk_spec;
-                n = a.nblk_spec;
-            } else if (q < 2 * K + 4) {
-                src = a.part_tot + (size_t)(q - 2 * K) * a.nblk_spec;
-                n = a.nblk_spec;
-            } else if (q < nq) {
-                const int w = q - 2 * K - 4;
-                src = a.part_vg + (size_t)w * a.nblk_vg;
-                n = a.nblk_vg;
-                is_min = w == 7;
-            }
-            mn[u] = is_min;
-            cnt[u] = n;
-            const double none = is_min ? (double)(2.0f * 3.14f) : 0.0;
-#pragma unroll
-            for (int r = 0; r < RB; ++r) v[u][r] = lane + 64 * r < n ? src[lane + 64 * r] : none;
-        }
-#pragma unroll
-        for (int u = 0; u < QB; ++u) {
-            double acc = v[u][0];
-#pragma unroll
-            for (int r = 1; r < RB; ++r)
-                if (lane + 64 * r < cnt[u]) acc = mn[u] ? fmin(acc, v[u][r]) : acc + v[u][r];
-            acc = mn[u] ? wave_min(acc) : wave_sum(acc);
-            if (lane == 0 && q0 + 16 * u < nq) red[q0 + 16 * u] = acc;
-        }
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < K; b += blockDim.x) {
-        a.spec[K + b] = red[b];
-        a.spec[2 * K + b] = red[K + b];
-    }
-    if (threadIdx.x == 0) {
-        const double* t = red + 2 * K;
-        const double* v = red + 2 * K + 4;
-        const double N = a.npts, nu = a.nu;
-        const double dim = a.ndims == 2 ? 2.0 : 3.0;
-        const double urms = sqrt(v[4] / N), dudx2 = v[5] / N, eps = v[6] / N;
-        const double ufluc = urms / sqrt(dim);
-        const double tay = ufluc / sqrt(dudx2 / dim);
-        bastr_diag* d = a.diag;
-        d->Esspe = t[0];
-        d->Edspe = t[1];
-        d->k2Edspe = t[2];
-        d->kMEdspe = t[3];
-        d->umumtheta2 = v[0] / N;
-        d->umumijji = v[1] / N;
-        d->u2theta = v[2] / N;
-        d->dissp = v[3] / N;
-        d->eta_min = v[7];
-        d->Taylength = tay;
-        d->ReTay = ufluc * tay / nu;
-        d->Kollength = sqrt(sqrt(nu * nu * nu / eps));
-        if (isnan(t[0] + t[1])) {
-            d->nan_flag = 1;
-            if (*a.status > a.gate.it) *a.status = a.gate.it;
-        }
-    }
-}
-
 int launch_finalize(const FinArgs& a, hipStream_t s) {
-    if (a.nblk_spec <= kFin1MaxPartials && a.nblk_vg <= kFin1MaxPartials) {
-        const size_t shm = sizeof(double) * (2 * (size_t)a.allkmax + 12);
-        hipLaunchKernelGGL(finalize1_kernel, dim3(1, 1, zdim(a.nmem)), dim3(1024), shm, s, a);
-        return 0;
-    }
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(2 * a.allkmax + 12, 1, zdim(a.nmem)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(finalize_kernel, dim3(1, 1, zdim(a.nmem)), dim3(256), 0, s, a);
     return 0;
