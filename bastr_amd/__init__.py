@@ -48,6 +48,21 @@ def autotune(n, precision=abi.PREC_F64, ncols=None, nplanes=1):
     return (pts.value, wid.value), list(ms)
 
 
+def autotune_step(solver, iters=20):
+    """Time the column plans on the solver's (or ensemble's) own step, iters
+    iterations per plan from its current state, which is restored, and keep
+    the fastest (bastr_autotune_step): returns ((points per thread, width
+    multiplier), [ms per iteration of the plans (16, 1), (8, 1), (16, 2),
+    (8, 2)], NaN where absent)."""
+    L = load()
+    pts, wid = C.c_int32(), C.c_int32()
+    ms = (C.c_double * 4)()
+    rc = L.bastr_autotune_step(solver._h, int(iters), C.byref(pts), C.byref(wid), ms)
+    if rc:
+        raise BastrError(rc, L.bastr_last_error().decode())
+    return (pts.value, wid.value), list(ms)
+
+
 def tune_set(n, points, precision=abi.PREC_F64, width=1):
     L = load()
     rc = L.bastr_tune_set(int(n), int(precision), int(points), int(width))

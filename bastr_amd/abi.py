@@ -95,6 +95,7 @@ NOALLOC_SIGNATURES = [
     ("step_noalloc", C.c_int, [_P, C.c_int32, C.POINTER(Diag), _D]),
     ("autotune", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                            C.POINTER(C.c_int32), _D]),
+    ("autotune_step", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
     ("tune_set", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("tune_get", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 ]

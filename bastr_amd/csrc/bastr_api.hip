@@ -216,6 +216,7 @@ struct bastr_handle {
     uint64_t gen = 0, epoch = 0;
     uint64_t plan_epoch = 0;  // column radix plans the captured graph was made with
     bool use_graph = true;
+    bool tune_pending = false;  // BASTR_TUNE=1: tune the column plans at the first bastr_step
     // post-processing arena (bastr_pp_*), allocated on first use
     void* pp[30] = {};
     int* pp_status = nullptr;  // always INT_MAX: the post-processing gate stays open
@@ -1235,36 +1236,44 @@ int step_prepare(bastr_handle* h, int nsteps) {
     return 0;
 }
 
-// Time the two radix plans of the single-term column passes of length n
+// Time the radix plans of the single-term column passes of length n
 // (precision prec) on the current device, in the shape of a real pass: ncols
-// columns of each of nplanes planes, four jobs (as a stage's y-inverse), the
-// inverse and the forward transform, launched `reps` times between HIP events
-// after a warm-up launch; ms[v] for the plans (points, width) = (16, 1),
+// columns of each of nplanes planes, with the pruned passes' cut along the
+// pass (n/3), launched as an RK3 iteration of a 2-D step launches them (one
+// 7-job inverse, two 4-job inverses, three 2-job forwards).  Every launch
+// reads and writes arrays at other addresses than the launch before it (slots
+// of two buffers of at least 768 MiB), so the passes stream from HBM as a
+// step's do rather than from the 256 MB Infinity Cache, and each launch has the
+// real pass's workgroup count (a pass over more planes than the real one hides
+// how evenly its workgroups fill the CUs: 2048-point FP64 2-D forwards are 344
+// workgroups).  The mix runs `reps` times between HIP events after a warm-up
+// mix; ms[v] = milliseconds per mix for the plans (points, width) = (16, 1),
 // (8, 1), (16, 2), (8, 2), NaN where a plan does not exist.
 template <typename T>
 int time_plans(int n, int ncols, int nplanes, double* ms) {
     const int reps = 5;
-    // at least 768 MiB per buffer, so that the timed passes stream from HBM as a
-    // step's do (the 256 MB Infinity Cache would hold smaller buffers)
-    const size_t plane_bytes = sizeof(cx<T>) * (size_t)ncols * n;
-    nplanes = std::max<int>(nplanes, (int)(((size_t)768 << 20) / plane_bytes + 1));
-    nplanes = std::min<int>(nplanes, (int)(((size_t)1 << 28) / ((size_t)ncols * n)));
-    const size_t elems = (size_t)ncols * n * nplanes;
+    // one array of the pass; at most 2^25 elements so that 8 slots fit 2^28
+    size_t F = (size_t)ncols * n;
+    nplanes = std::max(1, std::min<int>(nplanes, (int)(((size_t)1 << 25) / F)));
+    F *= (size_t)nplanes;
+    const size_t fbytes = sizeof(cx<T>) * F;
+    const size_t nslot = std::max<size_t>(8, std::min<size_t>((((size_t)768 << 20) + fbytes - 1) / fbytes,
+                                                              ((size_t)1 << 28) / F));
+    const size_t elems = nslot * F;
     SyncLock lk(g_sync_mu);
-    void *in = nullptr, *out = nullptr, *out1 = nullptr, *tw = nullptr;
+    void *in = nullptr, *out = nullptr, *tw = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
     auto cleanup = [&]() {
         if (s) (void)hipStreamSynchronize(s);
-        for (void* p : {in, out, out1, tw})
+        for (void* p : {in, out, tw})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (s) (void)hipStreamDestroy(s);
     };
-    bool ok = hipMalloc(&in, sizeof(cx<T>) * elems) == hipSuccess && hipMalloc(&out, sizeof(cx<T>) * elems) == hipSuccess &&
-              hipMalloc(&out1, sizeof(cx<T>) * elems) == hipSuccess;
+    bool ok = hipMalloc(&in, sizeof(cx<T>) * elems) == hipSuccess && hipMalloc(&out, sizeof(cx<T>) * elems) == hipSuccess;
     ok = ok && hipMemset(in, 0, sizeof(cx<T>) * elems) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
@@ -1282,36 +1291,40 @@ int time_plans(int n, int ncols, int nplanes, double* ms) {
     }
     const int prec = sizeof(T) == 4 ? 1 : 0;
     const int keep_p = k3d::colx_points(n, prec), keep_w = k3d::colx_width(n, prec);
-    for (int v = 0; v < 4 && ok; ++v) {
-        ms[v] = std::nan("");
-        const int pts = (v & 1) ? 8 : 16, wid = (v & 2) ? 2 : 1;
-        if (!k3d::colx_plan_ok(n, prec, pts, wid) || ncols % k3d::colx_plan_cb(n, prec, pts, wid) != 0) continue;
-        k3d::set_colx_plan(n, prec, pts, wid);
+    // the mix: (jobs, inverse) per launch
+    const int mix[6][2] = {{7, 1}, {4, 1}, {4, 1}, {2, 0}, {2, 0}, {2, 0}};
+    size_t slot = 0;
+    auto launch = [&](int njob, bool inv) {
         k3d::ColXArgs<T> c{};
         c.gate = Gate{st, 0, 0};
-        c.in[0] = C<T>(in);
-        c.in[1] = C<T>(in);
-        c.out[0] = C<T>(out);
-        c.out[1] = C<T>(out1);
-        c.job[0] = job1(0, k3d::M_ONE, 0);
-        c.job[1] = job1(1, k3d::M_IK, 1);
-        c.job[2] = job1(0, k3d::M_IK, 0);
-        c.job[3] = job1(1, k3d::M_ONE, 1);
-        c.njob = 4;
+        c.in[0] = C<T>(in) + (slot % nslot) * F;
+        c.in[1] = C<T>(in) + ((slot + 1) % nslot) * F;
+        for (int q = 0; q < njob; ++q) {
+            c.out[q] = C<T>(out) + ((slot + q) % nslot) * F;
+            c.job[q] = job1(q & 1, inv && (q / 2) % 2 == 1 ? k3d::M_IK : k3d::M_ONE, q);
+        }
+        slot += 7;
+        c.njob = njob;
         c.pitch = ncols;
         c.ncols = ncols;
         c.plane_stride = (long)ncols * n;
         c.nplanes = nplanes;
         c.scale = 1.0;
+        c.kcut = n / 3 > 0 ? n / 3 : 0;
         c.tw = C<T>(tw);
         c.mstride = 0;
         c.nmem = 1;
-        k3d::ColXArgs<T> cf = c;  // the forward pass: the same jobs without multipliers
-        for (int q = 0; q < cf.njob; ++q) cf.job[q].mult[0] = k3d::M_ONE;
+        return k3d::launch_colx<T>(n, inv, c, s) == 0;
+    };
+    for (int v = 0; v < 4 && ok; ++v) {
+        ms[v] = std::nan("");
+        const int pts = (v & 1) ? 8 : 16, wid = (v & 2) ? 2 : 1;
+        if (!k3d::colx_plan_ok(n, prec, pts, wid) || ncols % k3d::colx_plan_cb(n, prec, pts, wid) != 0) continue;
+        k3d::set_colx_plan(n, prec, pts, wid);
         float t0 = 0.f;
         for (int r = 0; r <= reps && ok; ++r) {
             if (r == 1) ok = hipEventRecord(e0, s) == hipSuccess;
-            ok = ok && k3d::launch_colx<T>(n, true, c, s) == 0 && k3d::launch_colx<T>(n, false, cf, s) == 0;
+            for (int m = 0; m < 6 && ok; ++m) ok = launch(mix[m][0], mix[m][1] != 0);
         }
         ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
              hipEventElapsedTime(&t0, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
@@ -1403,6 +1416,97 @@ template <typename T>
 int step_impl(bastr_handle* h, int nsteps, bastr_diag* diags, double* spectra) {
     const int rc = step_launch<T>(h, nsteps);
     return rc ? rc : step_collect(h, nsteps, diags, spectra);
+}
+
+// The column plans timed on the handle's own step (bastr_autotune_step): each
+// plan valid for every column length of the handle runs `iters` iterations
+// from the handle's state, once untimed (graph capture, caches) and once
+// between HIP events on its stream, in two interleaved rounds (the faster
+// round counts); the spectral state is copied aside before and back after
+// every run, and nstep, time and the diagnostics of later calls are those of
+// a handle that was never tuned.  Keeps the default (16, 1) unless another
+// plan is at least 3 % faster.  ms[v]: milliseconds per iteration.
+template <typename T>
+int autotune_step_impl(bastr_handle* h, int iters, double* ms, int* best_out) {
+    int rc = 0;
+    if (h->prologue_pending) {
+        if ((rc = (h->nd == 3 ? run_prologue3<T>(h) : run_prologue<T>(h)))) return rc;
+        h->prologue_pending = false;
+    }
+    const int nd = h->nd, prec = h->prec;
+    const int lens[2] = {h->n1, nd == 3 ? h->n2 : 0};
+    const size_t bytes = h->nhalf * 2 * h->esz * (size_t)h->nmem;
+    bool ok = true;
+    void* snap[3] = {};
+    for (int c = 0; c < nd; ++c) snap[c] = dalloc(bytes, &ok);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    int keep_p[2], keep_w[2];
+    for (int i = 0; i < 2; ++i)
+        if (lens[i]) keep_p[i] = k3d::colx_points(lens[i], prec), keep_w[i] = k3d::colx_width(lens[i], prec);
+    const uint32_t prof = h->prof_mask;
+    h->prof_mask = 0;  // the tuning runs stay out of the per-kernel timings
+    auto copy_state = [&](bool save) {
+        for (int c = 0; c < nd && ok; ++c)
+            ok = hipMemcpyAsync(save ? snap[c] : h->U[c], save ? h->U[c] : snap[c], bytes, hipMemcpyDeviceToDevice,
+                                h->stream) == hipSuccess;
+    };
+    copy_state(true);
+    std::vector<int> st(h->nmem);
+    bool nan_seen = false;
+    for (int v = 0; v < 4; ++v) ms[v] = std::nan("");
+    for (int round = 0; round < 2 && ok && !rc && !nan_seen; ++round) {
+        for (int v = 0; v < 4 && ok && !rc && !nan_seen; ++v) {
+            const int pts = (v & 1) ? 8 : 16, wid = (v & 2) ? 2 : 1;
+            bool valid = true;
+            for (int i = 0; i < 2; ++i)
+                if (lens[i] && !k3d::colx_plan_ok(lens[i], prec, pts, wid)) valid = false;
+            if (!valid) continue;
+            for (int i = 0; i < 2; ++i)
+                if (lens[i]) k3d::set_colx_plan(lens[i], prec, pts, wid);
+            float tm = 0.f;
+            for (int run = 0; run < 2 && ok && !rc; ++run) {
+                copy_state(false);
+                if (run == 1) ok = ok && hipEventRecord(e0, h->stream) == hipSuccess;
+                if (ok) rc = step_launch<T>(h, iters);
+                if (run == 1) ok = ok && hipEventRecord(e1, h->stream) == hipSuccess;
+            }
+            ok = ok && !rc && hipMemcpyAsync(st.data(), h->status, sizeof(int) * h->nmem, hipMemcpyDeviceToHost,
+                                             h->stream) == hipSuccess;
+            ok = ok && hipStreamSynchronize(h->stream) == hipSuccess && hipEventElapsedTime(&tm, e0, e1) == hipSuccess;
+            for (int m = 0; m < h->nmem && ok; ++m) nan_seen = nan_seen || st[m] != INT_MAX;
+            const double per = (double)tm / iters;
+            if (ok && !rc && !nan_seen && (std::isnan(ms[v]) || per < ms[v])) ms[v] = per;
+        }
+    }
+    copy_state(false);
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    h->prof_mask = prof;
+    for (int c = 0; c < nd; ++c)
+        if (snap[c]) {
+            SyncLock lk(g_sync_mu);
+            (void)hipFree(snap[c]);
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    int best = 0;
+    if (ok && !rc && !nan_seen && !std::isnan(ms[0]))
+        for (int v = 1; v < 4; ++v)
+            if (!std::isnan(ms[v]) && ms[v] < 0.97 * ms[0] && ms[v] < ms[best]) best = v;
+    const bool good = ok && !rc && !nan_seen && !std::isnan(ms[0]);
+    for (int i = 0; i < 2; ++i)
+        if (lens[i]) {
+            if (good) k3d::set_colx_plan(lens[i], prec, (best & 1) ? 8 : 16, (best & 2) ? 2 : 1);
+            else k3d::set_colx_plan(lens[i], prec, keep_p[i], keep_w[i]);
+        }
+    if (rc) return rc;
+    if (!ok) return fail(BASTR_EDEVICE, "bastr_autotune_step: a device call failed while timing the plans");
+    if (nan_seen)
+        return fail(BASTR_EINVAL, "bastr_autotune_step: the field reached a NaN iteration within %d iterations "
+                                  "(tune on an earlier state or with fewer iterations)", iters);
+    if (std::isnan(ms[0])) return fail(BASTR_EDEVICE, "bastr_autotune_step: the default plan was not timed");
+    *best_out = best;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1722,23 +1826,11 @@ static int create_impl(const bastr_params* p, int nmem, const int* forced, bastr
     h->npts = (size_t)h->n0 * h->n1 * h->n2;
     h->nhalf = (size_t)h->hp * h->n1 * h->n2;
     {
-        // "1": time both radix plans of each column length this handle uses, once per
-        // length and precision in the process (bastr_autotune), before its first step
-        static std::mutex tuned_mu;
-        static std::vector<int> tuned;
+        // "1": the first bastr_step call tunes the column plans of this handle's
+        // lengths on its own step (bastr_autotune_step), once per length and
+        // precision in the process
         const char* tv = std::getenv("BASTR_TUNE");
-        if (tv && std::strcmp(tv, "1") == 0) {
-            // the pass shape: the step's kept columns, every plane of a 3-D field up to 64
-            const int ncols = (h->prune ? h->cols : h->hp) / 16 * 16;
-            const int planes = nd == 3 ? std::min(h->n1, 64) : 1;
-            for (int len : {h->n1, nd == 3 ? h->n2 : 0}) {
-                if (!len || ncols < 16) continue;
-                std::lock_guard<std::mutex> tl(tuned_mu);
-                const int key = len * 2 + h->prec;
-                if (std::find(tuned.begin(), tuned.end(), key) != tuned.end()) continue;
-                if (bastr_autotune(len, h->prec, ncols, planes, nullptr, nullptr, nullptr) == 0) tuned.push_back(key);
-            }
-        }
+        h->tune_pending = tv && std::strcmp(tv, "1") == 0;
     }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
@@ -2176,6 +2268,20 @@ int bastr_step(bastr_handle* h, int32_t nsteps, bastr_diag* diags, double* spect
     if (nsteps < 0) return fail(BASTR_EINVAL, "bastr_step: nsteps < 0");
     if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_step: fields not set");
     if (nsteps == 0) return BASTR_OK;
+    if (h->tune_pending) {
+        // BASTR_TUNE=1: once per column length and precision in the process; a
+        // failed tuning (a NaN within its iterations) leaves the plans as they were
+        static std::mutex tuned_mu;
+        static std::vector<int> tuned;
+        h->tune_pending = false;
+        std::lock_guard<std::mutex> tl(tuned_mu);
+        const int key = h->n1 * 2 + h->prec;
+        if (std::find(tuned.begin(), tuned.end(), key) == tuned.end()) {
+            int32_t pts, wid;
+            double ms[4];
+            if (bastr_autotune_step(h, h->nd == 3 ? 2 : 8, &pts, &wid, ms) == BASTR_OK) tuned.push_back(key);
+        }
+    }
     return h->prec == BASTR_PREC_F32 ? step_impl<float>(h, nsteps, diags, spectra)
                                      : step_impl<double>(h, nsteps, diags, spectra);
 }
@@ -2281,6 +2387,23 @@ int bastr_autotune(int32_t n, int32_t precision, int32_t ncols, int32_t nplanes,
     k3d::set_colx_plan(n, precision, pts, wid);
     if (points) *points = pts;
     if (width) *width = wid;
+    if (ms)
+        for (int v = 0; v < 4; ++v) ms[v] = t[v];
+    return BASTR_OK;
+}
+
+int bastr_autotune_step(bastr_handle* h, int32_t iters, int32_t* points, int32_t* width, double* ms) {
+    if (!h) return fail(BASTR_EINVAL, "bastr_autotune_step: NULL handle");
+    if (!h->have_fields) return fail(BASTR_EINVAL, "bastr_autotune_step: fields not set");
+    if (iters < 1 || iters > kMaxGraphSteps)
+        return fail(BASTR_EINVAL, "bastr_autotune_step: iters = %d not in [1, %d]", iters, kMaxGraphSteps);
+    double t[4];
+    int best = 0;
+    const int rc = h->prec == BASTR_PREC_F32 ? autotune_step_impl<float>(h, iters, t, &best)
+                                             : autotune_step_impl<double>(h, iters, t, &best);
+    if (rc) return rc;
+    if (points) *points = (best & 1) ? 8 : 16;
+    if (width) *width = (best & 2) ? 2 : 1;
     if (ms)
         for (int v = 0; v < 4; ++v) ms[v] = t[v];
     return BASTR_OK;

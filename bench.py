@@ -780,13 +780,18 @@ def main():
             n_col = cg["grid"][1]
             key = f"{n_col} {'fp32' if cg['prec'] == 1 else 'fp64'}"
             if key not in tune:
-                # the workload's pass shape: its kept columns (cut rounded up to 16), and
-                # 64 of a 3-D field's planes
-                ncols = (cg["grid"][0] // 3 + 15) // 16 * 16
-                (pts, wid), ms = bastr_amd.autotune(n_col, cg["prec"], ncols, 64 if len(cg["grid"]) == 3 else 1)
+                # timed on the workload's own step (bastr_autotune_step): a handle
+                # holding the workload's field, a few iterations in, every plan run
+                # from that state (restored after each run)
+                t_h = new_field(bastr_amd, cfg_params(nm))
+                nd3 = len(cg["grid"]) == 3
+                run_steps(t_h, 2 if nd3 else 5, 5)
+                (pts, wid), ms = bastr_amd.autotune_step(t_h, 3 if nd3 else 20)
+                t_h.close()
                 tune[key] = {"points_per_thread": pts, "width": wid,
-                             "ms": {f"{p}x{w}": (None if m != m else round(m, 4))
-                                    for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)}}
+                             "ms_per_iteration": {f"{p}x{w}": (None if m != m else round(m, 4))
+                                                  for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)},
+                             "timed_on": f"{nm}: {3 if nd3 else 20} iterations per plan and round, two rounds"}
         print(f"  autotune: {tune}", file=sys.stderr, flush=True)
 
     rec, last = measure(lib, bastr_amd, name, steps, args.warmup, args.chunk, copy=True)

@@ -220,20 +220,39 @@ int bastr_pp_transfer(bastr_handle* h, const double* u1, const double* u2, const
  * are remade).
  * bastr_autotune times the plans of length n on the current device in the
  * shape of a real pass (ncols columns, a multiple of 16, of each of nplanes
- * planes, more planes where that is under 768 MiB per buffer so that the pass
- * streams from HBM; four jobs, inverse and forward) and keeps the fastest
+ * planes, entries past n/3 along the pass pruned as the step prunes them),
+ * launched as one 2-D RK3 iteration launches its column passes (a 7-job and
+ * two 4-job inverses, three 2-job forwards), each launch on arrays the one
+ * before it did not touch so that it streams from HBM, and keeps the fastest
  * (the default plan unless another is at least 3 % faster);
- * ms[0..3]: the plans (points, width) = (16, 1), (8, 1), (16, 2), (8, 2), NaN
+ * ms[0..3]: milliseconds per such mix for the plans (points, width) =
+ * (16, 1), (8, 1), (16, 2), (8, 2), NaN
  * where a plan does not exist (8 points below n = 256; a plan needing more than
  * 1024 threads or 160 KiB of LDS per workgroup).  Setting BASTR_TUNE=1 in the
- * environment makes bastr_create tune each column length of the handle once
- * per process, in the handle's own pass shape.  bastr_tune_set fixes a plan,
+ * environment at bastr_create makes the handle's first bastr_step call run
+ * bastr_autotune_step (8 iterations per plan in 2-D, 2 in 3-D) once per
+ * column length and precision in the process.  bastr_tune_set fixes a plan,
  * bastr_tune_get reads the one in use.  The 8-point plans differ in rounding
  * (within the parity tolerances); the block width changes no arithmetic.  A
  * fixed table gives reproducible bits.
  */
 int bastr_autotune(int32_t n, int32_t precision, int32_t ncols, int32_t nplanes, int32_t* points, int32_t* width,
                    double* ms);
+/*
+ * bastr_autotune_step times the same four plans on the handle's own step: for
+ * each plan valid for every column length of the handle (ja, and ka in 3-D),
+ * `iters` iterations (1..256) from the handle's current state, once untimed
+ * and once between HIP events, in two interleaved rounds; the spectral state
+ * is restored after every run, so nstep, time and the results of later calls
+ * are those of an untuned handle (the fields must be set).  It keeps the
+ * fastest plan for those lengths (the default unless another is at least 3 %
+ * faster); ms[0..3]: milliseconds per iteration, as for bastr_autotune.  A
+ * pass timed alone misses what the step around it does to it (the caches the
+ * previous kernel leaves, how evenly a pass of one plane fills the CUs), so
+ * this is the tuner to use where a handle exists.  Fails with BASTR_EINVAL,
+ * leaving the plans as they were, if a run reaches a NaN iteration.
+ */
+int bastr_autotune_step(bastr_handle* h, int32_t iters, int32_t* points, int32_t* width, double* ms);
 int bastr_tune_set(int32_t n, int32_t precision, int32_t points, int32_t width);
 int bastr_tune_get(int32_t n, int32_t precision, int32_t* points, int32_t* width);
 

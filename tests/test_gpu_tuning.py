@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(autouse=True)
 def default_plans():
     yield
-    for n in (256, 512, 1024, 2048):
+    for n in (32, 64, 128, 256, 512, 1024, 2048):
         for prec in (0, 1):
             bastr_amd.tune_set(n, 16, prec)
 
@@ -125,3 +125,85 @@ def test_group_graph_with_a_replaced_handle(hip):
         c2.step(5)
     for x, y in ((a, a2), (c, c2)):
         assert all(np.array_equal(f, g) for f, g in zip(x.get_fields(), y.get_fields()))
+
+
+@pytest.mark.parametrize("shape,prec", [((256, 256), 0), ((512, 512), 1), ((32, 32, 32), 0)])
+def test_autotune_step_restores_the_state(hip, shape, prec):
+    """bastr_autotune_step times the plans on the handle's own step and puts
+    the state back: the handle then steps exactly as a fresh handle with the
+    chosen plan set by hand (fields, diagnostics, nstep, time)."""
+    print(f"case autotune_step {shape} prec={prec}", flush=True)
+    nd = len(shape)
+    p = make_params(*shape, forcemethod=2, lproject=True, kmax=128, precision=prec)
+    u = mixed_field(shape, seed=41) if nd == 2 else hitgen(shape, k0=4, seed=41)
+    lens = sorted(set(shape[1:]))
+    for n in lens:
+        bastr_amd.tune_set(n, 16, prec)
+    a = Solver(hip, "bastr", p)
+    a.set_fields(*u)
+    a.step(3)
+    plan, ms = bastr_amd.autotune_step(a, iters=6)
+    print("autotune_step", shape, prec, plan, ms, flush=True)
+    assert np.isfinite(ms[0]) and ms[0] > 0
+    times = {q: t for q, t in zip(PLANS, ms) if np.isfinite(t)}
+    fast = {q: t for q, t in times.items() if t < 0.97 * times[(16, 1)]}
+    assert plan == (min(fast, key=fast.get) if fast else (16, 1))
+    for n in lens:
+        assert bastr_amd.tune_get(n, prec) == plan
+    ra, da, sa = a.step(4)
+    # the same calls on a fresh handle: 3 iterations with the default plan, 4 with the chosen one
+    for n in lens:
+        bastr_amd.tune_set(n, 16, prec)
+    b = Solver(hip, "bastr", p)
+    b.set_fields(*u)
+    b.step(3)
+    for n in lens:
+        bastr_amd.tune_set(n, plan[0], prec, plan[1])
+    rb, db, sb = b.step(4)
+    assert ra == 0 and rb == 0
+    assert all(np.array_equal(x, y) for x, y in zip(a.get_fields(), b.get_fields()))
+    assert [(d.nstep, d.time, d.Esspe, d.Edspe) for d in da] == [(d.nstep, d.time, d.Esspe, d.Edspe) for d in db]
+    assert np.array_equal(sa, sb)
+    a.close()
+    b.close()
+
+
+def test_autotune_step_on_an_ensemble(hip):
+    print("case autotune_step ensemble 3 x 128^2", flush=True)
+    p = make_params(128, 128, forcemethod=2, lproject=True, kmax=64)
+    e = bastr_amd.ensemble(p, 3, forced=[1, 0, 1])
+    f = bastr_amd.ensemble(p, 3, forced=[1, 0, 1])
+    for m in range(3):
+        u = hitgen((128, 128), k0=3 + m, seed=50 + m)
+        e.set_member_fields(m, *u)
+        f.set_member_fields(m, *u)
+    bastr_amd.tune_set(128, 16)
+    assert e.step(1)[0] == 0 and f.step(1)[0] == 0  # both entry transforms with the default plan
+    plan, ms = bastr_amd.autotune_step(e, iters=4)
+    print("autotune_step ensemble", plan, ms, flush=True)
+    assert np.isfinite(ms[0])
+    assert bastr_amd.tune_get(128, 0) == plan
+    re, de, se = e.step(3)
+    rf, df, sf = f.step(3)
+    assert re == 0 and rf == 0
+    for m in range(3):
+        assert all(np.array_equal(x, y) for x, y in zip(e.get_member_fields(m), f.get_member_fields(m)))
+    assert np.array_equal(se, sf)
+    e.close()
+    f.close()
+
+
+def test_autotune_step_refuses_a_nan_run(hip):
+    """u = 0 with band forcing reaches NaN in its first iteration: the tuner
+    fails and leaves the plans as they were."""
+    print("case autotune_step NaN", flush=True)
+    n = 256
+    p = make_params(n, n, forcemethod=2, lproject=False, kmax=32)
+    g = Solver(hip, "bastr", p)
+    g.set_fields(np.zeros((n, n), order="F"), np.zeros((n, n), order="F"))
+    bastr_amd.tune_set(n, 8)
+    with pytest.raises(bastr_amd.BastrError):
+        bastr_amd.autotune_step(g, iters=2)
+    assert bastr_amd.tune_get(n, 0) == (8, 1)
+    g.close()
+    bastr_amd.tune_set(n, 16)
