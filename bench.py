@@ -230,6 +230,18 @@ def bench_ensemble(args):
 
     groups = build_groups()
     total_pts = sum(n * n * M for n, M in ENSEMBLE)
+    # column plans of each size, timed on its own ensemble handle's step
+    # (bastr_autotune_step; the state is restored, the timed run uses fresh handles)
+    tune = {}
+    if not args.no_tune:
+        for g in groups:
+            (pts, wid), ms = bastr_amd.autotune_step(g["ens"], 10)
+            tune[f"{g['n']} fp32"] = {
+                "points_per_thread": pts, "width": wid,
+                "ms_per_iteration": {f"{p}x{w}": (None if m != m else round(m, 4))
+                                     for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)},
+                "timed_on": f"the {g['M']} x {g['n']}^2 handle alone: 10 iterations per plan and round, two rounds"}
+        print(f"  autotune: {tune}", file=sys.stderr, flush=True)
 
     # each size's handle has its own HIP stream: bastr_step_group launches the
     # three sizes' iterations before waiting for any, so their kernels overlap
@@ -361,6 +373,7 @@ def bench_ensemble(args):
         "dtype": "f32",
         "data": "synthetic: seeded random-phase fields, E(k) = k^4 exp(-2 (k/k0)^2) (pure law), k0 uniform in "
                 "{3..8}, one seed per member, u_rms=1, built on the device",
+        "autotune": tune or None,
         "config": {"workload": "ensemble of 96 independent 2-D Burgers runs: 48 x 256^2, 32 x 512^2, 16 x 1024^2 "
                                "(28,311,552 points), Re=5000, dt=1e-4, RK3, band forcing k=5 E_t=0.075 on every "
                                "other member, free decay on the rest, lproject=t, spectra kmax=n/2",
@@ -498,6 +511,20 @@ def bench_closure(args):
     f = fine_run(fp, 21, 4.0)
     f.step(800, want_spectra=False)
     steps = 150
+    # column plans timed on both runs' own steps (bastr_autotune_step; states restored)
+    tune = {}
+    if not args.no_tune:
+        c = bastr_amd.solver(cp)
+        filter_to(f, c)
+        net.attach(c)
+        Closure.per_step(c, True)
+        for key, h in (("256 fp64 (closure run)", c), ("2048 fp64 (fine run)", f)):
+            (pts, wid), ms = bastr_amd.autotune_step(h, 20)
+            tune[key] = {"points_per_thread": pts, "width": wid,
+                         "ms_per_iteration": {f"{p}x{w}": (None if m != m else round(m, 4))
+                                              for (p, w), m in zip([(16, 1), (8, 1), (16, 2), (8, 2)], ms)}}
+        c.close()
+        print(f"  autotune: {tune}", file=sys.stderr, flush=True)
     t_cad, rc_cad = {}, {}
     for per in (True, False):
         c = bastr_amd.solver(cp)
@@ -567,6 +594,7 @@ def bench_closure(args):
         "dtype": "f64",
         "data": "synthetic: pure-law random-phase fields E(k) = k^4 exp(-2 (k/4)^2), u_rms=1; training seeds 1-20, "
                 "held-out seeds 21-25",
+        "autotune": tune or None,
         "config": {"workload": "learned closure: coarse 2-D 256^2 Burgers, dt=4e-4, Re=5000, band forcing k=5 "
                                "E_t=0.075, lproject=t, with a 36-64-64-2 GELU MLP (BF16 MFMA) trained on filtered "
                                "(k_c=85) 2048^2 runs of this solver",
