@@ -817,9 +817,9 @@ k3d::Geo3<T> geo3(const bastr_handle* h) {
     return g;
 }
 
-// z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: ZL_c = -(k2^2 + k3^2) u_c and
-// ZD = -i (k2^2 + k3^2)(k2 u2 + k3 u3), the y and z parts of Lap(u_c) and of
-// d2 Lap(u2) + d3 Lap(u3); k2 is the column's own wavenumber)
+// z-inverse of src[3]: Z_c, Z3_c = i k3 (and with vg: ZL_c = -(k1^2 + k2^2 + k3^2) u_c
+// and ZD = -i (k1^2 + k2^2 + k3^2)(k2 u2 + k3 u3), Lap(u_c) and d2 Lap(u2) +
+// d3 Lap(u3); k1 and k2 are the column's own wavenumbers)
 template <typename T>
 int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 plain 1 stage 2 vg*/, bool prune) {
     k3d::ColXArgs<T> c{};
@@ -838,8 +838,9 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
         d.in[1] = 2; d.mult[1] = k3d::M_LAP_IK;
         d.out = 9;
         c.job[nj++] = d;
-        c.kc_div = h->hp;
+        c.kc_div = h->hp;  // the column offset is ky hp + kx
         c.kc_n = h->n1;
+        c.kx_mod = h->hp;
     }
     c.njob = nj;
     c.pitch = h->hp * h->n1;
@@ -863,8 +864,7 @@ int run_zinv3(bastr_handle* h, const Gate& g, void* const* src, int mode /*0 pla
 }
 
 // y-inverse of the z-inverse outputs into W: Y_c, Y2_c = i k2 Y, Y3_c (and with
-// vg: L_c = y-part of -(k2^2 + k3^2) u_c, D = y-part of -(k2^2+k3^2)(i k2 u2 + i k3 u3),
-// the y-inverses of ZL_c and ZD)
+// vg: L_c = Lap(u_c), D = d2 Lap(u2) + d3 Lap(u3), the y-inverses of ZL_c and ZD)
 template <typename T>
 int run_yinv3(bastr_handle* h, const Gate& g, int mode /*0 plain 1 stage 2 vg*/, bool prune, int out0 = 0) {
     k3d::ColXArgs<T> c{};

@@ -22,7 +22,8 @@ using k2d::RowCfg;
 // powers, the Hermitian part of a c2c real-part inverse).
 // Multipliers of a column-pass term, k the pass wavenumber; the LAP forms also
 // use the column's own wavenumber kc (a z pass, whose columns sit at fixed ky):
-//   M_LAP = -(kc^2 + k^2), M_LAP_IKC = -i (kc^2 + k^2) kc, M_LAP_IK = -i (kc^2 + k^2) k.
+//   M_LAP = -(kc^2 + k^2), M_LAP_IKC = -i (kc^2 + k^2) kc, M_LAP_IK = -i (kc^2 + k^2) k
+//   (kc^2 + kx^2 in place of kc^2 where kx_mod is set).
 // Odd powers of a wavenumber vanish at that axis's Nyquist index.
 enum Mult { M_ONE = 0, M_IK = 1, M_MK2 = 2, M_MIK3 = 3, M_LAP = 4, M_LAP_IKC = 5, M_LAP_IK = 6 };
 
@@ -46,6 +47,8 @@ struct ColXArgs {
     int nplanes;
     double scale;        // applied to every output (1: none)
     int kc_div, kc_n;    // LAP multipliers: kc = ksigned(column offset / kc_div, kc_n) (kc_div 0: unused)
+    int kx_mod;          // > 0: LAP multipliers also take kx^2, kx = column offset mod kx_mod (a 3-D
+                         // z pass over (kx, ky) columns: the full Laplacian)
     int kcut;            // > 0: entries with |k| >= kcut along the pass are zero: an inverse
                          // pass does not read them, a forward pass does not write them
     int pskip_at, pskip; // plane p >= pskip_at is plane p + pskip (0, 0: planes in order)

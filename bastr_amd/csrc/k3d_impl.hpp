@@ -40,15 +40,16 @@ using k2d::xcd_block;
 // constant.  Every product is of integer-valued operands, exact or rounded
 // once, so each factor equals the direct formula's:
 //   M_ONE 1; M_IK i ko; M_MK2 -k^2; M_MIK3 -i ko^3 = i (-k^2) ko;
-//   M_LAP -(kc^2 + k^2); M_LAP_IKC -i (kc^2 + k^2) kco; M_LAP_IK -i (kc^2 + k^2) ko.
+//   M_LAP -(kc^2 + k^2); M_LAP_IKC -i (kc^2 + k^2) kco; M_LAP_IK -i (kc^2 + k^2) ko
+// (kc^2 + kx^2 for kc^2 with a column kx: the full 3-D Laplacian).
 template <typename T>
 struct MultCoef {
     T c0, c2, g;
     bool useko, usei;
 };
 template <typename T>
-__device__ __forceinline__ MultCoef<T> mult_coef(int mult, T kc, T kco) {
-    const T kc2 = -(kc * kc);
+__device__ __forceinline__ MultCoef<T> mult_coef(int mult, T kc, T kco, T kx = 0) {
+    const T kc2 = -(kc * kc + kx * kx);
     switch (mult) {
         case M_IK: return {(T)1, (T)0, (T)1, true, true};
         case M_MK2: return {(T)0, (T)-1, (T)1, false, false};
@@ -113,11 +114,13 @@ void colx_kernel(ColXArgs<T> a) {
     const LdsLine<T, true> L{lds + l, CB};
     const ColJob& jb = a.job[q];
     // the column's own wavenumber (LAP multipliers only)
-    T kc = 0, kco = 0;
+    T kc = 0, kco = 0, kx = 0;
     if (a.kc_div > 0) {
-        const unsigned ic = (unsigned)((size_t)plane * a.plane_stride + (size_t)cb * CB + l) / (unsigned)a.kc_div;
+        const unsigned co = (unsigned)((size_t)plane * a.plane_stride + (size_t)cb * CB + l);
+        const unsigned ic = co / (unsigned)a.kc_div;
         kc = (T)ksigned((int)ic, a.kc_n);
         kco = (2 * (int)ic == a.kc_n) ? (T)0 : kc;
+        if (a.kx_mod > 0) kx = (T)(co % (unsigned)a.kx_mod);
     }
     // entries |k| >= kcut along the pass: zero in an inverse pass's inputs,
     // not stored by a forward pass
@@ -135,7 +138,7 @@ void colx_kernel(ColXArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < E; ++k) v[k] = k2d::buf_load<T>(r, INV ? off(k) : at(k));
         if (INV && mult != M_ONE) {  // (forward jobs take no multiplier: colx_n)
-            const MultCoef<T> c = mult_coef<T>(mult, kc, kco);
+            const MultCoef<T> c = mult_coef<T>(mult, kc, kco, kx);
 #pragma unroll
             for (int k = 0; k < E; ++k) v[k] = apply_mult<T, N>(v[k], c, j + k * P);
         }
@@ -148,7 +151,7 @@ void colx_kernel(ColXArgs<T> a) {
             __builtin_amdgcn_sched_barrier(0);
             const auto r = __builtin_amdgcn_make_buffer_rsrc((void*)(a.in[jb.in[t]] + blk0), (short)0,
                                                              (int)k2d::kBufRange, k2d::kBufWord3);
-            const MultCoef<T> c = mult_coef<T>(jb.mult[t], kc, kco);
+            const MultCoef<T> c = mult_coef<T>(jb.mult[t], kc, kco, kx);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const cx<T> x = k2d::buf_load<T>(r, INV ? off(k) : at(k));
@@ -694,19 +697,13 @@ void vg3_kernel(Vg3Args<T> a) {
     if constexpr (DIS) {
         // the dissipation moment, sum 2 theta u_i Lap(u_i) + |u|^2 Lap(theta), from
         // u and theta above and two more x transforms: (Lap u1, Lap u2) and
-        // (Lap u3, Lap theta), Lap(u_c) = -k1^2 Y_c + L_c, Lap(theta) =
-        // -k1^2 theta + i k1 L_1 + D with theta = i k1 Y_1 + Y2_2 + Y3_3 (the
-        // rows already read come again from the L2, through laundered pointers
-        // so the compiler does not keep the first loads' values live)
+        // (Lap u3, Lap theta), with Lap(u_c) = L_c (the column passes' full
+        // Laplacian) and Lap(theta) = i k1 L_1 + D: rows not read before
         __builtin_amdgcn_sched_barrier(0);
-        auto Wd = [&](int q) { return Row(k2d::launder(a.w[q]), ro, lim); };
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
-            const int m = j + k * P;
-            const int h = m <= N / 2 ? m : N - m;
-            const T q = -((T)h * (T)h);
-            w[k] = pack_vals<T, N>(cadd(cscale(Wd(0)[h], q), W(9)[h]), cadd(cscale(Wd(1)[h], q), W(10)[h]), m);
+            w[k] = pack_rows<T, N>(W(9), W(10), j + k * P);
         }
         __builtin_amdgcn_sched_barrier(0);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
@@ -715,18 +712,15 @@ void vg3_kernel(Vg3Args<T> a) {
         for (int k = 0; k < E; ++k)
             s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
                      2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
+        auto L1 = Row(k2d::launder(a.w[9]), ro, lim);  // read again, from the L2
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
             const int m = j + k * P;
             const int h = m <= N / 2 ? m : N - m;
-            const T k1 = (T)h;
-            const T k1o = (h == N / 2) ? (T)0 : k1;
-            const T q = -(k1 * k1);
-            const cx<T> l1 = W(9)[h];
-            const cx<T> thh = cadd(cscale(cmuli(Wd(0)[h]), k1o), cadd(Wd(4)[h], Wd(8)[h]));
-            const cx<T> lth = cadd(cadd(cscale(thh, q), cscale(cmuli(l1), k1o)), W(12)[h]);
-            w[k] = pack_vals<T, N>(cadd(cscale(Wd(2)[h], q), W(11)[h]), lth, m);
+            const T k1o = (h == N / 2) ? (T)0 : (T)h;
+            const cx<T> lth = cadd(cscale(cmuli(L1[h]), k1o), W(12)[h]);
+            w[k] = pack_vals<T, N>(W(11)[h], lth, m);
         }
         __builtin_amdgcn_sched_barrier(0);
         line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u3), Lap(theta)
@@ -768,25 +762,22 @@ void vg3_dis_kernel(Vg3Args<T> a) {
     auto W = [&](int q) { return Row(a.w[q], ro, lim); };
     // one load phase: each of the nine input half rows is read once and feeds
     // the packed transforms (u1, u2), (u3, theta), (Lap u1, Lap u2),
-    // (Lap u3, Lap theta); theta = i k1 Y_1 + Y2_2 + Y3_3,
-    // Lap(theta) = -k1^2 theta + i k1 L_1 + D, Lap(u_c) = -k1^2 Y_c + L_c
+    // (Lap u3, Lap theta); theta = i k1 Y_1 + Y2_2 + Y3_3, Lap(u_c) = L_c (the
+    // column passes' full Laplacian), Lap(theta) = i k1 L_1 + D
     cx<T> z[E], t[E], w[E], v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         __builtin_amdgcn_sched_barrier(0);
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
-        const T k1 = (T)h;
-        const T k1o = (h == N / 2) ? (T)0 : k1;
-        const T q = -(k1 * k1);
+        const T k1o = (h == N / 2) ? (T)0 : (T)h;
         const cx<T> y1 = W(0)[h], y2 = W(1)[h], y3 = W(2)[h];
         const cx<T> th = cadd(cscale(cmuli(y1), k1o), cadd(W(4)[h], W(8)[h]));
         const cx<T> l1 = W(9)[h];
         z[k] = pack_vals<T, N>(y1, y2, m);
         t[k] = pack_vals<T, N>(y3, th, m);
-        w[k] = pack_vals<T, N>(cadd(cscale(y1, q), l1), cadd(cscale(y2, q), W(10)[h]), m);
-        const cx<T> lth = cadd(cadd(cscale(th, q), cscale(cmuli(l1), k1o)), W(12)[h]);
-        v[k] = pack_vals<T, N>(cadd(cscale(y3, q), W(11)[h]), lth, m);
+        w[k] = pack_vals<T, N>(l1, W(10)[h], m);
+        v[k] = pack_vals<T, N>(W(11)[h], cadd(cscale(cmuli(l1), k1o), W(12)[h]), m);
     }
     __builtin_amdgcn_sched_barrier(0);
     line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
