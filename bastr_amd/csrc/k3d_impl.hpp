@@ -593,13 +593,12 @@ using Vg3Cfg = k2d::VgCfg<T, N, 8>;
 // (b23, b32) for the symmetric terms, then (b22, b33) completes theta with
 // b11 from the (u3, b11) transform.  Only u, b11 and later theta are held
 // across transforms, so 8 points per thread fit in registers and a 512-point
-// line is one wavefront.  DIS (FP64): the dissipation moment follows from u,
-// theta and two more transforms (Lap u and Lap theta), so every moment of the
-// iteration comes from this one pass (at 512^3 3.82 ms against 2.25 + 1.77
-// ms for two passes); in FP32 the two-pass form is faster (2.52 against
-// 2.74 ms: the merged kernel's registers allow 3 waves per SIMD, the two
-// passes 4 and 5) and vg3_dis_kernel adds the dissipation moment.
-template <typename T, int N, bool DIS>
+// line is one wavefront.  The dissipation moment follows from u, theta and two
+// more transforms (Lap u and Lap theta, from rows the column passes formed
+// with the full Laplacian), so every moment of the iteration comes from this
+// one pass, each input row read once (512^3: 3.20 ms FP64, 2.37 ms FP32,
+// against two passes that both read u: 3.72 ms FP64, 2.47 ms FP32).
+template <typename T, int N>
 __global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
 void vg3_kernel(Vg3Args<T> a) {
     using C = Vg3Cfg<T, N>;
@@ -670,14 +669,14 @@ void vg3_kernel(Vg3Args<T> a) {
 #pragma unroll
     for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(W(4), W(8), j + k * P);
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d2 u2, d3 u3
-    T th[DIS ? E : 1];  // theta at the thread's points, for the dissipation moment
+    T th[E];  // theta at the thread's points, for the dissipation moment
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
         const double b11k = b11[k], b22 = w[k].x, b33 = w[k].y;
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
         const double div = b11k + b22 + b33;
-        if constexpr (DIS) th[k] = (T)div;
+        th[k] = (T)div;
         s_um2 += div * div * uu;
         s_ijji += (b11k * b11k + b22 * b22 + b33 * b33) * uu;
         s_u2t += div * uu;
@@ -694,110 +693,44 @@ void vg3_kernel(Vg3Args<T> a) {
         k2d::vg_reduce<C::THREADS, 3>(v0, red, 0, a.part);
         k2d::vg_reduce<C::THREADS, 4>(v1, red, 4, a.part);
     }
-    if constexpr (DIS) {
-        // the dissipation moment, sum 2 theta u_i Lap(u_i) + |u|^2 Lap(theta), from
-        // u and theta above and two more x transforms: (Lap u1, Lap u2) and
-        // (Lap u3, Lap theta), with Lap(u_c) = L_c (the column passes' full
-        // Laplacian) and Lap(theta) = i k1 L_1 + D: rows not read before
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
-            w[k] = pack_rows<T, N>(W(9), W(10), j + k * P);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
-        double s_dis = 0;
-#pragma unroll
-        for (int k = 0; k < E; ++k)
-            s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
-                     2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
-        auto L1 = Row(k2d::launder(a.w[9]), ro, lim);  // read again, from the L2
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
-            const int m = j + k * P;
-            const int h = m <= N / 2 ? m : N - m;
-            const T k1o = (h == N / 2) ? (T)0 : (T)h;
-            const cx<T> lth = cadd(cscale(cmuli(L1[h]), k1o), W(12)[h]);
-            w[k] = pack_vals<T, N>(W(11)[h], lth, m);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u3), Lap(theta)
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
-            const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-            s_dis += 2.0 * uu3 * (double)th[k] * (double)w[k].x + uu * (double)w[k].y;
-        }
-        if (!valid) s_dis = 0;
-        double vd[1] = {s_dis};
-        k2d::vg_reduce<C::THREADS, 1>(vd, red, 3, a.part);
-    }
-}
-
-// The dissipation moment of velgrad_calculate3D (solution.F90:1188-1286),
-//   sum 2 u_i theta Lap(u_i) + |u|^2 Lap(theta),
-// in its own pass (part[3 * gridDim.x + block]): u, (u3, theta),
-// (Lap u1, Lap u2), (Lap u3, Lap theta).
-template <typename T, int N>
-__global__ __launch_bounds__((Vg3Cfg<T, N>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
-void vg3_dis_kernel(Vg3Args<T> a) {
-    using C = Vg3Cfg<T, N>;
-    constexpr int E = C::E, P = C::P, LR = C::L;
-    if (a.gate.closed()) return;
-    __shared__ cx<T> lds[LR * lds_line_cx(N)];
-    __shared__ double red[C::THREADS / 64][8];
-    const int tid = threadIdx.x;
-    const int l = tid / P, j = tid % P;
-    const int nrows = a.g.n1 * a.g.n2;
-    const int row0 = blockIdx.x * LR + l;
-    const bool valid = row0 < nrows;
-    const int row = valid ? row0 : nrows - 1;
-    const LdsLine<T, false, (P <= 64)> Ls{lds + l * lds_line_cx(N), 1};
-    const size_t ro = (size_t)(P >= 64 ? __builtin_amdgcn_readfirstlane(row) : row) * a.g.hp;
-    constexpr bool BUF = P >= 64;
-    const int lim = a.g.kxlim;  // entries from kxlim on are zero
-    using Row = HalfRow<T, BUF>;
-    auto W = [&](int q) { return Row(a.w[q], ro, lim); };
-    // one load phase: each of the nine input half rows is read once and feeds
-    // the packed transforms (u1, u2), (u3, theta), (Lap u1, Lap u2),
-    // (Lap u3, Lap theta); theta = i k1 Y_1 + Y2_2 + Y3_3, Lap(u_c) = L_c (the
-    // column passes' full Laplacian), Lap(theta) = i k1 L_1 + D
-    cx<T> z[E], t[E], w[E], v[E];
+    // the dissipation moment, sum 2 theta u_i Lap(u_i) + |u|^2 Lap(theta), from
+    // u and theta above and two more x transforms: (Lap u1, Lap u2) and
+    // (Lap u3, Lap theta), with Lap(u_c) = L_c (the column passes' full
+    // Laplacian) and Lap(theta) = i k1 L_1 + D: rows not read before
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        __builtin_amdgcn_sched_barrier(0);
-        const int m = j + k * P;
-        const int h = m <= N / 2 ? m : N - m;
-        const T k1o = (h == N / 2) ? (T)0 : (T)h;
-        const cx<T> y1 = W(0)[h], y2 = W(1)[h], y3 = W(2)[h];
-        const cx<T> th = cadd(cscale(cmuli(y1), k1o), cadd(W(4)[h], W(8)[h]));
-        const cx<T> l1 = W(9)[h];
-        z[k] = pack_vals<T, N>(y1, y2, m);
-        t[k] = pack_vals<T, N>(y3, th, m);
-        w[k] = pack_vals<T, N>(l1, W(10)[h], m);
-        v[k] = pack_vals<T, N>(W(11)[h], cadd(cscale(cmuli(l1), k1o), W(12)[h]), m);
+        if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
+        w[k] = pack_rows<T, N>(W(9), W(10), j + k * P);
     }
     __builtin_amdgcn_sched_barrier(0);
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    line_fft<T, N, E, true, false>(t, j, Ls, a.tw);  // u3, theta
     line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u1), Lap(u2)
-    line_fft<T, N, E, true, false>(v, j, Ls, a.tw);  // Lap(u3), Lap(theta)
     double s_dis = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        s_dis += 2.0 * (double)z[k].x * (double)t[k].y * (double)w[k].x +
-                 2.0 * (double)z[k].y * (double)t[k].y * (double)w[k].y;
+        s_dis += 2.0 * (double)z[k].x * (double)th[k] * (double)w[k].x +
+                 2.0 * (double)z[k].y * (double)th[k] * (double)w[k].y;
+    auto L1 = Row(k2d::launder(a.w[9]), ro, lim);  // read again, from the L2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        const double u1 = z[k].x, u2 = z[k].y, uu3 = t[k].x;
+        if (k % (E / 2) == 0) __builtin_amdgcn_sched_barrier(0);
+        const int m = j + k * P;
+        const int h = m <= N / 2 ? m : N - m;
+        const T k1o = (h == N / 2) ? (T)0 : (T)h;
+        const cx<T> lth = cadd(cscale(cmuli(L1[h]), k1o), W(12)[h]);
+        w[k] = pack_vals<T, N>(W(11)[h], lth, m);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // Lap(u3), Lap(theta)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const double u1 = z[k].x, u2 = z[k].y, uu3 = u3[k];
         const double uu = u1 * u1 + u2 * u2 + uu3 * uu3;
-        s_dis += 2.0 * uu3 * (double)t[k].y * (double)v[k].x + uu * (double)v[k].y;
+        s_dis += 2.0 * uu3 * (double)th[k] * (double)w[k].x + uu * (double)w[k].y;
     }
     if (!valid) s_dis = 0;
-    double vals[1] = {s_dis};
-    k2d::vg_reduce<C::THREADS, 1>(vals, red, 3, a.part);
+    double vd[1] = {s_dis};
+    k2d::vg_reduce<C::THREADS, 1>(vd, red, 3, a.part);
 }
 
 // shell spectra, 3-D (spectra_compute3D :1018-1108): u_d = (u.k)/|k|,
@@ -1080,12 +1013,7 @@ static int vg3_n(const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
     using C = Vg3Cfg<T, N>;
     const int g = (nrows + C::L - 1) / C::L;
     if (g != nblk) return -1;
-    if constexpr (sizeof(T) == 8) {
-        hipLaunchKernelGGL((vg3_kernel<T, N, true>), dim3(g), dim3(C::THREADS), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((vg3_kernel<T, N, false>), dim3(g), dim3(C::THREADS), 0, s, a);
-        hipLaunchKernelGGL((vg3_dis_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
-    }
+    hipLaunchKernelGGL((vg3_kernel<T, N>), dim3(g), dim3(C::THREADS), 0, s, a);
     return 0;
 }
 
