@@ -625,6 +625,10 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
     const int tstride = 4 * gridDim.x;
     const int t1 = ntask;
     double tot[4] = {0, 0, 0, 0};
+    // the member's fields through buffer resources with 32-bit offsets
+    // (launch_spectra checks that a field fits 2^31 bytes)
+    const __amdgpu_buffer_rsrc_t rU0 = __builtin_amdgcn_make_buffer_rsrc((void*)a.U[0], (short)0, (int)kBufRange, kBufWord3);
+    const __amdgpu_buffer_rsrc_t rU1 = __builtin_amdgcn_make_buffer_rsrc((void*)a.U[1], (short)0, (int)kBufRange, kBufWord3);
     constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
     for (int t = blockIdx.x * 4 + wave; t < t1; t += tstride * TB) {
         cx<T> u1[TB], u2[TB];
@@ -637,13 +641,12 @@ __global__ __launch_bounds__(256) void spectra_kernel(SpecArgs<T> a) {
             const int kx = (tq - r0 * nbx) * 64 + lane;
             r_[q] = r;
             kx_[q] = kx;
-            if (tq < t1 && kx < nh) {
-                u1[q] = a.U[0][(size_t)r * hp + kx];
-                u2[q] = a.U[1][(size_t)r * hp + kx];
-            } else {
-                u1[q] = {0, 0};
-                u2[q] = {0, 0};
-            }
+            // lanes past the summed modes read zero through an offset past the
+            // buffer's range: no branch around the loads, which then all issue
+            // before the first is waited for
+            const unsigned o = (tq < t1 && kx < nh) ? (unsigned)(((size_t)r * hp + kx) * sizeof(cx<T>)) : kBufRange;
+            u1[q] = buf_load<T>(rU0, o);
+            u2[q] = buf_load<T>(rU1, o);
         }
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
@@ -901,9 +904,10 @@ int launch_velgrad(int n, const VgArgs<T>& a, int nrows, hipStream_t s, int* nbl
 }
 template <typename T>
 int launch_spectra(const SpecArgs<T>& a, int nblk, hipStream_t s) {
-    {  // the kernel's task index is 32-bit
+    {  // the kernel's task index is 32-bit, its field offsets 32-bit bytes
         const long nh = a.nkx > 0 ? a.nkx : a.g.nh, nrow = a.nky > 0 ? a.nky : a.g.n1;
         if (nrow * ((nh + 63) / 64) + 16L * nblk >= INT_MAX) return -1;
+        if ((long)a.g.hp * a.g.n1 * (long)sizeof(cx<T>) >= (long)kBufRange) return -1;
     }
     const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
     hipLaunchKernelGGL((spectra_kernel<T>), dim3(nblk, 1, zdim(a.g.nmem)), dim3(256), shm, s, a);

@@ -753,6 +753,12 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
     const int ntask = ny * nz * nbx;  // < 2^31 (launch_spec3 checks)
     const int tstride = 4 * gridDim.x;
     double tot[4] = {0, 0, 0, 0};
+    // the fields through buffer resources with 32-bit offsets (launch_spec3
+    // checks that a field fits 2^31 bytes)
+    __amdgpu_buffer_rsrc_t rU[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        rU[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.U[c], (short)0, (int)k2d::kBufRange, k2d::kBufWord3);
     constexpr int TB = 4;  // tasks per wave iteration: all loads issued before the arithmetic
     for (int t = blockIdx.x * 4 + wave; t < ntask; t += tstride * TB) {
         cx<T> u[TB][3];
@@ -768,10 +774,13 @@ __global__ __launch_bounds__(256) void spec3_kernel(Spec3Args<T> a) {
             ky_[q] = ksigned(iy, n1);
             kz_[q] = ksigned(iz, n2);
             kx_[q] = kx;
+            // lanes past the summed modes read zero through an offset past the
+            // buffer's range: no branch around the loads, which then all issue
+            // before the first is waited for
             const bool ok = tq < ntask && kx < nh;
-            const size_t o = ok ? ((size_t)iz * n1 + iy) * hp + kx : 0;
+            const unsigned o = ok ? (unsigned)((((size_t)iz * n1 + iy) * hp + kx) * sizeof(cx<T>)) : k2d::kBufRange;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) u[q][c] = ok ? a.U[c][o] : cx<T>{(T)0, (T)0};
+            for (int c = 0; c < 3; ++c) u[q][c] = k2d::buf_load<T>(rU[c], o);
         }
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
@@ -1107,9 +1116,10 @@ int launch_vg3(int n, const Vg3Args<T>& a, int nrows, hipStream_t s, int nblk) {
 }
 template <typename T>
 int launch_spec3(const Spec3Args<T>& a, int nblk, hipStream_t s) {
-    {  // the kernel's task index is 32-bit
+    {  // the kernel's task index is 32-bit, its field offsets 32-bit bytes
         const long nh = a.nkx > 0 ? a.nkx : a.g.nh, ny = a.nky > 0 ? a.nky : a.g.n1, nz = a.nkz > 0 ? a.nkz : a.g.n2;
         if (ny * nz * ((nh + 63) / 64) + 4L * nblk * 4 >= INT_MAX) return -1;
+        if ((long)a.g.hp * a.g.n1 * a.g.n2 * (long)sizeof(cx<T>) >= (long)k2d::kBufRange) return -1;
     }
     const size_t shm = sizeof(double) * (4 * (size_t)(a.allkmax + 1) * 2 + 16);
     hipLaunchKernelGGL((spec3_kernel<T>), dim3(nblk), dim3(256), shm, s, a);
