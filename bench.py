@@ -759,6 +759,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="2d-2048: time the headline workload only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default radix plans (no autotuning)")
+    ap.add_argument("--plans", default=None, metavar="N:PxW,...",
+                    help="fix the column plans instead of tuning (e.g. 256:8x1,2048:16x1, applied to both "
+                         "precisions): the profiles' runs use the plans the tuned run picked")
     ap.add_argument("--serial", action="store_true", help="2d-ensemble96: step the size groups one after another")
     ap.add_argument("--closure-kernels", action="store_true",
                     help="2d-closure: time only the network's kernels at the workload's shapes")
@@ -777,6 +780,18 @@ def main():
         s.close()
         print(f"copy of {args.copy_only} MiB x 5: {g.value:.1f} GB/s (rc {rc})")
         sys.exit(0 if rc == 0 else 1)
+    if args.plans:
+        # a fixed plan table, as a tuned run left it: no tuning iterations
+        import bastr_amd
+        for item in args.plans.split(","):
+            n, pw = item.split(":")
+            pts, wid = pw.split("x")
+            for prec in (0, 1):
+                try:
+                    bastr_amd.tune_set(int(n), int(pts), prec, int(wid))
+                except Exception:
+                    pass  # no such plan in this precision: the default stays
+        args.no_tune = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole periodic domain is resident in one "
               "MI355X's HBM); --gpus must be 1", file=sys.stderr)
