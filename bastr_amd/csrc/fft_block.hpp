@@ -308,6 +308,117 @@ __device__ __forceinline__ void line_fft(cx<T>* v, int j, const LdsLine<T, SPLIT
 }
 
 // ---------------------------------------------------------------------------
+// Two lines in lockstep (one direction): each pass's twiddle factors are formed
+// once for both, and each exchange moves both lines through two T buffers of
+// one line's length (real parts, then imaginary parts), the LDS footprint of
+// one complex line, with both lines' loads and stores between two barriers.
+// The arithmetic per line is line_fft's, so the results are its bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int R, bool INV>
+__device__ __forceinline__ void twiddle2(cx<T>* v, cx<T>* u, int b, int Ns, const cx<T>* __restrict__ tw) {
+    if (Ns == 1) return;
+    const int bm = b & (Ns - 1);
+    const int step = bm * (N / (Ns * R));
+    cx<T> w1 = tw[step];
+    if (INV) w1.y = -w1.y;
+    cx<T> t = w1;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        if (r % 4 == 0) {
+            t = tw[step * r];
+            if (INV) t.y = -t.y;
+        } else if (r > 1) {
+            t = cmul(t, w1);
+        }
+        v[r] = cmul(v[r], t);
+        u[r] = cmul(u[r], t);
+    }
+}
+
+template <typename T, int N, int E, int NS, int LR, bool WAVE>
+__device__ __forceinline__ void exchange_pass2(const LdsLine<T, true, WAVE>& A, const LdsLine<T, true, WAVE>& B,
+                                               cx<T>* v, cx<T>* u, int b, int j) {
+    constexpr int P = N / E, S = 4;
+    constexpr bool LW = lin_write(P, NS, E, S), LRD = lin_read(N, P, E, LR, S);
+    auto rd = [](int q) { return LR == 1 ? q * P : (q / LR) * P + (q % LR) * (N / LR); };
+    T* const wa = A.base + lds_slot(b) * A.stride;
+    T* const wb = B.base + lds_slot(b) * B.stride;
+    T* const qa = A.base + lds_slot(j) * A.stride;
+    T* const qb = B.base + lds_slot(j) * B.stride;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const T xa = h ? v[r].y : v[r].x, xb = h ? u[r].y : u[r].x;
+            if constexpr (LW) {
+                wa[lin_off(r * NS, S) * A.stride] = xa;
+                wb[lin_off(r * NS, S) * B.stride] = xb;
+            } else {
+                A.at(b + r * NS) = xa;
+                B.at(b + r * NS) = xb;
+            }
+        }
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const T xa = LRD ? qa[lin_off(rd(q), S) * A.stride] : A.at(j + rd(q));
+            const T xb = LRD ? qb[lin_off(rd(q), S) * B.stride] : B.at(j + rd(q));
+            if (h) { v[q].y = xa; u[q].y = xb; }
+            else { v[q].x = xa; u[q].x = xb; }
+        }
+    }
+}
+
+template <typename T, int N, int E, bool INV, bool WAVE, int p>
+__device__ __forceinline__ void fft_passes2(cx<T>* v, cx<T>* u, int j, const LdsLine<T, true, WAVE>& A,
+                                            const LdsLine<T, true, WAVE>& B, const cx<T>* __restrict__ tw) {
+    using S = FftShape<N, E>;
+    constexpr int NS = ipow(E, p);
+    twiddle2<T, N, E, INV>(v, u, j, NS, tw);
+    dft<T, E, INV>(v);
+    dft<T, E, INV>(u);
+    if constexpr (p < S::NFULL - 1 || S::LAST > 1) {
+        const int b = (j / NS) * NS * E + (j & (NS - 1));
+        exchange_pass2<T, N, E, NS, (p < S::NFULL - 1 ? 1 : S::LAST)>(A, B, v, u, b, j);
+    }
+    if constexpr (p + 1 < S::NFULL) fft_passes2<T, N, E, INV, WAVE, p + 1>(v, u, j, A, B, tw);
+}
+
+// line_fft of v and of u, in lockstep through the split buffers A and B
+template <typename T, int N, int E, bool INV, bool WAVE>
+__device__ __forceinline__ void line_fft2(cx<T>* v, cx<T>* u, int j, const LdsLine<T, true, WAVE>& A,
+                                          const LdsLine<T, true, WAVE>& B, const cx<T>* __restrict__ tw) {
+    using S = FftShape<N, E>;
+    constexpr int P = S::P;
+    fft_passes2<T, N, E, INV, WAVE, 0>(v, u, j, A, B, tw);
+    if constexpr (S::LAST > 1) {
+        constexpr int Ns = ipow(E, S::NFULL);
+        constexpr int Lr = S::LAST;
+        constexpr int Q = E / Lr;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            twiddle2<T, N, Lr, INV>(v + q * Lr, u + q * Lr, j + q * P, Ns, tw);
+            dft<T, Lr, INV>(v + q * Lr);
+            dft<T, Lr, INV>(u + q * Lr);
+        }
+        cx<T> t[E], w[E];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int r = 0; r < Lr; ++r) {
+                t[q + r * Q] = v[q * Lr + r];
+                w[q + r * Q] = u[q * Lr + r];
+            }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            v[k] = t[k];
+            u[k] = w[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // wavefront reductions (64 lanes), deterministic butterfly order
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double x) {

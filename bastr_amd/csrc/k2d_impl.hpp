@@ -414,8 +414,12 @@ void row_kernel(RowArgs<T> a) {
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         zx[k] = cscale(cmuli(z[k]), kxs);
     }
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);   // (u1, u2)
-    line_fft<T, N, E, true, false>(zx, j, Ls, a.tw);  // (d1 u1, d1 u2)
+    {
+        // the two transforms in lockstep through the row's LDS as two split lines
+        T* const tl = (T*)(lds + l * lds_line_cx(N));
+        const LdsLine<T, true, (P <= 64)> La{tl, 1}, Lb{tl + lds_line(N), 1};
+        line_fft2<T, N, E, true, (P <= 64)>(z, zx, j, La, Lb, a.tw);  // (u1, u2), (d1 u1, d1 u2)
+    }
     cx<T> n[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) n[k] = {-(z[k].x * zx[k].x), -(z[k].x * zx[k].y)};
@@ -503,8 +507,12 @@ void row_stage_vg_kernel(VgArgs<T> a) {
         const T kxs = (m == N / 2) ? (T)0 : (T)ksigned(m, N);
         w[k] = cscale(cmuli(z[k]), kxs);
     }
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // d1u1, d1u2
+    {
+        // the two transforms in lockstep through the row's LDS as two split lines
+        T* const tl = (T*)(lds + l * lds_line_cx(N));
+        const LdsLine<T, true> La{tl, 1}, Lb{tl + lds_line(N), 1};
+        line_fft2<T, N, E, true, false>(z, w, j, La, Lb, a.tw);  // (u1, u2), (d1u1, d1u2)
+    }
     // z and w stay live for the moments, so the nonlinear term is formed after
     // the third transform (not held across it)
     cx<T> y[E];
