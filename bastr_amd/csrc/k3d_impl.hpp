@@ -524,12 +524,16 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         z[k] = pack_rows<T, N>(Y1, Y2, j + k * P);
         if constexpr (KEEP) kp[j + k * P] = z[k];
     }
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // (u1, u2)
     {
         T n3[E];
 #pragma unroll
         for (int k = 0; k < E; ++k) w[k] = pack_a_ika<T, N>(Y3, j + k * P);
-        line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (u3, d1 u3)
+        {
+            // the two transforms in lockstep through the row's LDS as two split lines
+            T* const tl = (T*)(lds + l * lds_line_cx(N));
+            const LdsLine<T, true, (P <= 64)> La{tl, 1}, Lb{tl + lds_line(N), 1};
+            line_fft2<T, N, E, true, (P <= 64)>(z, w, j, La, Lb, a.tw);  // (u1, u2), (u3, d1 u3)
+        }
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             u3[k] = w[k].x;
@@ -634,14 +638,18 @@ void vg3_kernel(Vg3Args<T> a) {
     double s_um2 = 0, s_ijji = 0, s_u2t = 0, s_urms = 0, s_dudx = 0, s_eps = 0, dmax = 0;
 #pragma unroll
     for (int k = 0; k < E; ++k) z[k] = pack_rows<T, N>(W(0), W(1), j + k * P);
-    line_fft<T, N, E, true, false>(z, j, Ls, a.tw);  // u1, u2
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int m = j + k * P;
         const int h = m <= N / 2 ? m : N - m;
         w[k] = pack_vals<T, N>(W(2)[h], d1(W(0), h), m);
     }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // u3, d1 u1
+    {
+        // the two transforms in lockstep through the row's LDS as two split lines
+        T* const tl = (T*)(lds + l * lds_line_cx(N));
+        const LdsLine<T, true, (P <= 64)> La{tl, 1}, Lb{tl + lds_line(N), 1};
+        line_fft2<T, N, E, true, (P <= 64)>(z, w, j, La, Lb, a.tw);  // (u1, u2), (u3, d1 u1)
+    }
 #pragma unroll
     for (int k = 0; k < E; ++k) { u3[k] = w[k].x; b11[k] = w[k].y; }
     // the symmetric pairs b_ij, b_ji: i j = 1 2, 1 3, 2 3
