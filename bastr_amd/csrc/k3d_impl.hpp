@@ -563,17 +563,21 @@ __global__ __launch_bounds__((RowCfg<T, N>::THREADS)) void row3_kernel(Row3Args<
         const cx<T> zs = KEEP ? kp[m] : pack_rows<T, N>(Y1, Y2, m);
         w[k] = cscale(cmuli(zs), kxs);
     }
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d1 u1, d1 u2)
-#pragma unroll
-    for (int k = 0; k < E; ++k) n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) w[k] = pack_rows<T, N>(HalfRow<T, BUF>(a.hin[3], ro, lim), HalfRow<T, BUF>(a.hin[4], ro, lim), j + k * P);
-    line_fft<T, N, E, true, false>(w, j, Ls, a.tw);  // (d2 u1, d2 u2)
+    for (int k = 0; k < E; ++k) n12[k] = pack_rows<T, N>(HalfRow<T, BUF>(a.hin[3], ro, lim), HalfRow<T, BUF>(a.hin[4], ro, lim), j + k * P);
+    {
+        // (d1 u1, d1 u2) and (d2 u1, d2 u2) in lockstep; n12 holds the second
+        T* const tl = (T*)(lds + l * lds_line_cx(N));
+        const LdsLine<T, true, (P <= 64)> La{tl, 1}, Lb{tl + lds_line(N), 1};
+        line_fft2<T, N, E, true, (P <= 64)>(w, n12, j, La, Lb, a.tw);
+    }
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-        n12[k].x = n12[k].x - z[k].y * w[k].x;
-        n12[k].y = n12[k].y - z[k].y * w[k].y;
+        const cx<T> d2 = n12[k];
+        n12[k] = {-(z[k].x * w[k].x), -(z[k].x * w[k].y)};
+        n12[k].x = n12[k].x - z[k].y * d2.x;
+        n12[k].y = n12[k].y - z[k].y * d2.y;
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
