@@ -19,7 +19,9 @@ namespace k2d {
 
 // Column-pass launch shape for (T, N): CB adjacent columns per workgroup,
 // E points per thread (EMAX from N = 256 up, else 8), P = N/E threads per
-// column, about 256 threads per workgroup.  CB*sizeof(complex) >= 64 B in
+// column, about 256 threads per workgroup (64 for columns of at most 256
+// points: a small grid's pass then spreads over four times the CUs; measured at
+// 256^2 FP64, 0.0736 -> 0.0713 ms per iteration).  CB*sizeof(complex) >= 64 B in
 // FP64 up to 2048-point columns, 128 B in FP32 up to 1024 (measured: FP32
 // columns of 64-byte rows fetched 1.5x their bytes), 32 B otherwise, keeps
 // each row access to whole sectors (the neighbouring column blocks, run on
@@ -37,7 +39,8 @@ struct ColCfg {
     static constexpr int P = N / E;
     static constexpr int CBMIN =
         (sizeof(T) == 8 && N <= 2048 ? 64 : (sizeof(T) == 4 && N <= 1024 ? 128 : 32)) / (int)sizeof(cx<T>);
-    static constexpr int CBW = (256 / P) > CBMIN ? (256 / P) : CBMIN;
+    static constexpr int TW = (N <= 256 && P < 64) ? 64 : 256;  // threads a block aims at
+    static constexpr int CBW = (TW / P) > CBMIN ? (TW / P) : CBMIN;
     static constexpr int CB = (CBW > 16 ? 16 : CBW) * CBM;
     static constexpr int THREADS = CB * P;
     static constexpr int LDS_BYTES = CB * (N + N / 16) * (int)sizeof(T);
@@ -50,11 +53,15 @@ struct ColCfg {
 // Row-pass launch shape for (T, N0): L rows per workgroup, 8 points per
 // thread (measured: FP32 lines of 1024-2048 points at 16 points per thread,
 // two LDS exchanges instead of three, ran 1.3x slower in the stage pass).
+// About 256 threads per workgroup, but one wavefront where a row takes fewer
+// than 64 threads (rows of at most 256 points): a small grid's pass then
+// spreads over four times the CUs (256^2 FP64: row stage 10.5 -> 9.0 us,
+// velocity gradients 17.3 -> 13.3 us; the 96-member ensemble unchanged).
 template <typename T, int N>
 struct RowCfg {
     static constexpr int E = 8;
     static constexpr int P = N / E;
-    static constexpr int L0 = P >= 256 ? 1 : 256 / P;
+    static constexpr int L0 = P >= 256 ? 1 : (P < 64 ? 64 / P : 256 / P);
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
@@ -71,7 +78,7 @@ template <typename T, int N, int EE = 8>
 struct VgCfg {
     static constexpr int E = EE;
     static constexpr int P = N / E;
-    static constexpr int L0 = P >= 256 ? 1 : 256 / P;
+    static constexpr int L0 = P >= 256 ? 1 : (P < 64 ? 64 / P : 256 / P);
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
