@@ -207,3 +207,29 @@ def test_autotune_step_refuses_a_nan_run(hip):
     assert bastr_amd.tune_get(n, 0) == (8, 1)
     g.close()
     bastr_amd.tune_set(n, 16)
+
+
+def test_bastr_tune_env_tunes_at_the_first_step(hip, monkeypatch):
+    """BASTR_TUNE=1 at bastr_create: the first bastr_step tunes the handle's
+    column length on its own step and then steps exactly as a handle created
+    with the chosen plan already set (fields, diagnostics, nstep, time)."""
+    print("case BASTR_TUNE=1 first-step tuning 256^2", flush=True)
+    p = make_params(256, 256, forcemethod=2, lproject=True, kmax=128)
+    u = mixed_field((256, 256), seed=61)
+    bastr_amd.tune_set(256, 16, 0)
+    monkeypatch.setenv("BASTR_TUNE", "1")
+    a = Solver(hip, "bastr", p)
+    monkeypatch.delenv("BASTR_TUNE")
+    a.set_fields(*u)
+    ra, da, sa = a.step(5)
+    plan = bastr_amd.tune_get(256, 0)
+    print("BASTR_TUNE plan", plan, flush=True)
+    b = Solver(hip, "bastr", p)  # the plan table is process-wide: b steps with the chosen plan
+    b.set_fields(*u)
+    rb, db, sb = b.step(5)
+    assert ra == 0 and rb == 0
+    assert all(np.array_equal(x, y) for x, y in zip(a.get_fields(), b.get_fields()))
+    assert [(d.nstep, d.time, d.Esspe, d.Edspe) for d in da] == [(d.nstep, d.time, d.Esspe, d.Edspe) for d in db]
+    assert np.array_equal(sa, sb)
+    a.close()
+    b.close()
