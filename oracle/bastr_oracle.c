@@ -137,40 +137,61 @@ static void plannd_free(plannd* P) {
     for (int a = 0; a < 3; ++a) plan1d_free(&P->p[a]);
 }
 
-/* In-place unnormalised c2c over a Fortran-order (n0,n1,n2) array. */
+/* In-place unnormalised c2c over a Fortran-order (n0,n1,n2) array.  Lines
+ * along axes 1 and 2 are gathered CB at a time (CB adjacent axis-0 indices
+ * share each cache line of the gather), and every thread allocates its line
+ * buffers once; each line's transform is fft_line's, so the result does not
+ * depend on the blocking or the thread count. */
+#define CB 4
 static void fft_nd(const plannd* P, cplx* a, int sign) {
     const int n0 = P->n[0], n1 = P->n[1], n2 = P->n[2];
     const int nmax = n0 > n1 ? (n0 > n2 ? n0 : n2) : (n1 > n2 ? n1 : n2);
-    /* axis 0: contiguous lines */
-    OMP_FOR
-    for (long line = 0; line < (long)n1 * n2; ++line) {
-        cplx* tmp = (cplx*)malloc(sizeof(cplx) * (size_t)nmax);
-        fft_line(a + (size_t)line * n0, &P->p[0], sign, tmp);
-        free(tmp);
-    }
-    /* axis 1 */
-    OMP_FOR
-    for (long line = 0; line < (long)n0 * n2; ++line) {
-        const long i = line % n0, k = line / n0;
-        cplx* buf = (cplx*)malloc(sizeof(cplx) * (size_t)nmax * 2);
-        cplx* tmp = buf + nmax;
-        for (int j = 0; j < n1; ++j) buf[j] = a[i + (size_t)n0 * (j + (size_t)n1 * k)];
-        fft_line(buf, &P->p[1], sign, tmp);
-        for (int j = 0; j < n1; ++j) a[i + (size_t)n0 * (j + (size_t)n1 * k)] = buf[j];
+    _Pragma("omp parallel num_threads(bo_get_threads())")
+    {
+        cplx* buf = (cplx*)malloc(sizeof(cplx) * (size_t)nmax * (CB + 1));
+        cplx* tmp = buf + (size_t)nmax * CB;
+        /* axis 0: contiguous lines */
+        _Pragma("omp for schedule(static)")
+        for (long line = 0; line < (long)n1 * n2; ++line)
+            fft_line(a + (size_t)line * n0, &P->p[0], sign, tmp);
+        /* axis 1: blocks of CB lines at consecutive i */
+        const long nb0 = (n0 + CB - 1) / CB;
+        _Pragma("omp for schedule(static)")
+        for (long blk = 0; blk < nb0 * n2; ++blk) {
+            const long i0 = (blk % nb0) * CB, k = blk / nb0;
+            const int nb = (int)(n0 - i0 < CB ? n0 - i0 : CB);
+            for (int j = 0; j < n1; ++j) {
+                const cplx* src = a + i0 + (size_t)n0 * (j + (size_t)n1 * k);
+                for (int b = 0; b < nb; ++b) buf[(size_t)b * nmax + j] = src[b];
+            }
+            for (int b = 0; b < nb; ++b) fft_line(buf + (size_t)b * nmax, &P->p[1], sign, tmp);
+            for (int j = 0; j < n1; ++j) {
+                cplx* dst = a + i0 + (size_t)n0 * (j + (size_t)n1 * k);
+                for (int b = 0; b < nb; ++b) dst[b] = buf[(size_t)b * nmax + j];
+            }
+        }
+        if (n2 > 1) {
+            /* axis 2: blocks of CB lines at consecutive (i, j) offsets */
+            const long nl = (long)n0 * n1, nb2 = (nl + CB - 1) / CB;
+            _Pragma("omp for schedule(static)")
+            for (long blk = 0; blk < nb2; ++blk) {
+                const long l0 = blk * CB;
+                const int nb = (int)(nl - l0 < CB ? nl - l0 : CB);
+                for (int k = 0; k < n2; ++k) {
+                    const cplx* src = a + (size_t)l0 + (size_t)nl * k;
+                    for (int b = 0; b < nb; ++b) buf[(size_t)b * nmax + k] = src[b];
+                }
+                for (int b = 0; b < nb; ++b) fft_line(buf + (size_t)b * nmax, &P->p[2], sign, tmp);
+                for (int k = 0; k < n2; ++k) {
+                    cplx* dst = a + (size_t)l0 + (size_t)nl * k;
+                    for (int b = 0; b < nb; ++b) dst[b] = buf[(size_t)b * nmax + k];
+                }
+            }
+        }
         free(buf);
     }
-    if (n2 > 1) {
-        OMP_FOR
-        for (long line = 0; line < (long)n0 * n1; ++line) {
-            cplx* buf = (cplx*)malloc(sizeof(cplx) * (size_t)nmax * 2);
-            cplx* tmp = buf + nmax;
-            for (int k = 0; k < n2; ++k) buf[k] = a[(size_t)line + (size_t)n0 * n1 * k];
-            fft_line(buf, &P->p[2], sign, tmp);
-            for (int k = 0; k < n2; ++k) a[(size_t)line + (size_t)n0 * n1 * k] = buf[k];
-            free(buf);
-        }
-    }
 }
+#undef CB
 
 int bo_fft_c2c(int n0, int n1, int n2, double* inout, int dir) {
     if (n0 <= 0 || n1 <= 0 || n2 < 0) return fail(BASTR_EINVAL, "bo_fft_c2c: bad extents");
