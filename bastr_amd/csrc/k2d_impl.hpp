@@ -469,10 +469,10 @@ __device__ __forceinline__ void vg_reduce(double* vals, double (*red)[8], int q0
 // transforms: (Lap u1, Lap u2), whose y-inverses the column pass formed with
 // the full multiplier -(k1^2 + k2^2), and Lap(theta) = i k1 Lap u1 + d2 Lap u2.
 // Occupancy asked of the compiler: 3 waves per SIMD (168 registers in FP64,
-// 128 in FP32; FP64 1024-point rows 2, which spill at 3).
+// 128 in FP32; FP64 1024- and 4096-point rows 2, which need more registers).
 template <typename T, int N>
 __global__ __launch_bounds__((VgCfg<T, N>::THREADS))
-__attribute__((amdgpu_waves_per_eu((sizeof(T) == 8 && N == 1024) ? 2 : 3)))
+__attribute__((amdgpu_waves_per_eu((sizeof(T) == 8 && (N == 1024 || N == 4096)) ? 2 : 3)))
 void row_stage_vg_kernel(VgArgs<T> a) {
     using C = VgCfg<T, N>;
     constexpr int E = C::E, P = C::P, LR = C::L;
