@@ -12,9 +12,15 @@ namespace k2d {
 // summation order: thread t adds partials t, t+256, ..., then a fixed LDS
 // tree.  Quantities: Es[1..K], Ed[1..K], 4 spectral totals, 7 velgrad sums,
 // then the eta_min minimum.
-__global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
+// BT threads per quantity: 256 with an LDS tree where a quantity has many
+// partials (3-D: one per row block), one wavefront with a shuffle butterfly
+// where it has few (at most 512: no barriers, as a small grid's reduction is
+// latency-bound; 256^2 FP64 0.0712 -> 0.0708 ms per iteration, while 2048
+// partials per quantity at 2048^2 were faster on 256 threads)
+template <int BT>
+__global__ __launch_bounds__(BT) void reduce_partials_kernel(FinArgs a) {
     if (a.gate.closed()) return;
-    __shared__ double sh[256];
+    __shared__ double sh[BT];
     const int K = a.allkmax;
     {  // ensemble member blockIdx.z: its partials and reduced sums
         const size_t m = blockIdx.z;
@@ -45,21 +51,26 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(FinArgs a) {
     // per row block: 65536 at 512^3)
     const double none = is_min ? (double)(2.0f * 3.14f) : 0.0;
     double acc = none;
-    for (int w0 = tid; w0 < n; w0 += 256 * 8) {
+    for (int w0 = tid; w0 < n; w0 += BT * 8) {
         double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = w0 + u * 256 < n ? src[w0 + u * 256] : none;
+        for (int u = 0; u < 8; ++u) v[u] = w0 + u * BT < n ? src[w0 + u * BT] : none;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (w0 + u * 256 < n) acc = is_min ? fmin(acc, v[u]) : acc + v[u];
+            if (w0 + u * BT < n) acc = is_min ? fmin(acc, v[u]) : acc + v[u];
     }
-    sh[tid] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) sh[tid] = is_min ? fmin(sh[tid], sh[tid + o]) : sh[tid] + sh[tid + o];
+    if constexpr (BT == 64) {
+        acc = is_min ? wave_min(acc) : wave_sum(acc);
+        if (tid == 0) a.red[q] = acc;
+    } else {
+        sh[tid] = acc;
         __syncthreads();
+        for (int o = BT / 2; o > 0; o >>= 1) {
+            if (tid < o) sh[tid] = is_min ? fmin(sh[tid], sh[tid + o]) : sh[tid] + sh[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) a.red[q] = sh[0];
     }
-    if (tid == 0) a.red[q] = sh[0];
 }
 
 // Diagnostics of the iteration from the reduced sums; NaN stop
@@ -108,7 +119,9 @@ __global__ void finalize_kernel(FinArgs a) {
 }
 
 int launch_finalize(const FinArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(2 * a.allkmax + 12, 1, zdim(a.nmem)), dim3(256), 0, s, a);
+    const dim3 grid(2 * a.allkmax + 12, 1, zdim(a.nmem));
+    if (a.nblk_spec <= 512 && a.nblk_vg <= 512) hipLaunchKernelGGL(reduce_partials_kernel<64>, grid, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(reduce_partials_kernel<256>, grid, dim3(256), 0, s, a);
     hipLaunchKernelGGL(finalize_kernel, dim3(1, 1, zdim(a.nmem)), dim3(256), 0, s, a);
     return 0;
 }
