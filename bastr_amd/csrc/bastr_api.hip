@@ -1479,8 +1479,14 @@ int autotune_step_impl(bastr_handle* h, int iters, double* ms, int* best_out) {
             if (ok && !rc && !nan_seen && (std::isnan(ms[v]) || per < ms[v])) ms[v] = per;
         }
     }
-    copy_state(false);
-    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    {
+        // put the state back whatever failed above (a run may have advanced it)
+        bool rok = true;
+        for (int c = 0; c < nd && snap[c]; ++c)
+            rok = rok && hipMemcpyAsync(h->U[c], snap[c], bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+        rok = rok && hipStreamSynchronize(h->stream) == hipSuccess;
+        ok = ok && rok;
+    }
     h->prof_mask = prof;
     for (int c = 0; c < nd; ++c)
         if (snap[c]) {
