@@ -1452,6 +1452,7 @@ int autotune_step_impl(bastr_handle* h, int iters, double* ms, int* best_out) {
                                 h->stream) == hipSuccess;
     };
     copy_state(true);
+    const bool saved = ok;  // the snapshots hold the state (restore only then)
     std::vector<int> st(h->nmem);
     bool nan_seen = false;
     for (int v = 0; v < 4; ++v) ms[v] = std::nan("");
@@ -1482,7 +1483,7 @@ int autotune_step_impl(bastr_handle* h, int iters, double* ms, int* best_out) {
     {
         // put the state back whatever failed above (a run may have advanced it)
         bool rok = true;
-        for (int c = 0; c < nd && snap[c]; ++c)
+        for (int c = 0; c < nd && saved; ++c)
             rok = rok && hipMemcpyAsync(h->U[c], snap[c], bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
         rok = rok && hipStreamSynchronize(h->stream) == hipSuccess;
         ok = ok && rok;
