@@ -92,11 +92,16 @@ __global__ void copy16_kernel(const double2* __restrict__ in, double2* __restric
     if (i < n) out[i] = in[i];
 }
 
+// Device allocation, zero-filled.  The fill runs on the null stream, which the
+// handles' non-blocking streams do not wait for: it is waited for here, or a
+// copy or kernel queued next on a handle's stream could run before it and be
+// overwritten with zeros (seen as a bastr_autotune_step snapshot that came
+// back zeroed now and then).
 void* dalloc(size_t bytes, bool* ok) {
     SyncLock lk(g_sync_mu);
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) { *ok = false; return nullptr; }
-    if (hipMemset(p, 0, bytes) != hipSuccess) { *ok = false; }
+    if (hipMemset(p, 0, bytes) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) { *ok = false; }
     return p;
 }
 
@@ -1274,7 +1279,7 @@ int time_plans(int n, int ncols, int nplanes, double* ms) {
         if (s) (void)hipStreamDestroy(s);
     };
     bool ok = hipMalloc(&in, sizeof(cx<T>) * elems) == hipSuccess && hipMalloc(&out, sizeof(cx<T>) * elems) == hipSuccess;
-    ok = ok && hipMemset(in, 0, sizeof(cx<T>) * elems) == hipSuccess;
+    ok = ok && hipMemset(in, 0, sizeof(cx<T>) * elems) == hipSuccess && hipStreamSynchronize(nullptr) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     if (ok) {
@@ -3242,6 +3247,7 @@ int bastr_closure_set_params(bastr_closure* net, const float* params, const floa
     HIPCHK(hipMemcpy(net->params, params, sizeof(float) * mlp::NPARAM, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(net->am, 0, sizeof(float) * mlp::NPARAM));
     HIPCHK(hipMemset(net->av, 0, sizeof(float) * mlp::NPARAM));
+    HIPCHK(hipStreamSynchronize(nullptr));  // the fills, before net->stream's next kernels
     net->t = 0;
     set_scales(net, scales);
     LAUNCHCHK(mlp::launch_pack(net->params, net->packed, net->stream));
