@@ -308,7 +308,11 @@ template <typename T, int N>
 struct Fwd2Cfg {
     using Base = ColCfg<T, N, sizeof(T) == 8 ? 8 : 16>;
     static constexpr int E = Base::E, P = Base::P;
-    static constexpr int CB = Base::CB * P > 512 ? (P >= 512 ? 1 : 512 / P) : Base::CB;
+    // FP64 columns of at most 256 points: two columns (one wavefront) per block,
+    // so a small grid's fused pass spreads over twice the CUs (its data sit in
+    // the caches, where 32-byte row segments cost little)
+    static constexpr bool NARROW = sizeof(T) == 8 && N <= 256 && N >= 64;
+    static constexpr int CB = NARROW ? 2 : (Base::CB * P > 512 ? (P >= 512 ? 1 : 512 / P) : Base::CB);
     static constexpr int THREADS = CB * P;
 };
 
@@ -1084,7 +1088,7 @@ static int cf2_n(int mode, const k2d::ColFwdArgs<T>& a, hipStream_t s) {
 template <typename T>
 bool cf2_full_rows(int n) {
     switch (n) {
-#define X(NN) case NN: return Fwd2Cfg<T, NN>::CB == Fwd2Cfg<T, NN>::Base::CB;
+#define X(NN) case NN: return Fwd2Cfg<T, NN>::CB == Fwd2Cfg<T, NN>::Base::CB || Fwd2Cfg<T, NN>::NARROW;
         COLX_SIZES(X)
 #undef X
     }
