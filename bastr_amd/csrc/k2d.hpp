@@ -54,14 +54,18 @@ struct ColCfg {
 // thread (measured: FP32 lines of 1024-2048 points at 16 points per thread,
 // two LDS exchanges instead of three, ran 1.3x slower in the stage pass).
 // About 256 threads per workgroup, but one wavefront where a row takes fewer
-// than 64 threads (rows of at most 256 points): a small grid's pass then
-// spreads over four times the CUs (256^2 FP64: row stage 10.5 -> 9.0 us,
-// velocity gradients 17.3 -> 13.3 us; the 96-member ensemble unchanged).
+// than 64 threads, and in FP64 rows of 64 to 256 points at 4 points per thread
+// (a 256-point row is one wavefront): a small grid's pass spreads over more CUs
+// and each thread's transform chain is half as long (256^2 FP64: row stage
+// 10.5 -> 9.0 us at one wavefront per block, iteration 0.0700 -> 0.0653 ms at 4
+// points per thread; in FP32 the 96-member ensemble measured 1.1 % slower at 4
+// points, so FP32 keeps 8).
 template <typename T, int N>
 struct RowCfg {
-    static constexpr int E = 8;
+    static constexpr bool SHORT = sizeof(T) == 8 && N <= 256 && N >= 64;
+    static constexpr int E = SHORT ? 4 : 8;
     static constexpr int P = N / E;
-    static constexpr int L0 = P >= 256 ? 1 : (P < 64 ? 64 / P : 256 / P);
+    static constexpr int L0 = P >= 256 ? 1 : (P < 64 || SHORT ? 64 / P : 256 / P);
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;
@@ -73,12 +77,16 @@ struct RowCfg {
 
 // Row-pass shape of the velocity-gradient kernels (8 points per thread, FP32
 // included: at 16 the stage-A kernel's four lines and seven FP64 moment sums
-// spill; the kernels order their transforms so that at most three lines are live).
-template <typename T, int N, int EE = 8>
+// spill; the kernels order their transforms so that at most three lines are
+// live).  EE = 0: the 2-D kernel's shape, in FP64 4 points per thread and one
+// wavefront per row block for rows of 64 to 256 points as in RowCfg; the 3-D
+// kernel asks for 8.
+template <typename T, int N, int EE = 0>
 struct VgCfg {
-    static constexpr int E = EE;
+    static constexpr bool SHORT = EE == 0 && sizeof(T) == 8 && N <= 256 && N >= 64;
+    static constexpr int E = EE ? EE : (SHORT ? 4 : 8);
     static constexpr int P = N / E;
-    static constexpr int L0 = P >= 256 ? 1 : (P < 64 ? 64 / P : 256 / P);
+    static constexpr int L0 = P >= 256 ? 1 : (P < 64 || SHORT ? 64 / P : 256 / P);
     static constexpr int LMAX = 65536 / (N * (int)sizeof(cx<T>));  // (+1/8 padding)
     static constexpr int L = L0 < LMAX ? L0 : (LMAX < 1 ? 1 : LMAX);
     static constexpr int THREADS = L * P;

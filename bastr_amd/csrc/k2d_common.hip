@@ -147,19 +147,20 @@ static int l_of(int n) {
 
 int col_block_width(int n, int prec) { return prec == BASTR_PREC_F32 ? cb_of<float>(n) : cb_of<double>(n); }
 int row_block_rows(int n, int prec) { return prec == BASTR_PREC_F32 ? l_of<float>(n) : l_of<double>(n); }
-template <typename T>
+template <typename T, int EE>
 static int lvg_of(int n) {
     switch (n) {
-#define X(NN) case NN: return VgCfg<T, NN>::L;
+#define X(NN) case NN: return VgCfg<T, NN, EE>::L;
         X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 #undef X
     }
     return -1;
 }
-// rows per workgroup of the velocity-gradient x pass (2-D VgCfg and 3-D Vg3Cfg alike)
+// rows per workgroup of the velocity-gradient x pass (2-D VgCfg, 3-D Vg3Cfg:
+// 8 points per thread)
 int vg_block_rows(int n, int prec, int nd) {
-    (void)nd;
-    return prec == BASTR_PREC_F32 ? lvg_of<float>(n) : lvg_of<double>(n);
+    if (nd == 3) return prec == BASTR_PREC_F32 ? lvg_of<float, 8>(n) : lvg_of<double, 8>(n);
+    return prec == BASTR_PREC_F32 ? lvg_of<float, 0>(n) : lvg_of<double, 0>(n);
 }
 
 }  // namespace k2d
