@@ -307,12 +307,13 @@ void cf3_kernel(Cf3Args<T> a) {
 template <typename T, int N>
 struct Fwd2Cfg {
     using Base = ColCfg<T, N, sizeof(T) == 8 ? 8 : 16>;
-    static constexpr int E = Base::E, P = Base::P;
-    // FP64 columns of at most 256 points: two columns (one wavefront) per block,
-    // so a small grid's fused pass spreads over twice the CUs (its data sit in
-    // the caches, where 32-byte row segments cost little)
+    // FP64 columns of 64 to 256 points: 4 points per thread and one wavefront
+    // per block, so a small grid's fused pass spreads over more CUs with
+    // shorter transform chains (its data sit in the caches, where short row
+    // segments cost little)
     static constexpr bool NARROW = sizeof(T) == 8 && N <= 256 && N >= 64;
-    static constexpr int CB = NARROW ? 2 : (Base::CB * P > 512 ? (P >= 512 ? 1 : 512 / P) : Base::CB);
+    static constexpr int E = NARROW ? 4 : Base::E, P = N / E;
+    static constexpr int CB = NARROW ? 64 / P : (Base::CB * P > 512 ? (P >= 512 ? 1 : 512 / P) : Base::CB);
     static constexpr int THREADS = CB * P;
 };
 
