@@ -126,10 +126,13 @@ int launch_finalize(const FinArgs& a, hipStream_t s) {
     return 0;
 }
 
+// Column-block width that pads the half-spectrum pitch: the default plan's,
+// or for FP64 columns of 64 to 256 points (4 points per thread, narrow blocks)
+// the 8-point plan's, so the pitch stays a multiple of every plan's width.
 template <typename T>
 static int cb_of(int n) {
     switch (n) {
-#define X(NN) case NN: return ColCfg<T, NN>::CB;
+#define X(NN) case NN: return ColCfg<T, NN>::SHORT && ColCfg<T, NN, 8>::CB > ColCfg<T, NN>::CB ? ColCfg<T, NN, 8>::CB : ColCfg<T, NN>::CB;
         X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 #undef X
     }
